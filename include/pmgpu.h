@@ -1,0 +1,203 @@
+/*
+ * pmgpu.h -- C ABI of the MI355X (gfx950) PM / P3M force step.
+ *
+ * This is the drop-in boundary: a host library that wants the GPU path of the
+ * reference's PMMethod / P3MMethod / Grid / FFTAdapter / leapfrog (the C++
+ * classes in include/pmsim/ are one such host, Python ctypes is another)
+ * binds exactly these symbols.  Plain pointers and sizes only; no C++ types
+ * and no exceptions cross it.
+ *
+ * Units: every quantity at this boundary is in the reference's code units
+ * (include/unitConversions.h:8-46 of the reference): positions in grid
+ * cells, velocities in cells per step, dt = 1, masses multiplied by
+ * 4*pi*G*DT^2/H^3, so that lap(phi) = rho and a = -grad(phi).
+ *
+ * Particle arrays handed in and out are host arrays in the caller's particle
+ * order ((x,y,z) interleaved, n*3 floats); the library keeps its own
+ * device-resident, cell-sorted copy and maps back on output.
+ *
+ * Threading: a context owns one HIP stream.  A context is not re-entrant;
+ * distinct contexts are independent.  Calls enqueue work on the context's
+ * stream and return; the getters synchronise.
+ *
+ * Every call returns PMGPU_OK (0) or an error code; pmgpu_last_error() gives
+ * the message.
+ */
+#ifndef PMGPU_H
+#define PMGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------- */
+#define PMGPU_OK 0
+#define PMGPU_ERR_INVALID_ARGUMENT 1 /* std::invalid_argument in the reference */
+#define PMGPU_ERR_NOT_READY 2        /* e.g. a step before particles / G are set */
+#define PMGPU_ERR_DEVICE 3           /* a HIP runtime error */
+#define PMGPU_ERR_OUT_OF_MEMORY 4
+#define PMGPU_ERR_UNSUPPORTED 5      /* valid in the reference, not on this path */
+
+/* ---- enums: same order as the reference's enum classes --------------- */
+/* InterpolationScheme, include/pmConfig.h:3 */
+#define PMGPU_NGP 0
+#define PMGPU_CIC 1
+#define PMGPU_TSC 2
+/* FiniteDiffScheme, include/pmConfig.h:4 */
+#define PMGPU_TWO_POINT 0
+#define PMGPU_FOUR_POINT 1
+/* GreensFunction, include/pmConfig.h:5 */
+#define PMGPU_DISCRETE_LAPLACIAN 0
+#define PMGPU_S1_OPTIMAL 1
+#define PMGPU_S2_OPTIMAL 2
+#define PMGPU_POOR_MAN 3
+/* CloudShape, include/greensFunctions.h:7 */
+#define PMGPU_S1 0
+#define PMGPU_S2 1
+/* external field kinds (the reference takes a std::function; the two
+   kinds its demos use are a zero field and sphRadDecrField,
+   source/externalFields.cpp:4-16) */
+#define PMGPU_EXT_NONE 0
+#define PMGPU_EXT_SPH_RAD_DECR 1
+
+typedef struct pmgpu_ctx pmgpu_ctx;
+
+/*
+ * Construction parameters of the mesh part.  Replaces the PMMethod
+ * constructor arguments (reference include/pmMethod.h:14-26) and the Grid
+ * constructor (include/grid.h:12): grid points, interpolation, finite
+ * difference, Green's function, particleDiameter (here already in code
+ * units, as PMMethod stores it, source/pmMethod.cpp:56) and the effective
+ * box (code units) used by escapedComputationalBox (pmMethod.cpp:146-156);
+ * a box component <= 0 disables that test on its axis (periodic use: the
+ * deposit and the interpolation always wrap indices periodically).
+ * Grid dimensions must be powers of two in [32, 1024].
+ */
+typedef struct pmgpu_pm_params {
+  int32_t grid[3];
+  int32_t interpolation;
+  int32_t finite_diff;
+  int32_t greens_function;
+  float particle_diameter;
+  float box[3];
+  int32_t ext_kind;      /* PMGPU_EXT_* */
+  float ext_params[6];   /* sphRadDecrField: center x,y,z, R, M, G (original units) */
+  float H, DT;           /* original-unit scales, for the external field only */
+} pmgpu_pm_params;
+
+/*
+ * P3M short-range parameters.  Replaces the P3MMethod constructor
+ * (reference include/p3mMethod.h:13-24, source/p3mMethod.cpp:19-48), all in
+ * code units: cutoff radius re, particle diameter a, softening length,
+ * cloud shape, useSRForceTable, and the computational box of the chaining
+ * mesh.  enableYSorting only changes the reference's summation order and has
+ * no counterpart here.
+ */
+typedef struct pmgpu_p3m_params {
+  float cutoff;
+  float particle_diameter;
+  float softening;
+  int32_t cloud_shape;
+  int32_t use_table;
+  float box[3];
+} pmgpu_p3m_params;
+
+/* ---- lifetime ------------------------------------------------------- */
+/* Replaces Grid(gridPoints, fftAdapter) + PMMethod(...) construction. */
+int pmgpu_create(const pmgpu_pm_params* params, pmgpu_ctx** out);
+void pmgpu_destroy(pmgpu_ctx* ctx);
+/* Message of the last failing call on ctx (ctx == NULL: of the last
+   pmgpu_create on this thread). */
+const char* pmgpu_last_error(const pmgpu_ctx* ctx);
+/* Blocks until all work enqueued on the context's stream is done. */
+int pmgpu_synchronize(pmgpu_ctx* ctx);
+
+/* ---- particles ------------------------------------------------------- */
+/* Uploads the particle state (replaces the PMMethod constructor's
+   state/masses, source/pmMethod.cpp:57-59).  vel may be NULL (zero). */
+int pmgpu_set_particles(pmgpu_ctx* ctx, int64_t n, const float* pos, const float* vel,
+                        const float* mass);
+/* Replaces PMMethod::getParticles (include/pmMethod.h:33): copies the
+   state back in the caller's particle order.  Any pointer may be NULL. */
+int pmgpu_get_particles(pmgpu_ctx* ctx, float* pos, float* vel, float* acc);
+/* Overwrites positions / velocities / accelerations (caller order) without
+   changing masses; NULL leaves a field as it is.  Lets a host that edited
+   its AoS mirror push it back (PMMethod::getParticles returns a mutable
+   reference in the reference). */
+int pmgpu_update_particles(pmgpu_ctx* ctx, const float* pos, const float* vel,
+                           const float* acc);
+int64_t pmgpu_num_particles(const pmgpu_ctx* ctx);
+
+/* ---- force step ------------------------------------------------------ */
+/* PMMethod::initGreensFunction (source/pmMethod.cpp:164-185), evaluated in
+   FP64 on the device; the stored influence function is the real,
+   inversion-symmetrised G with the inverse-FFT 1/N folded in. */
+int pmgpu_init_greens(pmgpu_ctx* ctx);
+/* PMMethod::pmMethodStep (source/pmMethod.cpp:137-144): deposit -> R2C FFT
+   -> x G(k) -> C2R FFT -> finite-difference field -> interpolation; writes
+   the acceleration (plus the external field). */
+int pmgpu_pm_step(pmgpu_ctx* ctx);
+/* Configures the P3M short-range pass (P3MMethod constructor). */
+int pmgpu_p3m_configure(pmgpu_ctx* ctx, const pmgpu_p3m_params* params);
+/* P3MMethod::calculateShortRangeForces + correctAccelerations
+   (source/p3mMethod.cpp:168-192, 50-57): adds the short-range correction
+   to the acceleration of every particle. */
+int pmgpu_p3m_short_range(pmgpu_ctx* ctx);
+
+/* ---- leapfrog (source/leapfrog.cpp:5-24, dt in code units) ----------- */
+#define PMGPU_HALF_KICK 0     /* setHalfStepVelocities: v += dt/2 a      */
+#define PMGPU_INTEGER_STEP 1  /* setIntegerStepVelocities: vi = v + dt/2 a */
+#define PMGPU_KICK 2          /* updateVelocities: v += dt a             */
+#define PMGPU_DRIFT 3         /* updatePositions: x += dt v              */
+int pmgpu_leapfrog(pmgpu_ctx* ctx, int32_t op, float dt);
+/* setIntegerStepVelocities result (caller order, n*3), for diagnostics. */
+int pmgpu_get_integer_step_velocities(pmgpu_ctx* ctx, float* vel);
+
+/*
+ * The arithmetic of PMMethod::run / P3MMethod::run (source/pmMethod.cpp:
+ * 72-122, source/p3mMethod.cpp:73-154) without I/O or diagnostics: when
+ * `initial` is nonzero, first the force and the half kick; then `iterations`
+ * times: drift, escape test, force (+ short range when `p3m`), kick.
+ * Kick and the next drift are fused on the device.  When a particle leaves
+ * the box the remaining work is skipped on the device (the reference breaks
+ * out of its loop); *iterations_done (may be NULL, then no sync) receives
+ * the count of completed iterations.
+ */
+int pmgpu_run(pmgpu_ctx* ctx, int32_t iterations, int32_t initial, int32_t p3m,
+              int32_t* iterations_done);
+/* escapedComputationalBox (pmMethod.cpp:151-156) of the current state. */
+int pmgpu_escaped(pmgpu_ctx* ctx, int32_t* escaped);
+
+/* ---- grids (Grid accessors, include/grid.h:18-44) -------------------- */
+#define PMGPU_GRID_DENSITY 0   /* Nx*Ny*Nz floats, x fastest */
+#define PMGPU_GRID_POTENTIAL 1 /* Nx*Ny*Nz floats */
+#define PMGPU_GRID_FIELD 2     /* Nx*Ny*Nz*3 floats, (Ex,Ey,Ez) per cell */
+#define PMGPU_GRID_GREENS 3    /* (Nx/2+1)*Ny*Nz floats: symmetrised G, times Nx*Ny*Nz */
+int pmgpu_get_grid(pmgpu_ctx* ctx, int32_t which, float* out);
+
+/* ---- learned influence-function correction --------------------------- */
+/* Multiplies G(k) by (1 + MLP(kx/kN, ky/kN, kz/kN, order)); weights is the
+   packed parameter vector of pmgpu_mlp_num_params() floats, NULL restores
+   the analytic G.  Takes effect at the next pmgpu_init_greens. */
+int pmgpu_set_greens_correction(pmgpu_ctx* ctx, const float* weights);
+int64_t pmgpu_mlp_num_params(void);
+
+/* ---- profiling -------------------------------------------------------- */
+/* When enabled, every kernel of the force step is bracketed by HIP events
+   on the context's stream; pmgpu_profile_read returns per-stage totals
+   (milliseconds) and launch counts.  Names are static strings. */
+int pmgpu_profile_enable(pmgpu_ctx* ctx, int32_t on);
+int pmgpu_profile_reset(pmgpu_ctx* ctx);
+int pmgpu_profile_read(pmgpu_ctx* ctx, int32_t max, const char** names, double* total_ms,
+                       int64_t* launches, int32_t* count);
+/* Ordered pairs with r < cutoff in the current state (not on the step path:
+   used to express the short-range kernel's rate in algorithmic FLOP/s). */
+int pmgpu_p3m_count_pairs(pmgpu_ctx* ctx, int64_t* pairs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PMGPU_H */

@@ -1,0 +1,309 @@
+"""Python front end of libpmgpu, the MI355X PM / P3M force step.
+
+A thin ctypes layer over the C ABI in include/pmgpu.h.  It mirrors the
+reference's PMMethod / P3MMethod surface (include/pmMethod.h:12-76,
+include/p3mMethod.h:11-63 of the reference) for tests and benchmarks:
+
+    pm = PM(grid=(64, 64, 64), interpolation=CIC, finite_diff=TWO_POINT,
+            greens_function=DISCRETE_LAPLACIAN, particle_diameter=0.0,
+            box=(64, 64, 64))
+    pm.set_particles(pos, vel, mass)     # code units, caller order
+    pm.init_greens()                     # PMMethod::initGreensFunction
+    pm.pm_step()                         # PMMethod::pmMethodStep
+    pos, vel, acc = pm.get_particles()
+
+The native library must be built (particlesimulation_amd.build); there is no
+fallback: loading fails loudly when libpmgpu.so is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpmgpu.so")
+
+# enums, same numbering as the reference's enum classes (include/pmConfig.h)
+NGP, CIC, TSC = 0, 1, 2
+TWO_POINT, FOUR_POINT = 0, 1
+DISCRETE_LAPLACIAN, S1_OPTIMAL, S2_OPTIMAL, POOR_MAN = 0, 1, 2, 3
+S1, S2 = 0, 1
+EXT_NONE, EXT_SPH_RAD_DECR = 0, 1
+HALF_KICK, INTEGER_STEP, KICK, DRIFT = 0, 1, 2, 3
+GRID_DENSITY, GRID_POTENTIAL, GRID_FIELD, GRID_GREENS = 0, 1, 2, 3
+
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_NOT_READY = 2
+ERR_DEVICE = 3
+ERR_OUT_OF_MEMORY = 4
+ERR_UNSUPPORTED = 5
+
+# The C ABI symbols declared in include/pmgpu.h.
+EXPORTS = (
+    "pmgpu_create", "pmgpu_destroy", "pmgpu_last_error", "pmgpu_synchronize",
+    "pmgpu_set_particles", "pmgpu_get_particles", "pmgpu_update_particles", "pmgpu_num_particles",
+    "pmgpu_init_greens", "pmgpu_pm_step", "pmgpu_p3m_configure", "pmgpu_p3m_short_range",
+    "pmgpu_leapfrog", "pmgpu_get_integer_step_velocities", "pmgpu_run", "pmgpu_escaped",
+    "pmgpu_get_grid", "pmgpu_set_greens_correction", "pmgpu_mlp_num_params",
+    "pmgpu_profile_enable", "pmgpu_profile_reset", "pmgpu_profile_read", "pmgpu_p3m_count_pairs",
+)
+
+
+class PMParams(C.Structure):
+    _fields_ = [
+        ("grid", C.c_int32 * 3),
+        ("interpolation", C.c_int32),
+        ("finite_diff", C.c_int32),
+        ("greens_function", C.c_int32),
+        ("particle_diameter", C.c_float),
+        ("box", C.c_float * 3),
+        ("ext_kind", C.c_int32),
+        ("ext_params", C.c_float * 6),
+        ("H", C.c_float),
+        ("DT", C.c_float),
+    ]
+
+
+class P3MParams(C.Structure):
+    _fields_ = [
+        ("cutoff", C.c_float),
+        ("particle_diameter", C.c_float),
+        ("softening", C.c_float),
+        ("cloud_shape", C.c_int32),
+        ("use_table", C.c_int32),
+        ("box", C.c_float * 3),
+    ]
+
+
+class PMGPUError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pmgpu error {code}: {msg}")
+        self.code = code
+
+
+class InvalidArgument(PMGPUError, ValueError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libpmgpu.so (raises when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python -m particlesimulation_amd.build`")
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    fp = C.POINTER(C.c_float)
+    L.pmgpu_create.argtypes = [C.POINTER(PMParams), C.POINTER(vp)]
+    L.pmgpu_destroy.argtypes = [vp]
+    L.pmgpu_destroy.restype = None
+    L.pmgpu_last_error.argtypes = [vp]
+    L.pmgpu_last_error.restype = C.c_char_p
+    L.pmgpu_synchronize.argtypes = [vp]
+    L.pmgpu_set_particles.argtypes = [vp, C.c_int64, fp, fp, fp]
+    L.pmgpu_get_particles.argtypes = [vp, fp, fp, fp]
+    L.pmgpu_update_particles.argtypes = [vp, fp, fp, fp]
+    L.pmgpu_num_particles.argtypes = [vp]
+    L.pmgpu_num_particles.restype = C.c_int64
+    L.pmgpu_init_greens.argtypes = [vp]
+    L.pmgpu_pm_step.argtypes = [vp]
+    L.pmgpu_p3m_configure.argtypes = [vp, C.POINTER(P3MParams)]
+    L.pmgpu_p3m_short_range.argtypes = [vp]
+    L.pmgpu_leapfrog.argtypes = [vp, C.c_int32, C.c_float]
+    L.pmgpu_get_integer_step_velocities.argtypes = [vp, fp]
+    L.pmgpu_run.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
+    L.pmgpu_escaped.argtypes = [vp, C.POINTER(C.c_int32)]
+    L.pmgpu_get_grid.argtypes = [vp, C.c_int32, fp]
+    L.pmgpu_set_greens_correction.argtypes = [vp, fp]
+    L.pmgpu_mlp_num_params.restype = C.c_int64
+    L.pmgpu_profile_enable.argtypes = [vp, C.c_int32]
+    L.pmgpu_profile_reset.argtypes = [vp]
+    L.pmgpu_profile_read.argtypes = [vp, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+    L.pmgpu_p3m_count_pairs.argtypes = [vp, C.POINTER(C.c_int64)]
+    _lib = L
+    return L
+
+
+def _fptr(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+class PM:
+    """Context of one simulation: PMMethod (+ P3MMethod) on the GPU."""
+
+    def __init__(self, grid, interpolation=CIC, finite_diff=TWO_POINT, greens_function=DISCRETE_LAPLACIAN,
+                 particle_diameter=0.0, box=None, ext_field=None, H=1.0, DT=1.0):
+        L = load_library()
+        p = PMParams()
+        p.grid[:] = [int(g) for g in grid]
+        p.interpolation = int(interpolation)
+        p.finite_diff = int(finite_diff)
+        p.greens_function = int(greens_function)
+        p.particle_diameter = float(particle_diameter)
+        box = grid if box is None else box
+        p.box[:] = [float(b) for b in box]
+        if ext_field is None:
+            p.ext_kind = EXT_NONE
+        else:
+            p.ext_kind = EXT_SPH_RAD_DECR
+            p.ext_params[:] = [float(v) for v in ext_field]
+        p.H = float(H)
+        p.DT = float(DT)
+        self.params = p
+        self.grid = tuple(int(g) for g in grid)
+        self._lib = L
+        self._ctx = C.c_void_p()
+        rc = L.pmgpu_create(C.byref(p), C.byref(self._ctx))
+        if rc != OK:
+            msg = L.pmgpu_last_error(None).decode()
+            raise (InvalidArgument if rc == ERR_INVALID_ARGUMENT else PMGPUError)(rc, msg)
+        self.n = 0
+
+    # -- plumbing ---------------------------------------------------------
+    def _check(self, rc):
+        if rc != OK:
+            msg = self._lib.pmgpu_last_error(self._ctx).decode()
+            raise (InvalidArgument if rc == ERR_INVALID_ARGUMENT else PMGPUError)(rc, msg)
+
+    def close(self):
+        if self._ctx:
+            self._lib.pmgpu_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- particles --------------------------------------------------------
+    def set_particles(self, pos, vel, mass):
+        pos = _f32(pos, (-1, 3))
+        mass = _f32(mass)
+        vel = None if vel is None else _f32(vel, (-1, 3))
+        self._keep = (pos, vel, mass)
+        self._check(self._lib.pmgpu_set_particles(self._ctx, len(mass), _fptr(pos), _fptr(vel), _fptr(mass)))
+        self.n = len(mass)
+
+    def get_particles(self):
+        pos = np.zeros((self.n, 3), np.float32)
+        vel = np.zeros((self.n, 3), np.float32)
+        acc = np.zeros((self.n, 3), np.float32)
+        self._check(self._lib.pmgpu_get_particles(self._ctx, _fptr(pos), _fptr(vel), _fptr(acc)))
+        return pos, vel, acc
+
+    def update_particles(self, pos=None, vel=None, acc=None):
+        a = [None if x is None else _f32(x, (-1, 3)) for x in (pos, vel, acc)]
+        self._check(self._lib.pmgpu_update_particles(self._ctx, *[_fptr(x) for x in a]))
+
+    # -- force step -------------------------------------------------------
+    def init_greens(self):
+        self._check(self._lib.pmgpu_init_greens(self._ctx))
+
+    def pm_step(self):
+        self._check(self._lib.pmgpu_pm_step(self._ctx))
+
+    def p3m_configure(self, cutoff, particle_diameter, softening=0.0, cloud_shape=S2, use_table=True, box=None):
+        q = P3MParams()
+        q.cutoff = float(cutoff)
+        q.particle_diameter = float(particle_diameter)
+        q.softening = float(softening)
+        q.cloud_shape = int(cloud_shape)
+        q.use_table = int(bool(use_table))
+        box = self.params.box[:] if box is None else box
+        q.box[:] = [float(b) for b in box]
+        self._check(self._lib.pmgpu_p3m_configure(self._ctx, C.byref(q)))
+
+    def p3m_short_range(self):
+        self._check(self._lib.pmgpu_p3m_short_range(self._ctx))
+
+    def p3m_count_pairs(self):
+        v = C.c_int64(0)
+        self._check(self._lib.pmgpu_p3m_count_pairs(self._ctx, C.byref(v)))
+        return v.value
+
+    def leapfrog(self, op, dt=1.0):
+        self._check(self._lib.pmgpu_leapfrog(self._ctx, int(op), float(dt)))
+
+    def integer_step_velocities(self):
+        v = np.zeros((self.n, 3), np.float32)
+        self._check(self._lib.pmgpu_get_integer_step_velocities(self._ctx, _fptr(v)))
+        return v
+
+    def run(self, iterations, initial=True, p3m=False, sync=True):
+        done = C.c_int32(0)
+        self._check(self._lib.pmgpu_run(self._ctx, int(iterations), int(bool(initial)), int(bool(p3m)),
+                                        C.byref(done) if sync else None))
+        return done.value if sync else None
+
+    def escaped(self):
+        v = C.c_int32(0)
+        self._check(self._lib.pmgpu_escaped(self._ctx, C.byref(v)))
+        return bool(v.value)
+
+    def synchronize(self):
+        self._check(self._lib.pmgpu_synchronize(self._ctx))
+
+    # -- grids ------------------------------------------------------------
+    def get_grid(self, which):
+        nx, ny, nz = self.grid
+        if which == GRID_FIELD:
+            out = np.zeros((nz, ny, nx, 3), np.float32)
+        elif which == GRID_GREENS:
+            out = np.zeros((nz, ny, nx // 2 + 1), np.float32)
+        else:
+            out = np.zeros((nz, ny, nx), np.float32)
+        self._check(self._lib.pmgpu_get_grid(self._ctx, int(which), _fptr(out)))
+        return out
+
+    def density(self):
+        return self.get_grid(GRID_DENSITY)
+
+    def potential(self):
+        return self.get_grid(GRID_POTENTIAL)
+
+    # -- learned correction ----------------------------------------------
+    def set_greens_correction(self, weights):
+        w = None if weights is None else _f32(weights)
+        self._check(self._lib.pmgpu_set_greens_correction(self._ctx, _fptr(w)))
+
+    # -- profiling --------------------------------------------------------
+    def profile(self, on=True):
+        self._check(self._lib.pmgpu_profile_enable(self._ctx, int(bool(on))))
+
+    def profile_reset(self):
+        self._check(self._lib.pmgpu_profile_reset(self._ctx))
+
+    def profile_read(self):
+        n = 64
+        names = (C.c_char_p * n)()
+        ms = (C.c_double * n)()
+        cnt = (C.c_int64 * n)()
+        k = C.c_int32(0)
+        self._check(self._lib.pmgpu_profile_read(self._ctx, n, names, ms, cnt, C.byref(k)))
+        return {names[i].decode(): (ms[i], cnt[i]) for i in range(k.value)}
+
+
+def mlp_num_params() -> int:
+    return int(load_library().pmgpu_mlp_num_params())

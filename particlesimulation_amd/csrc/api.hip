@@ -1,0 +1,615 @@
+// C ABI of libpmgpu (include/pmgpu.h): context lifetime, particle upload and
+// download, the force-step sequence and the leapfrog run loop.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pmgpu_internal.h"
+
+namespace pmgpu {
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+void check_hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    fail(PMGPU_ERR_OUT_OF_MEMORY, std::string(what) + ": " + hipGetErrorString(e));
+  fail(PMGPU_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+static void dalloc(T** p, int64_t count) {
+  if (*p) {
+    HIP_CHECK(hipFree(*p));
+    *p = nullptr;
+  }
+  if (count <= 0) count = 1;
+  HIP_CHECK(hipMalloc((void**)p, sizeof(T) * (size_t)count));
+  HIP_CHECK(hipMemset(*p, 0, sizeof(T) * (size_t)count));
+}
+
+template <typename T>
+static void dfree(T** p) {
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+}
+
+int stage_begin(pmgpu_ctx* c, const char* name) {
+  if (!c->profiling) return -1;
+  int s = -1;
+  for (size_t i = 0; i < c->stages.size(); ++i)
+    if (std::strcmp(c->stages[i].name, name) == 0) s = (int)i;
+  if (s < 0) {
+    c->stages.push_back(Stage{name});
+    s = (int)c->stages.size() - 1;
+  }
+  pmgpu_ctx::Pending p;
+  p.stage = s;
+  for (hipEvent_t* e : {&p.a, &p.b}) {
+    if (!c->ev_pool.empty()) {
+      *e = c->ev_pool.back();
+      c->ev_pool.pop_back();
+    } else {
+      HIP_CHECK(hipEventCreate(e));
+    }
+  }
+  HIP_CHECK(hipEventRecord(p.a, c->stream));
+  c->pending.push_back(p);
+  return (int)c->pending.size() - 1;
+}
+
+void stage_end(pmgpu_ctx* c, int token) {
+  if (token < 0) return;
+  HIP_CHECK(hipEventRecord(c->pending[token].b, c->stream));
+}
+
+static void profile_collect(pmgpu_ctx* c) {
+  if (c->pending.empty()) return;
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  for (auto& p : c->pending) {
+    float ms = 0.0f;
+    HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+    c->stages[p.stage].total_ms += ms;
+    c->stages[p.stage].launches += 1;
+    c->ev_pool.push_back(p.a);
+    c->ev_pool.push_back(p.b);
+  }
+  c->pending.clear();
+}
+
+static bool is_pow2_in(int v, int lo, int hi) { return v >= lo && v <= hi && (v & (v - 1)) == 0; }
+
+static void validate(const pmgpu_pm_params* p) {
+  if (!p) fail(PMGPU_ERR_INVALID_ARGUMENT, "params is NULL");
+  for (int i = 0; i < 3; ++i)
+    if (!is_pow2_in(p->grid[i], 32, 1024))
+      fail(PMGPU_ERR_UNSUPPORTED, "grid dimensions must be powers of two in [32, 1024]");
+  if (p->interpolation < 0 || p->interpolation > 2)
+    fail(PMGPU_ERR_INVALID_ARGUMENT, "Unkown interpolation scheme");
+  if (p->finite_diff < 0 || p->finite_diff > 1)
+    fail(PMGPU_ERR_INVALID_ARGUMENT, "Unknown finite difference type.");
+  if (p->greens_function < 0 || p->greens_function > 3)
+    fail(PMGPU_ERR_INVALID_ARGUMENT, "not implemented");
+  if ((p->greens_function == PMGPU_S1_OPTIMAL || p->greens_function == PMGPU_S2_OPTIMAL) &&
+      !(p->particle_diameter > 0.0f))
+    fail(PMGPU_ERR_INVALID_ARGUMENT, "optimal Green's function needs particle_diameter > 0");
+  if (p->ext_kind != PMGPU_EXT_NONE && p->ext_kind != PMGPU_EXT_SPH_RAD_DECR)
+    fail(PMGPU_ERR_INVALID_ARGUMENT, "unknown external field kind");
+  if (p->ext_kind == PMGPU_EXT_SPH_RAD_DECR && !(p->H > 0.0f && p->DT > 0.0f))
+    fail(PMGPU_ERR_INVALID_ARGUMENT, "external field needs H > 0 and DT > 0");
+}
+
+static void twiddles(float2** dst, int L, double denom_len) {
+  std::vector<float2> t(L);
+  for (int m = 0; m < L; ++m) {
+    double a = -2.0 * M_PI * (double)m / denom_len;
+    t[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  dalloc(dst, L);
+  HIP_CHECK(hipMemcpy(*dst, t.data(), sizeof(float2) * L, hipMemcpyHostToDevice));
+}
+
+static void ensure_scan_scratch(pmgpu_ctx* c, int64_t n) {
+  size_t need = scan_scratch_elems(n);
+  if (need > c->scan_tmp_elems) {
+    dalloc(&c->scan_tmp, (int64_t)need);
+    c->scan_tmp_elems = need;
+  }
+}
+
+static void alloc_particle_arrays(pmgpu_ctx* c, int64_t n) {
+  for (int b = 0; b < 2; ++b) {
+    dalloc(&c->pm[b], n);
+    dalloc(&c->vel[b], n);
+  }
+  dalloc(&c->acc, n);
+  dalloc(&c->key, n);
+  dalloc(&c->rank, n);
+  dfree(&c->vint);
+  c->fx_slots = 8 * n / DEPOSIT_CHUNK + 2;
+  dalloc(&c->fx_accum, c->fx_slots * TILE_CELLS);
+  dalloc(&c->slot_tile, c->fx_slots);
+  if (c->p3m_ready) {
+    dalloc(&c->sr_key, n);
+    dalloc(&c->sr_rank, n);
+    dalloc(&c->sr_pm, n);
+    dalloc(&c->sr_idx, n);
+  }
+}
+
+static void set_shift(pmgpu_ctx* c) {
+  // every cell sum of the deposit is bounded by 1.5 * sum|m| (the largest
+  // weight product of any scheme is 1.125^3 < 1.5); keep it below 2^62
+  if (c->mass_abs_sum > 0.0) {
+    int s = (int)std::floor(62.0 - std::log2(1.5 * c->mass_abs_sum)) - 1;
+    c->fx_shift = std::max(-120, std::min(120, s));
+  } else {
+    c->fx_shift = 0;
+  }
+}
+
+static void reset_escape(pmgpu_ctx* c) {
+  int32_t v = INT_MAX;
+  HIP_CHECK(hipMemcpyAsync(c->escape, &v, sizeof(v), hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+static void require_particles(pmgpu_ctx* c) {
+  if (c->n <= 0) fail(PMGPU_ERR_NOT_READY, "no particles: call pmgpu_set_particles first");
+}
+
+// The force step of PMMethod::pmMethodStep on the current state.
+static void pm_force(pmgpu_ctx* c, int interp_mode, float kick) {
+  tile_sort(c);
+  deposit(c);
+  fft_forward_green_inverse(c);
+  interpolate(c, interp_mode, kick, c->iter);
+}
+
+}  // namespace pmgpu
+
+using namespace pmgpu;
+
+static thread_local std::string g_create_error;
+
+#define PMGPU_API_BEGIN(ctx) \
+  if (!(ctx)) return PMGPU_ERR_INVALID_ARGUMENT; \
+  try {
+#define PMGPU_API_END(ctx)        \
+  }                               \
+  catch (const Error& e) {        \
+    (ctx)->err = e.msg;           \
+    return e.code;                \
+  }                               \
+  catch (const std::exception& e) { \
+    (ctx)->err = e.what();        \
+    return PMGPU_ERR_DEVICE;      \
+  }                               \
+  return PMGPU_OK;
+
+extern "C" {
+
+int pmgpu_create(const pmgpu_pm_params* params, pmgpu_ctx** out) {
+  if (!out) return PMGPU_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  pmgpu_ctx* c = nullptr;
+  try {
+    validate(params);
+    c = new pmgpu_ctx();
+    c->prm = *params;
+    c->nx = params->grid[0];
+    c->ny = params->grid[1];
+    c->nz = params->grid[2];
+    c->nxh = c->nx / 2 + 1;
+    c->M = (int64_t)c->nx * c->ny * c->nz;
+    c->K = (int64_t)c->nxh * c->ny * c->nz;
+    c->ntx = c->nx / TILE_X;
+    c->nty = c->ny / TILE_Y;
+    c->ntz = c->nz / TILE_Z;
+    c->ntiles = c->ntx * c->nty * c->ntz;
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    HIP_CHECK(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    dalloc(&c->rho, c->M);
+    dalloc(&c->phi, c->M);
+    dalloc(&c->spec, c->K);
+    dalloc(&c->greens, c->K);
+    twiddles(&c->tw_x, c->nx / 2, c->nx / 2);
+    twiddles(&c->tw_y, c->ny, c->ny);
+    twiddles(&c->tw_z, c->nz, c->nz);
+    twiddles(&c->tw_r2c, c->nx / 2 + 1, c->nx);
+    const int64_t nkeys = (int64_t)c->ntiles * NCLASS;
+    dalloc(&c->cls_count, nkeys + 1);
+    dalloc(&c->cls_start, nkeys + 1);
+    dalloc(&c->tile_nchunk, c->ntiles + 1);
+    dalloc(&c->tile_chunk_start, c->ntiles + 1);
+    dalloc(&c->tile_slot, c->ntiles + 1);
+    dalloc(&c->int_nchunk, c->ntiles + 1);
+    dalloc(&c->int_chunk_start, c->ntiles + 1);
+    dalloc(&c->counters, 16);
+    dalloc(&c->escape, 1);
+    ensure_scan_scratch(c, nkeys);
+    std::vector<int4> tab = deposit_range_table(c->prm.interpolation);
+    c->dep_table_len = (int)tab.size();
+    dalloc(&c->dep_table, std::max<int64_t>(1, (int64_t)tab.size()));
+    if (!tab.empty())
+      HIP_CHECK(hipMemcpy(c->dep_table, tab.data(), sizeof(int4) * tab.size(), hipMemcpyHostToDevice));
+    reset_escape(c);
+  } catch (const Error& e) {
+    g_create_error = e.msg;
+    if (c) pmgpu_destroy(c);
+    return e.code;
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    if (c) pmgpu_destroy(c);
+    return PMGPU_ERR_DEVICE;
+  }
+  *out = c;
+  return PMGPU_OK;
+}
+
+void pmgpu_destroy(pmgpu_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (int b = 0; b < 2; ++b) {
+    dfree(&c->pm[b]);
+    dfree(&c->vel[b]);
+  }
+  dfree(&c->acc); dfree(&c->vint); dfree(&c->key); dfree(&c->rank);
+  dfree(&c->cls_count); dfree(&c->cls_start); dfree(&c->tile_nchunk); dfree(&c->tile_chunk_start);
+  dfree(&c->tile_slot); dfree(&c->slot_tile); dfree(&c->int_nchunk); dfree(&c->int_chunk_start);
+  dfree(&c->dep_table); dfree(&c->counters); dfree(&c->fx_accum); dfree(&c->scan_tmp);
+  dfree(&c->rho); dfree(&c->spec); dfree(&c->phi); dfree(&c->greens); dfree(&c->field);
+  dfree(&c->tw_x); dfree(&c->tw_y); dfree(&c->tw_z); dfree(&c->tw_r2c);
+  dfree(&c->sr_table); dfree(&c->sr_count); dfree(&c->sr_start); dfree(&c->sr_key);
+  dfree(&c->sr_rank); dfree(&c->sr_pm); dfree(&c->sr_idx); dfree(&c->sr_nchunk);
+  dfree(&c->sr_chunk_start); dfree(&c->pair_count); dfree(&c->escape);
+  for (auto& p : c->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* pmgpu_last_error(const pmgpu_ctx* c) {
+  return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+int pmgpu_synchronize(pmgpu_ctx* c) {
+  PMGPU_API_BEGIN(c)
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  PMGPU_API_END(c)
+}
+
+int64_t pmgpu_num_particles(const pmgpu_ctx* c) { return c ? c->n : -1; }
+
+int pmgpu_set_particles(pmgpu_ctx* c, int64_t n, const float* pos, const float* vel,
+                        const float* mass) {
+  PMGPU_API_BEGIN(c)
+  if (n < 0 || (n > 0 && (!pos || !mass)))
+    fail(PMGPU_ERR_INVALID_ARGUMENT, "pos and mass are required");
+  if (n > (int64_t)INT_MAX / 2) fail(PMGPU_ERR_UNSUPPORTED, "too many particles");
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  alloc_particle_arrays(c, n);
+  c->n = n;
+  std::vector<float4> hp(n), hv(n);
+  double s = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    hp[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], mass[i]);
+    float vx = vel ? vel[3 * i] : 0.f, vy = vel ? vel[3 * i + 1] : 0.f, vz = vel ? vel[3 * i + 2] : 0.f;
+    int32_t id = (int32_t)i;
+    float w;
+    std::memcpy(&w, &id, sizeof(w));
+    hv[i] = make_float4(vx, vy, vz, w);
+    s += std::fabs((double)mass[i]);
+  }
+  c->cur = 0;
+  if (n > 0) {
+    HIP_CHECK(hipMemcpy(c->pm[0], hp.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(c->vel[0], hv.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+  }
+  c->mass_abs_sum = s;
+  set_shift(c);
+  c->sorted_valid = false;
+  c->run_iter = 0;
+  reset_escape(c);
+  PMGPU_API_END(c)
+}
+
+int pmgpu_get_particles(pmgpu_ctx* c, float* pos, float* vel, float* acc) {
+  PMGPU_API_BEGIN(c)
+  const int64_t n = c->n;
+  if (n == 0) return PMGPU_OK;
+  std::vector<float4> hp(n), hv(n), ha;
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  HIP_CHECK(hipMemcpy(hv.data(), c->vel[c->cur], sizeof(float4) * n, hipMemcpyDeviceToHost));
+  if (pos) HIP_CHECK(hipMemcpy(hp.data(), c->pm[c->cur], sizeof(float4) * n, hipMemcpyDeviceToHost));
+  if (acc) {
+    ha.resize(n);
+    HIP_CHECK(hipMemcpy(ha.data(), c->acc, sizeof(float4) * n, hipMemcpyDeviceToHost));
+  }
+  for (int64_t s = 0; s < n; ++s) {
+    int32_t id;
+    std::memcpy(&id, &hv[s].w, sizeof(id));
+    if (id < 0 || id >= n) fail(PMGPU_ERR_DEVICE, "corrupted particle index");
+    if (pos) {
+      pos[3 * id] = hp[s].x;
+      pos[3 * id + 1] = hp[s].y;
+      pos[3 * id + 2] = hp[s].z;
+    }
+    if (vel) {
+      vel[3 * id] = hv[s].x;
+      vel[3 * id + 1] = hv[s].y;
+      vel[3 * id + 2] = hv[s].z;
+    }
+    if (acc) {
+      acc[3 * id] = ha[s].x;
+      acc[3 * id + 1] = ha[s].y;
+      acc[3 * id + 2] = ha[s].z;
+    }
+  }
+  PMGPU_API_END(c)
+}
+
+int pmgpu_update_particles(pmgpu_ctx* c, const float* pos, const float* vel, const float* acc) {
+  PMGPU_API_BEGIN(c)
+  const int64_t n = c->n;
+  if (n == 0) return PMGPU_OK;
+  std::vector<float4> hp(n), hv(n), ha(n);
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  HIP_CHECK(hipMemcpy(hv.data(), c->vel[c->cur], sizeof(float4) * n, hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(hp.data(), c->pm[c->cur], sizeof(float4) * n, hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(ha.data(), c->acc, sizeof(float4) * n, hipMemcpyDeviceToHost));
+  for (int64_t s = 0; s < n; ++s) {
+    int32_t id;
+    std::memcpy(&id, &hv[s].w, sizeof(id));
+    if (pos) {
+      hp[s].x = pos[3 * id];
+      hp[s].y = pos[3 * id + 1];
+      hp[s].z = pos[3 * id + 2];
+    }
+    if (vel) {
+      hv[s].x = vel[3 * id];
+      hv[s].y = vel[3 * id + 1];
+      hv[s].z = vel[3 * id + 2];
+    }
+    if (acc) ha[s] = make_float4(acc[3 * id], acc[3 * id + 1], acc[3 * id + 2], 0.f);
+  }
+  HIP_CHECK(hipMemcpy(c->pm[c->cur], hp.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(c->vel[c->cur], hv.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(c->acc, ha.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+  if (pos) c->sorted_valid = false;
+  c->run_iter = 0;
+  reset_escape(c);
+  PMGPU_API_END(c)
+}
+
+int pmgpu_init_greens(pmgpu_ctx* c) {
+  PMGPU_API_BEGIN(c)
+  float* corr = nullptr;
+  if (!c->mlp_weights.empty()) corr = mlp_correction_grid(c);
+  greens_compute(c, corr);
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (corr) HIP_CHECK(hipFree(corr));
+  c->greens_ready = true;
+  PMGPU_API_END(c)
+}
+
+int pmgpu_pm_step(pmgpu_ctx* c) {
+  PMGPU_API_BEGIN(c)
+  require_particles(c);
+  if (!c->greens_ready) fail(PMGPU_ERR_NOT_READY, "call pmgpu_init_greens first");
+  c->iter = INT_MIN;
+  pm_force(c, MODE_ACC, 0.0f);
+  PMGPU_API_END(c)
+}
+
+int pmgpu_p3m_configure(pmgpu_ctx* c, const pmgpu_p3m_params* p) {
+  PMGPU_API_BEGIN(c)
+  if (!p) fail(PMGPU_ERR_INVALID_ARGUMENT, "params is NULL");
+  if (!(p->cutoff > 0.0f)) fail(PMGPU_ERR_INVALID_ARGUMENT, "cutoff must be > 0");
+  if (p->cloud_shape != PMGPU_S1 && p->cloud_shape != PMGPU_S2)
+    fail(PMGPU_ERR_INVALID_ARGUMENT, "not implemented");
+  if (!(p->particle_diameter > 0.0f)) fail(PMGPU_ERR_INVALID_ARGUMENT, "particle_diameter must be > 0");
+  for (int i = 0; i < 3; ++i)
+    if (!(p->box[i] > 0.0f)) fail(PMGPU_ERR_INVALID_ARGUMENT, "box must be > 0");
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->p3m = *p;
+  const double side = 0.5 * (double)p->cutoff * (1.0 + 1e-5);
+  int m[3];
+  for (int i = 0; i < 3; ++i) m[i] = std::max(1, (int)std::floor((double)p->box[i] / side));
+  if ((int64_t)m[0] * m[1] * m[2] > 64ll * 1024 * 1024)
+    fail(PMGPU_ERR_UNSUPPORTED, "cutoff too small for the box (too many short-range cells)");
+  c->mcx = m[0];
+  c->mcy = m[1];
+  c->mcz = m[2];
+  c->hcx = p->box[0] / m[0];
+  c->hcy = p->box[1] / m[1];
+  c->hcz = p->box[2] / m[2];
+  c->ncell_sr = m[0] * m[1] * m[2];
+  dalloc(&c->sr_count, c->ncell_sr + 1);
+  dalloc(&c->sr_start, c->ncell_sr + 1);
+  dalloc(&c->sr_nchunk, c->ncell_sr + 1);
+  dalloc(&c->sr_chunk_start, c->ncell_sr + 1);
+  dalloc(&c->sr_table, SR_TABLE);
+  dalloc(&c->pair_count, 1);
+  ensure_scan_scratch(c, c->ncell_sr);
+  c->p3m_ready = true;
+  if (c->n > 0) {
+    dalloc(&c->sr_key, c->n);
+    dalloc(&c->sr_rank, c->n);
+    dalloc(&c->sr_pm, c->n);
+    dalloc(&c->sr_idx, c->n);
+  }
+  p3m_setup(c);
+  PMGPU_API_END(c)
+}
+
+int pmgpu_p3m_short_range(pmgpu_ctx* c) {
+  PMGPU_API_BEGIN(c)
+  require_particles(c);
+  if (!c->p3m_ready) fail(PMGPU_ERR_NOT_READY, "call pmgpu_p3m_configure first");
+  c->iter = INT_MIN;
+  p3m_short_range(c, MODE_ACC, 0.0f, c->iter);
+  PMGPU_API_END(c)
+}
+
+int pmgpu_p3m_count_pairs(pmgpu_ctx* c, int64_t* pairs) {
+  PMGPU_API_BEGIN(c)
+  require_particles(c);
+  if (!c->p3m_ready) fail(PMGPU_ERR_NOT_READY, "call pmgpu_p3m_configure first");
+  if (!pairs) fail(PMGPU_ERR_INVALID_ARGUMENT, "pairs is NULL");
+  *pairs = p3m_count_pairs(c);
+  PMGPU_API_END(c)
+}
+
+int pmgpu_leapfrog(pmgpu_ctx* c, int32_t op, float dt) {
+  PMGPU_API_BEGIN(c)
+  require_particles(c);
+  if (op < PMGPU_HALF_KICK || op > PMGPU_DRIFT) fail(PMGPU_ERR_INVALID_ARGUMENT, "unknown leapfrog op");
+  if (op == PMGPU_INTEGER_STEP && !c->vint) dalloc(&c->vint, c->n);
+  leapfrog_op(c, op, dt, INT_MIN);
+  if (op == PMGPU_DRIFT) c->sorted_valid = false;
+  PMGPU_API_END(c)
+}
+
+int pmgpu_get_integer_step_velocities(pmgpu_ctx* c, float* vel) {
+  PMGPU_API_BEGIN(c)
+  if (!vel) fail(PMGPU_ERR_INVALID_ARGUMENT, "vel is NULL");
+  if (!c->vint) fail(PMGPU_ERR_NOT_READY, "call pmgpu_leapfrog(PMGPU_INTEGER_STEP) first");
+  const int64_t n = c->n;
+  std::vector<float4> h(n);
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  HIP_CHECK(hipMemcpy(h.data(), c->vint, sizeof(float4) * n, hipMemcpyDeviceToHost));
+  for (int64_t s = 0; s < n; ++s) {
+    int32_t id;
+    std::memcpy(&id, &h[s].w, sizeof(id));
+    vel[3 * id] = h[s].x;
+    vel[3 * id + 1] = h[s].y;
+    vel[3 * id + 2] = h[s].z;
+  }
+  PMGPU_API_END(c)
+}
+
+int pmgpu_run(pmgpu_ctx* c, int32_t iterations, int32_t initial, int32_t p3m,
+              int32_t* iterations_done) {
+  PMGPU_API_BEGIN(c)
+  require_particles(c);
+  if (!c->greens_ready) fail(PMGPU_ERR_NOT_READY, "call pmgpu_init_greens first");
+  if (p3m && !c->p3m_ready) fail(PMGPU_ERR_NOT_READY, "call pmgpu_p3m_configure first");
+  if (iterations < 0) fail(PMGPU_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
+  // run-loop iteration numbers continue across calls; escape[0] holds the
+  // first iteration whose drift left the box
+  const int base = c->run_iter;
+  auto force = [&](int mode, float kick) {
+    if (p3m) {
+      pm_force(c, MODE_ACC, 0.0f);
+      p3m_short_range(c, MODE_ACC | mode, kick, c->iter);
+    } else {
+      pm_force(c, MODE_ACC | mode, kick);
+    }
+  };
+  if (initial) {
+    c->iter = base - 1;
+    force(MODE_KICK | (iterations > 0 ? MODE_DRIFT : 0), 0.5f);
+  } else if (iterations > 0) {
+    leapfrog_op(c, PMGPU_DRIFT, 1.0f, base);
+  }
+  for (int it = 0; it < iterations; ++it) {
+    c->iter = base + it;
+    force(MODE_KICK | (it + 1 < iterations ? MODE_DRIFT : 0), 1.0f);
+  }
+  c->run_iter = base + iterations;
+  c->sorted_valid = false;
+  if (iterations_done) {
+    int32_t e = INT_MAX;
+    HIP_CHECK(hipMemcpyAsync(&e, c->escape, sizeof(e), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    int64_t done = (int64_t)e - base;
+    *iterations_done = (int32_t)std::max<int64_t>(0, std::min<int64_t>(iterations, done));
+  }
+  PMGPU_API_END(c)
+}
+
+int pmgpu_escaped(pmgpu_ctx* c, int32_t* escaped) {
+  PMGPU_API_BEGIN(c)
+  if (!escaped) fail(PMGPU_ERR_INVALID_ARGUMENT, "escaped is NULL");
+  *escaped = 0;
+  if (c->n == 0) return PMGPU_OK;
+  zero_fill_i32(c, c->counters + 8, 1);
+  escape_check(c, c->counters + 8);
+  int32_t h = 0;
+  HIP_CHECK(hipMemcpyAsync(&h, c->counters + 8, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  *escaped = h != 0;
+  PMGPU_API_END(c)
+}
+
+int pmgpu_get_grid(pmgpu_ctx* c, int32_t which, float* out) {
+  PMGPU_API_BEGIN(c)
+  if (!out) fail(PMGPU_ERR_INVALID_ARGUMENT, "out is NULL");
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  switch (which) {
+    case PMGPU_GRID_DENSITY:
+      HIP_CHECK(hipMemcpy(out, c->rho, sizeof(float) * c->M, hipMemcpyDeviceToHost));
+      break;
+    case PMGPU_GRID_POTENTIAL:
+      HIP_CHECK(hipMemcpy(out, c->phi, sizeof(float) * c->M, hipMemcpyDeviceToHost));
+      break;
+    case PMGPU_GRID_FIELD:
+      if (!c->field) dalloc(&c->field, 3 * c->M);
+      field_grid(c);
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+      HIP_CHECK(hipMemcpy(out, c->field, sizeof(float) * 3 * c->M, hipMemcpyDeviceToHost));
+      break;
+    case PMGPU_GRID_GREENS: {
+      HIP_CHECK(hipMemcpy(out, c->greens, sizeof(float) * c->K, hipMemcpyDeviceToHost));
+      const double m = (double)c->M;
+      for (int64_t i = 0; i < c->K; ++i) out[i] = (float)((double)out[i] * m);
+      break;
+    }
+    default:
+      fail(PMGPU_ERR_INVALID_ARGUMENT, "unknown grid");
+  }
+  PMGPU_API_END(c)
+}
+
+int pmgpu_profile_enable(pmgpu_ctx* c, int32_t on) {
+  PMGPU_API_BEGIN(c)
+  c->profiling = on != 0;
+  PMGPU_API_END(c)
+}
+
+int pmgpu_profile_reset(pmgpu_ctx* c) {
+  PMGPU_API_BEGIN(c)
+  profile_collect(c);
+  for (auto& s : c->stages) {
+    s.total_ms = 0.0;
+    s.launches = 0;
+  }
+  PMGPU_API_END(c)
+}
+
+int pmgpu_profile_read(pmgpu_ctx* c, int32_t max, const char** names, double* total_ms,
+                       int64_t* launches, int32_t* count) {
+  PMGPU_API_BEGIN(c)
+  profile_collect(c);
+  int k = 0;
+  for (auto& s : c->stages) {
+    if (k >= max) break;
+    if (names) names[k] = s.name;
+    if (total_ms) total_ms[k] = s.total_ms;
+    if (launches) launches[k] = s.launches;
+    ++k;
+  }
+  if (count) *count = k;
+  PMGPU_API_END(c)
+}
+
+}  // extern "C"

@@ -1,0 +1,306 @@
+// Mass deposition (PMMethod::spreadMass, source/pmMethod.cpp:200-277) as a
+// deterministic segmented reduction over the tile-sorted particles.
+//
+// Each tile (TILE_X x TILE_Y x TILE_Z cells) gathers every particle whose
+// assignment stencil reaches into it: its own particles (one contiguous
+// range) plus the boundary-class ranges of its 26 neighbours.  That virtual
+// list is cut into work items of DEPOSIT_CHUNK particles.  A work item sums
+// its contributions in LDS as 64-bit fixed point (integer adds are
+// associative, so the sum does not depend on thread or atomic order), then
+//  - a tile with one work item writes its 512 cells straight to the density
+//    grid (every cell is written exactly once: no clearing pass, no float
+//    atomics),
+//  - a tile with several work items adds its partial sums into a per-tile
+//    fixed-point slot with integer atomics; a finalize pass converts them.
+// The contribution of one particle to one cell is formed in FP32 in the
+// reference's operation order ((m/8)*fx*fy*fz for TSC, m*tx*ty*tz for CIC)
+// and quantised at 2^-shift with shift chosen so that no cell sum can
+// overflow 2^62.
+#include "pmgpu_internal.h"
+
+namespace pmgpu {
+
+namespace {
+
+constexpr int DEP_THREADS = 256;
+constexpr int MAX_RANGES = 99;
+
+// Slots of the device counter block (ctx->counters).
+constexpr int CNT_SLOTS = 0;
+
+__device__ __forceinline__ int wrapc(int a, int n) {
+  a %= n;
+  return a < 0 ? a + n : a;
+}
+
+__device__ __forceinline__ int anchor_of(float x, int scheme) {
+  float f = scheme == PMGPU_NGP ? roundf(x) : floorf(x);
+  f = fminf(fmaxf(f, -1.0e9f), 1.0e9f);
+  return (int)f;
+}
+
+// Per-axis stencil: first offset relative to the anchor and weights.
+// TSC: PMMethod::TSCAssignmentFunc (pmMethod.cpp:187-198) values f(d,-1),
+// f(d,0), f(d,1); the 1/8 factor is applied to the mass.
+__device__ __forceinline__ void axis_weights(int scheme, float d, float w[3]) {
+  if (scheme == PMGPU_TSC) {
+    w[0] = (0.5f - d) * (0.5f - d);
+    w[1] = 1.5f - 2.0f * d * d;
+    w[2] = (0.5f + d) * (0.5f + d);
+  } else if (scheme == PMGPU_CIC) {
+    w[0] = 1.0f - d;
+    w[1] = d;
+    w[2] = 0.0f;
+  } else {
+    w[0] = 1.0f;
+    w[1] = 0.0f;
+    w[2] = 0.0f;
+  }
+}
+
+// Number of ranges and their virtual lengths for tile t.  Entry 0 is the own
+// tile (all 27 classes); entries 1.. come from the host-built table.
+__device__ __forceinline__ void tile_range(int r, int t, int ntx, int nty, int ntz,
+                                           const int4* __restrict__ table,
+                                           const int32_t* __restrict__ cls_start, int32_t* start,
+                                           int32_t* len) {
+  if (r == 0) {
+    int32_t s = cls_start[t * NCLASS];
+    *start = s;
+    *len = cls_start[t * NCLASS + NCLASS] - s;
+    return;
+  }
+  int4 e = table[r - 1];
+  int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
+  int ux = wrapc(tx + e.x, ntx), uy = wrapc(ty + e.y, nty), uz = wrapc(tz + e.z, ntz);
+  int k = (ux + ntx * (uy + nty * uz)) * NCLASS + e.w;
+  int32_t s = cls_start[k];
+  *start = s;
+  *len = cls_start[k + 1] - s;
+}
+
+// Per tile: number of deposit work items (>= 1: every tile writes its cells)
+// and of interpolation work items (own particles only), plus a fixed-point
+// slot for tiles split over several deposit work items.
+__global__ void k_deposit_plan(int ntiles, int ntx, int nty, int ntz, int nranges,
+                               const int4* __restrict__ table,
+                               const int32_t* __restrict__ cls_start,
+                               int32_t* __restrict__ dep_nchunk, int32_t* __restrict__ int_nchunk,
+                               int32_t* __restrict__ slot, int32_t* __restrict__ slot_tile,
+                               int32_t* __restrict__ counters, int64_t max_slots,
+                               const int32_t* __restrict__ escape, int iteration) {
+  if (escape[0] <= iteration) return;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  int64_t total = 0;
+  int32_t own = 0;
+  for (int r = 0; r < nranges; ++r) {
+    int32_t s, l;
+    tile_range(r, t, ntx, nty, ntz, table, cls_start, &s, &l);
+    if (r == 0) own = l;
+    total += l;
+  }
+  int nch = (int)((total + DEPOSIT_CHUNK - 1) / DEPOSIT_CHUNK);
+  if (nch < 1) nch = 1;
+  dep_nchunk[t] = nch;
+  int_nchunk[t] = (own + INTERP_CHUNK - 1) / INTERP_CHUNK;
+  int s = -1;
+  if (nch > 1) {
+    s = atomicAdd(&counters[CNT_SLOTS], 1);
+    if (s >= max_slots) s = -2;  // cannot happen with the host's bound; flagged below
+    else slot_tile[s] = t;
+  }
+  slot[t] = s;
+}
+
+__device__ __forceinline__ int find_item(const int32_t* __restrict__ starts, int n, int w) {
+  // largest t with starts[t] <= w (starts has n+1 entries, non-decreasing)
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (starts[mid] <= w) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(DEP_THREADS) void k_deposit(
+    const float4* __restrict__ pm, int nx, int ny, int nz, int ntx, int nty, int ntz,
+    int ntiles, int scheme, int nranges, const int4* __restrict__ table,
+    const int32_t* __restrict__ cls_start, const int32_t* __restrict__ chunk_start,
+    const int32_t* __restrict__ slot, long long* __restrict__ fx_accum, int shift,
+    float* __restrict__ rho, const int32_t* __restrict__ escape, int iteration) {
+  if (escape[0] <= iteration) return;
+  __shared__ unsigned long long acc[TILE_CELLS];
+  __shared__ int32_t rstart[MAX_RANGES], rpre[MAX_RANGES + 1];
+  __shared__ int s_tile, s_chunk, s_nchunk;
+
+  const int total = chunk_start[ntiles];
+  const int lo = scheme == PMGPU_TSC ? -1 : 0;
+  const int cnt = scheme == PMGPU_TSC ? 3 : (scheme == PMGPU_CIC ? 2 : 1);
+
+  for (int w = blockIdx.x; w < total; w += gridDim.x) {
+    if (threadIdx.x == 0) {
+      int t = find_item(chunk_start, ntiles, w);
+      s_tile = t;
+      s_chunk = w - chunk_start[t];
+      s_nchunk = chunk_start[t + 1] - chunk_start[t];
+    }
+    for (int i = threadIdx.x; i < TILE_CELLS; i += DEP_THREADS) acc[i] = 0ull;
+    __syncthreads();
+    const int t = s_tile;
+    if (threadIdx.x < nranges) {
+      int32_t s, l;
+      tile_range(threadIdx.x, t, ntx, nty, ntz, table, cls_start, &s, &l);
+      rstart[threadIdx.x] = s;
+      rpre[threadIdx.x + 1] = l;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      rpre[0] = 0;
+      for (int r = 1; r <= nranges; ++r) rpre[r] += rpre[r - 1];
+    }
+    __syncthreads();
+    const int ox = (t % ntx) * TILE_X, oy = ((t / ntx) % nty) * TILE_Y,
+              oz = (t / (ntx * nty)) * TILE_Z;
+    const int vbeg = s_chunk * DEPOSIT_CHUNK;
+    const int vend = min(vbeg + DEPOSIT_CHUNK, rpre[nranges]);
+    for (int v = vbeg + (int)threadIdx.x; v < vend; v += DEP_THREADS) {
+      // range containing virtual index v
+      int a = 0, b = nranges;
+      while (b - a > 1) {
+        int mid = (a + b) >> 1;
+        if (rpre[mid] <= v) a = mid;
+        else b = mid;
+      }
+      const float4 p = pm[rstart[a] + (v - rpre[a])];
+      const int ax = anchor_of(p.x, scheme), ay = anchor_of(p.y, scheme),
+                az = anchor_of(p.z, scheme);
+      float wx[3], wy[3], wz[3];
+      axis_weights(scheme, p.x - (float)ax, wx);
+      axis_weights(scheme, p.y - (float)ay, wy);
+      axis_weights(scheme, p.z - (float)az, wz);
+      const float m = scheme == PMGPU_TSC ? p.w / 8.0f : p.w;
+      // local coordinates of the stencil's first cell relative to the tile,
+      // after periodic wrap
+      int bx = wrapc(ax + lo, nx) - ox, by = wrapc(ay + lo, ny) - oy, bz = wrapc(az + lo, nz) - oz;
+      for (int i = 0; i < cnt; ++i) {
+        int lx = bx + i;
+        if (lx >= nx - ox) lx -= nx;  // wrapped past the grid end
+        if (lx < 0 || lx >= TILE_X) continue;
+        const float t1 = m * wx[i];
+        for (int j = 0; j < cnt; ++j) {
+          int ly = by + j;
+          if (ly >= ny - oy) ly -= ny;
+          if (ly < 0 || ly >= TILE_Y) continue;
+          const float t2 = t1 * wy[j];
+          for (int k = 0; k < cnt; ++k) {
+            int lz = bz + k;
+            if (lz >= nz - oz) lz -= nz;
+            if (lz < 0 || lz >= TILE_Z) continue;
+            const float t3 = t2 * wz[k];
+            long long q = __float2ll_rn(ldexpf(t3, shift));
+            atomicAdd(&acc[lx + TILE_X * (ly + TILE_Y * lz)], (unsigned long long)q);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (s_nchunk == 1) {
+      for (int i = threadIdx.x; i < TILE_CELLS; i += DEP_THREADS) {
+        int lx = i % TILE_X, ly = (i / TILE_X) % TILE_Y, lz = i / (TILE_X * TILE_Y);
+        float v = ldexpf((float)(long long)acc[i], -shift);
+        rho[(int64_t)(ox + lx) + (int64_t)nx * ((oy + ly) + (int64_t)ny * (oz + lz))] = v;
+      }
+    } else {
+      const int s = slot[t];
+      if (s >= 0) {
+        for (int i = threadIdx.x; i < TILE_CELLS; i += DEP_THREADS) {
+          unsigned long long v = acc[i];
+          if (v) atomicAdd((unsigned long long*)&fx_accum[(int64_t)s * TILE_CELLS + i], v);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_deposit_finalize(int nx, int ny, int ntx, int nty, long long* __restrict__ fx_accum,
+                                   const int32_t* __restrict__ slot_tile,
+                                   const int32_t* __restrict__ counters, int64_t max_slots,
+                                   int shift, float* __restrict__ rho,
+                                   const int32_t* __restrict__ escape, int iteration) {
+  if (escape[0] <= iteration) return;
+  const int nslots = (int)min((int64_t)counters[CNT_SLOTS], max_slots);
+  for (int s = blockIdx.x; s < nslots; s += gridDim.x) {
+    const int t = slot_tile[s];
+    const int ox = (t % ntx) * TILE_X, oy = ((t / ntx) % nty) * TILE_Y,
+              oz = (t / (ntx * nty)) * TILE_Z;
+    for (int i = threadIdx.x; i < TILE_CELLS; i += blockDim.x) {
+      int lx = i % TILE_X, ly = (i / TILE_X) % TILE_Y, lz = i / (TILE_X * TILE_Y);
+      long long* q = &fx_accum[(int64_t)s * TILE_CELLS + i];
+      rho[(int64_t)(ox + lx) + (int64_t)nx * ((oy + ly) + (int64_t)ny * (oz + lz))] =
+          ldexpf((float)*q, -shift);
+      *q = 0;
+    }
+  }
+}
+
+}  // namespace
+
+// Neighbour ranges for a scheme: (dx, dy, dz, class) of every neighbour tile
+// class whose particles can reach the tile.  TSC reaches -1..+1 cells, CIC
+// 0..+1 (so only the lower neighbour's upper face matters), NGP none.
+std::vector<int4> deposit_range_table(int scheme) {
+  std::vector<int4> out;
+  int dlo = scheme == PMGPU_NGP ? 0 : -1;
+  int dhi = scheme == PMGPU_TSC ? 1 : 0;
+  for (int dz = dlo; dz <= dhi; ++dz)
+    for (int dy = dlo; dy <= dhi; ++dy)
+      for (int dx = dlo; dx <= dhi; ++dx) {
+        if (dx == 0 && dy == 0 && dz == 0) continue;
+        int d[3] = {dx, dy, dz};
+        // classes compatible per axis: d=-1 -> upper face (2), d=+1 -> lower face (0), 0 -> any
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+          if (d[a] == -1) lo[a] = hi[a] = 2;
+          else if (d[a] == 1) lo[a] = hi[a] = 0;
+          else { lo[a] = 0; hi[a] = 2; }
+        }
+        for (int cz = lo[2]; cz <= hi[2]; ++cz)
+          for (int cy = lo[1]; cy <= hi[1]; ++cy)
+            for (int cx = lo[0]; cx <= hi[0]; ++cx)
+              out.push_back(make_int4(dx, dy, dz, cx + 3 * cy + 9 * cz));
+      }
+  return out;
+}
+
+void deposit(pmgpu_ctx* c) {
+  const int nranges = 1 + c->dep_table_len;
+  int tok = stage_begin(c, "deposit_plan");
+  zero_fill_i32(c, c->counters, 4);
+  hipLaunchKernelGGL(k_deposit_plan, dim3((c->ntiles + 255) / 256), dim3(256), 0, c->stream,
+                     c->ntiles, c->ntx, c->nty, c->ntz, nranges, c->dep_table, c->cls_start,
+                     c->tile_nchunk, c->int_nchunk, c->tile_slot, c->slot_tile, c->counters,
+                     c->fx_slots, c->escape, c->iter);
+  HIP_CHECK(hipGetLastError());
+  exclusive_scan_i32(c, c->tile_nchunk, c->tile_chunk_start, c->ntiles);
+  exclusive_scan_i32(c, c->int_nchunk, c->int_chunk_start, c->ntiles);
+  stage_end(c, tok);
+
+  tok = stage_begin(c, "deposit");
+  int64_t bound = (int64_t)c->ntiles + 8 * c->n / DEPOSIT_CHUNK + 1;
+  int grid = (int)std::min<int64_t>(bound, (int64_t)c->num_cu * 8);
+  hipLaunchKernelGGL(k_deposit, dim3(grid), dim3(DEP_THREADS), 0, c->stream, c->pm[c->cur], c->nx,
+                     c->ny, c->nz, c->ntx, c->nty, c->ntz, c->ntiles, c->prm.interpolation, nranges,
+                     c->dep_table, c->cls_start, c->tile_chunk_start, c->tile_slot, c->fx_accum,
+                     c->fx_shift, c->rho, c->escape, c->iter);
+  HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_deposit_finalize, dim3(256), dim3(256), 0, c->stream, c->nx, c->ny, c->ntx,
+                     c->nty, c->fx_accum, c->slot_tile, c->counters, c->fx_slots, c->fx_shift,
+                     c->rho, c->escape, c->iter);
+  HIP_CHECK(hipGetLastError());
+  stage_end(c, tok);
+}
+
+}  // namespace pmgpu

@@ -1,0 +1,368 @@
+// 3-D real-to-complex FFT -> x G(k) -> complex-to-real FFT, replacing
+// Grid::fftDensity, PMMethod::findFourierPotential and Grid::invFftPotential
+// (source/grid.cpp:50-56, source/pmMethod.cpp:340-350, FFTWAdapter.cpp:27-43).
+//
+// Five passes over HBM, each a batch of 1-D transforms staged in LDS:
+//   x  (R2C): rows of Nx reals -> Nx/2+1 complex, via an Nx/2 complex FFT
+//             and the even/odd split;
+//   y        : forward along y;
+//   z        : forward along z, multiply by the real influence function G,
+//              inverse along z -- one kernel, the pencil never leaves LDS;
+//   y        : inverse;
+//   x  (C2R): Nx/2+1 complex -> Nx reals.
+// The spectrum is stored [z][y][kx] with pitch Nx/2+1 (kx fastest).  Every
+// 1-D transform is a Stockham autosort FFT of radix-8 / radix-4 stages with
+// L/8 threads per transform; the inverse is unnormalised and 1/(NxNyNz) is
+// folded into G (the reference's adapters divide after the inverse,
+// FFTWAdapter.cpp:39-40).
+#include "pmgpu_internal.h"
+
+namespace pmgpu {
+
+namespace {
+
+__device__ __forceinline__ float2 operator+(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 operator-(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 scale2(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 rot90(float2 a) {
+  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft2(float2* a) {
+  float2 t = a[0];
+  a[0] = t + a[1];
+  a[1] = t - a[1];
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+  float2 t0 = a0 + a2, t1 = a0 - a2, t2 = a1 + a3, t3 = rot90<INV>(a1 - a3);
+  a0 = t0 + t2;
+  a2 = t0 - t2;
+  a1 = t1 + t3;
+  a3 = t1 - t3;
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft8(float2* a) {
+  float2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+  float2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+  dft4<INV>(e0, e1, e2, e3);
+  dft4<INV>(o0, o1, o2, o3);
+  const float h = 0.70710678118654752f;
+  // W8^1 = h(1 -/+ i), W8^2 = -/+ i, W8^3 = h(-1 -/+ i)
+  o1 = INV ? make_float2(h * (o1.x - o1.y), h * (o1.x + o1.y))
+           : make_float2(h * (o1.x + o1.y), h * (o1.y - o1.x));
+  o2 = rot90<INV>(o2);
+  o3 = INV ? make_float2(-h * (o3.x + o3.y), h * (o3.x - o3.y))
+           : make_float2(h * (o3.y - o3.x), -h * (o3.x + o3.y));
+  a[0] = e0 + o0;
+  a[4] = e0 - o0;
+  a[1] = e1 + o1;
+  a[5] = e1 - o1;
+  a[2] = e2 + o2;
+  a[6] = e2 - o2;
+  a[3] = e3 + o3;
+  a[7] = e3 - o3;
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2* a) {
+  if constexpr (R == 8) dft8<INV>(a);
+  else if constexpr (R == 4) dft4<INV>(a[0], a[1], a[2], a[3]);
+  else dft2<INV>(a);
+}
+
+// One Stockham stage of radix R over a transform of length L stored in LDS
+// (element n of column col at buf[col*cs + n*ns]).  The L/8 threads of a
+// column each do 8/R butterflies.  tw[m] = exp(-2 pi i m / L).
+template <int L, int R, int NS, bool INV>
+__device__ __forceinline__ void stockham_stage(float2* buf, int cs, int ns, int col, int t,
+                                               bool active, const float2* __restrict__ tw) {
+  constexpr int T = L / 8, NB = 8 / R, LR = L / R;
+  float2 v[8];
+  if (active) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int j = t + b * T;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b * R + r] = buf[col * cs + (j + r * LR) * ns];
+    }
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int j = t + b * T;
+      const int k = j & (NS - 1);
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[(r * k) * (L / (NS * R))];
+          if (INV) w.y = -w.y;
+          v[b * R + r] = cmul(v[b * R + r], w);
+        }
+      }
+      dft<R, INV>(&v[b * R]);
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[col * cs + (base + r * NS) * ns] = v[b * R + r];
+    }
+  }
+  __syncthreads();
+}
+
+template <int L, bool INV, int NS, int... Rs>
+struct Stages;
+template <int L, bool INV, int NS>
+struct Stages<L, INV, NS> {
+  static __device__ __forceinline__ void run(float2*, int, int, int, int, bool, const float2*) {}
+};
+template <int L, bool INV, int NS, int R, int... Rs>
+struct Stages<L, INV, NS, R, Rs...> {
+  static __device__ __forceinline__ void run(float2* buf, int cs, int ns, int col, int t,
+                                             bool active, const float2* tw) {
+    stockham_stage<L, R, NS, INV>(buf, cs, ns, col, t, active, tw);
+    Stages<L, INV, NS * R, Rs...>::run(buf, cs, ns, col, t, active, tw);
+  }
+};
+
+// Radix plans: L/8 threads per transform, 8 points per thread per stage.
+template <int L, bool INV>
+__device__ __forceinline__ void fft_lds(float2* buf, int cs, int ns, int col, int t, bool active,
+                                        const float2* tw) {
+  if constexpr (L == 16) Stages<L, INV, 1, 4, 4>::run(buf, cs, ns, col, t, active, tw);
+  else if constexpr (L == 32) Stages<L, INV, 1, 8, 4>::run(buf, cs, ns, col, t, active, tw);
+  else if constexpr (L == 64) Stages<L, INV, 1, 8, 8>::run(buf, cs, ns, col, t, active, tw);
+  else if constexpr (L == 128) Stages<L, INV, 1, 8, 4, 4>::run(buf, cs, ns, col, t, active, tw);
+  else if constexpr (L == 256) Stages<L, INV, 1, 8, 8, 4>::run(buf, cs, ns, col, t, active, tw);
+  else if constexpr (L == 512) Stages<L, INV, 1, 8, 8, 8>::run(buf, cs, ns, col, t, active, tw);
+  else Stages<L, INV, 1, 8, 8, 4, 4>::run(buf, cs, ns, col, t, active, tw);
+}
+
+// ---------------------------------------------------------------------------
+// x pass, real -> complex.  L = Nx/2 complex points per row, ROWS rows per
+// workgroup of 256 threads.
+template <int L>
+__global__ __launch_bounds__(256) void k_fft_x_r2c(const float* __restrict__ rho,
+                                                   float2* __restrict__ spec,
+                                                   const float2* __restrict__ tw,
+                                                   const float2* __restrict__ tw_r2c,
+                                                   const int32_t* __restrict__ escape,
+                                                   int iteration) {
+  if (escape[0] <= iteration) return;
+  constexpr int T = L / 8, ROWS = 256 / T, P = L + 2;
+  __shared__ float2 buf[ROWS * P];
+  const int row0 = blockIdx.x * ROWS;
+  const float2* src = reinterpret_cast<const float2*>(rho) + (int64_t)row0 * L;
+  for (int e = threadIdx.x; e < ROWS * L; e += 256) buf[(e / L) * P + (e % L)] = src[e];
+  __syncthreads();
+  const int col = threadIdx.x / T, t = threadIdx.x % T;
+  fft_lds<L, false>(buf, P, 1, col, t, true, tw);
+  float2* dst = spec + (int64_t)(row0 + col) * (L + 1);
+  for (int k = t; k <= L; k += T) {
+    float2 zk = buf[col * P + (k & (L - 1))];
+    float2 zc = conjf2(buf[col * P + ((L - k) & (L - 1))]);
+    float2 e = scale2(zk + zc, 0.5f);
+    float2 o = scale2(rot90<false>(zk - zc), 0.5f);  // (zk - zc) / (2i)
+    dst[k] = e + cmul(tw_r2c[k], o);
+  }
+}
+
+// x pass, complex -> real (unnormalised).
+template <int L>
+__global__ __launch_bounds__(256) void k_fft_x_c2r(const float2* __restrict__ spec,
+                                                   float* __restrict__ phi,
+                                                   const float2* __restrict__ tw,
+                                                   const float2* __restrict__ tw_r2c,
+                                                   const int32_t* __restrict__ escape,
+                                                   int iteration) {
+  if (escape[0] <= iteration) return;
+  constexpr int T = L / 8, ROWS = 256 / T, P = L + 2;
+  __shared__ float2 buf[ROWS * P];
+  const int row0 = blockIdx.x * ROWS;
+  const float2* src = spec + (int64_t)row0 * (L + 1);
+  for (int e = threadIdx.x; e < ROWS * (L + 1); e += 256)
+    buf[(e / (L + 1)) * P + (e % (L + 1))] = src[e];
+  __syncthreads();
+  const int col = threadIdx.x / T, t = threadIdx.x % T;
+  float2 z[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int k = t + i * T;
+    float2 xk = buf[col * P + k];
+    float2 xc = conjf2(buf[col * P + (L - k)]);
+    // Z'[k] = (Xk + Xc) + i W^-k (Xk - Xc), W = exp(-2 pi i / Nx)
+    z[i] = (xk + xc) + rot90<true>(cmul(conjf2(tw_r2c[k]), xk - xc));
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 8; ++i) buf[col * P + t + i * T] = z[i];
+  __syncthreads();
+  fft_lds<L, true>(buf, P, 1, col, t, true, tw);
+  float2* dst = reinterpret_cast<float2*>(phi) + (int64_t)row0 * L;
+  for (int e = threadIdx.x; e < ROWS * L; e += 256) dst[e] = buf[(e / L) * P + (e % L)];
+}
+
+// Column tiles for the y and z passes: TW columns of kx per workgroup, the
+// last tile also takes the Nyquist column kx = Nx/2.
+template <int L>
+struct ColTile {
+  static constexpr int T = L / 8;
+  static constexpr int TW = L <= 256 ? 16 : (L == 512 ? 8 : 4);
+  static constexpr int NC = TW + 1;
+  static constexpr int THREADS = NC * T;
+};
+
+// y pass (forward or inverse), in place on the spectrum.
+template <int L, bool INV>
+__global__ __launch_bounds__(ColTile<L>::THREADS) void k_fft_y(float2* __restrict__ spec, int nxh,
+                                                               int ntile,
+                                                               const float2* __restrict__ tw,
+                                                               const int32_t* __restrict__ escape,
+                                                               int iteration) {
+  if (escape[0] <= iteration) return;
+  using CT = ColTile<L>;
+  __shared__ float2 buf[L * CT::NC];
+  const int tile = blockIdx.x, z = blockIdx.y;
+  const int c0 = tile * CT::TW;
+  const int ncols = tile == ntile - 1 ? CT::TW + 1 : CT::TW;
+  float2* base = spec + (int64_t)z * L * nxh + c0;
+  for (int e = threadIdx.x; e < L * ncols; e += CT::THREADS) {
+    const int n = e / ncols, col = e % ncols;
+    buf[n * CT::NC + col] = base[(int64_t)n * nxh + col];
+  }
+  __syncthreads();
+  const int col = threadIdx.x % CT::NC, t = threadIdx.x / CT::NC;
+  fft_lds<L, INV>(buf, 1, CT::NC, col, t, col < ncols, tw);
+  for (int e = threadIdx.x; e < L * ncols; e += CT::THREADS) {
+    const int n = e / ncols, col2 = e % ncols;
+    base[(int64_t)n * nxh + col2] = buf[n * CT::NC + col2];
+  }
+}
+
+// z pass: forward along z, multiply by G, inverse along z.
+template <int L>
+__global__ __launch_bounds__(ColTile<L>::THREADS) void k_fft_z_green(
+    float2* __restrict__ spec, const float* __restrict__ greens, int nxh, int ny, int ntile,
+    const float2* __restrict__ tw, const int32_t* __restrict__ escape, int iteration) {
+  if (escape[0] <= iteration) return;
+  using CT = ColTile<L>;
+  __shared__ float2 buf[L * CT::NC];
+  const int tile = blockIdx.x, y = blockIdx.y;
+  const int c0 = tile * CT::TW;
+  const int ncols = tile == ntile - 1 ? CT::TW + 1 : CT::TW;
+  const int64_t zstride = (int64_t)ny * nxh;
+  const int64_t off = (int64_t)y * nxh + c0;
+  for (int e = threadIdx.x; e < L * ncols; e += CT::THREADS) {
+    const int n = e / ncols, col = e % ncols;
+    buf[n * CT::NC + col] = spec[off + n * zstride + col];
+  }
+  __syncthreads();
+  const int col = threadIdx.x % CT::NC, t = threadIdx.x / CT::NC;
+  fft_lds<L, false>(buf, 1, CT::NC, col, t, col < ncols, tw);
+  for (int e = threadIdx.x; e < L * ncols; e += CT::THREADS) {
+    const int n = e / ncols, col2 = e % ncols;
+    buf[n * CT::NC + col2] = scale2(buf[n * CT::NC + col2], greens[off + n * zstride + col2]);
+  }
+  __syncthreads();
+  fft_lds<L, true>(buf, 1, CT::NC, col, t, col < ncols, tw);
+  for (int e = threadIdx.x; e < L * ncols; e += CT::THREADS) {
+    const int n = e / ncols, col2 = e % ncols;
+    spec[off + n * zstride + col2] = buf[n * CT::NC + col2];
+  }
+}
+
+#define PMGPU_LAUNCH_X(KERNEL, L, grid, ...)                                               \
+  switch (L) {                                                                             \
+    case 16: hipLaunchKernelGGL(KERNEL<16>, grid, dim3(256), 0, __VA_ARGS__); break;       \
+    case 32: hipLaunchKernelGGL(KERNEL<32>, grid, dim3(256), 0, __VA_ARGS__); break;       \
+    case 64: hipLaunchKernelGGL(KERNEL<64>, grid, dim3(256), 0, __VA_ARGS__); break;       \
+    case 128: hipLaunchKernelGGL(KERNEL<128>, grid, dim3(256), 0, __VA_ARGS__); break;     \
+    case 256: hipLaunchKernelGGL(KERNEL<256>, grid, dim3(256), 0, __VA_ARGS__); break;     \
+    case 512: hipLaunchKernelGGL(KERNEL<512>, grid, dim3(256), 0, __VA_ARGS__); break;     \
+    default: fail(PMGPU_ERR_INVALID_ARGUMENT, "unsupported x length");                    \
+  }
+
+int rows_per_block(int L) {
+  return 256 / (L / 8);
+}
+
+template <int L, bool INV>
+void launch_y(pmgpu_ctx* c) {
+  using CT = ColTile<L>;
+  const int ntile = (c->nx / 2) / CT::TW;
+  hipLaunchKernelGGL((k_fft_y<L, INV>), dim3(ntile, c->nz), dim3(CT::THREADS), 0, c->stream,
+                     c->spec, c->nxh, ntile, c->tw_y, c->escape, c->iter);
+}
+
+template <int L>
+void launch_z(pmgpu_ctx* c) {
+  using CT = ColTile<L>;
+  const int ntile = (c->nx / 2) / CT::TW;
+  hipLaunchKernelGGL((k_fft_z_green<L>), dim3(ntile, c->ny), dim3(CT::THREADS), 0, c->stream,
+                     c->spec, c->greens, c->nxh, c->ny, ntile, c->tw_z, c->escape, c->iter);
+}
+
+template <bool INV>
+void y_pass(pmgpu_ctx* c) {
+  switch (c->ny) {
+    case 32: launch_y<32, INV>(c); break;
+    case 64: launch_y<64, INV>(c); break;
+    case 128: launch_y<128, INV>(c); break;
+    case 256: launch_y<256, INV>(c); break;
+    case 512: launch_y<512, INV>(c); break;
+    case 1024: launch_y<1024, INV>(c); break;
+    default: fail(PMGPU_ERR_INVALID_ARGUMENT, "unsupported ny");
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+void z_pass(pmgpu_ctx* c) {
+  switch (c->nz) {
+    case 32: launch_z<32>(c); break;
+    case 64: launch_z<64>(c); break;
+    case 128: launch_z<128>(c); break;
+    case 256: launch_z<256>(c); break;
+    case 512: launch_z<512>(c); break;
+    case 1024: launch_z<1024>(c); break;
+    default: fail(PMGPU_ERR_INVALID_ARGUMENT, "unsupported nz");
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+void fft_forward_green_inverse(pmgpu_ctx* c) {
+  const int L = c->nx / 2;
+  const int rows = c->ny * c->nz;
+  const dim3 gx(rows / rows_per_block(L));
+  int tok = stage_begin(c, "fft_x_r2c");
+  PMGPU_LAUNCH_X(k_fft_x_r2c, L, gx, c->stream, c->rho, c->spec, c->tw_x, c->tw_r2c, c->escape,
+                 c->iter);
+  HIP_CHECK(hipGetLastError());
+  stage_end(c, tok);
+  tok = stage_begin(c, "fft_y_fwd");
+  y_pass<false>(c);
+  stage_end(c, tok);
+  tok = stage_begin(c, "fft_z_green");
+  z_pass(c);
+  stage_end(c, tok);
+  tok = stage_begin(c, "fft_y_inv");
+  y_pass<true>(c);
+  stage_end(c, tok);
+  tok = stage_begin(c, "fft_x_c2r");
+  PMGPU_LAUNCH_X(k_fft_x_c2r, L, gx, c->stream, c->spec, c->phi, c->tw_x, c->tw_r2c, c->escape,
+                 c->iter);
+  HIP_CHECK(hipGetLastError());
+  stage_end(c, tok);
+}
+
+}  // namespace pmgpu

@@ -1,0 +1,322 @@
+// Field and interpolation (PMMethod::findFieldInCells + updateAccelerations,
+// source/pmMethod.cpp:279-338, 352-390) fused with the leapfrog kick and the
+// next drift (source/leapfrog.cpp:16-24).
+//
+// One work item = one tile's own particles (a contiguous range of the
+// tile-sorted arrays, cut into INTERP_CHUNK pieces).  The work item stages
+// the potential of the tile plus the stencil and finite-difference halo in
+// LDS, forms E = -grad(phi) on the stencil region in LDS (2- or 4-point,
+// pmMethod.cpp:355-365), and interpolates with the deposit's weights.  The
+// HBM traffic per step is the potential once (halo re-reads hit L2) plus the
+// particle state once.
+#include "pmgpu_internal.h"
+
+namespace pmgpu {
+
+namespace {
+
+__device__ __forceinline__ int wrapc(int a, int n) {
+  a %= n;
+  return a < 0 ? a + n : a;
+}
+
+__device__ __forceinline__ int anchor_of(float x, int scheme) {
+  float f = scheme == PMGPU_NGP ? roundf(x) : floorf(x);
+  f = fminf(fmaxf(f, -1.0e9f), 1.0e9f);
+  return (int)f;
+}
+
+__device__ __forceinline__ int find_item(const int32_t* __restrict__ starts, int n, int w) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (starts[mid] <= w) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+struct ExtField {
+  int kind;
+  float cx, cy, cz, R, M, G, H, DT;
+};
+
+// sphRadDecrField (source/externalFields.cpp:4-16) at pos*H, converted with
+// accelerationToCodeUnits (include/unitConversions.h:22-24).
+__device__ __forceinline__ float3 ext_field(const ExtField& e, float x, float y, float z) {
+  if (e.kind != PMGPU_EXT_SPH_RAD_DECR) return make_float3(0.f, 0.f, 0.f);
+  float dx = x * e.H - e.cx, dy = y * e.H - e.cy, dz = z * e.H - e.cz;
+  float r = sqrtf(dx * dx + dy * dy + dz * dz);
+  float g = r > e.R ? -e.G * e.M / (r * r) : -(e.G * e.M / (e.R * e.R * e.R)) * r * (4.0f - 3.0f * r / e.R);
+  float s = e.DT * e.DT * g / (r * e.H);
+  return make_float3(s * dx, s * dy, s * dz);
+}
+
+__device__ __forceinline__ void leapfrog_tail(float4& p, float4& v, float ax, float ay, float az,
+                                              int mode, float kick, float3 box,
+                                              int32_t* __restrict__ escape, int iteration) {
+  if (mode & MODE_KICK) {
+    v.x += kick * ax;
+    v.y += kick * ay;
+    v.z += kick * az;
+  }
+  if (mode & MODE_DRIFT) {
+    p.x += v.x;
+    p.y += v.y;
+    p.z += v.z;
+    bool inside = inside_box(p.x, p.y, p.z, box);
+    if (!inside) atomicMin(escape, iteration + 1);
+  }
+}
+
+template <int SCHEME, int FDS>
+struct Geo {
+  static constexpr int SLO = SCHEME == PMGPU_TSC ? -1 : 0;
+  static constexpr int SHI = SCHEME == PMGPU_NGP ? 0 : 1;
+  static constexpr int CNT = SHI - SLO + 1;
+  static constexpr int F = FDS == PMGPU_TWO_POINT ? 1 : 2;
+  static constexpr int EX = TILE_X + CNT - 1, EY = TILE_Y + CNT - 1, EZ = TILE_Z + CNT - 1;
+  static constexpr int PX = EX + 2 * F, PY = EY + 2 * F, PZ = EZ + 2 * F;
+};
+
+template <int SCHEME>
+__device__ __forceinline__ void interp_weights(float d, float w[3]) {
+  if constexpr (SCHEME == PMGPU_TSC) {
+    // TSCAssignmentFunc (pmMethod.cpp:187-198); the 0.125 is applied once
+    w[0] = (0.5f - d) * (0.5f - d);
+    w[1] = 1.5f - 2.0f * d * d;
+    w[2] = (0.5f + d) * (0.5f + d);
+  } else if constexpr (SCHEME == PMGPU_CIC) {
+    w[0] = 1.0f - d;
+    w[1] = d;
+  } else {
+    w[0] = 1.0f;
+  }
+}
+
+template <int SCHEME, int FDS>
+__global__ __launch_bounds__(256) void k_interp(
+    const float* __restrict__ phi, int nx, int ny, int nz, int ntx, int nty, int ntiles,
+    const int32_t* __restrict__ cls_start, const int32_t* __restrict__ chunk_start,
+    float4* __restrict__ pm, float4* __restrict__ vel, float4* __restrict__ acc, int mode,
+    float kick, float3 box, ExtField ext, int32_t* __restrict__ escape, int iteration) {
+  if (escape[0] <= iteration) return;
+  using Gm = Geo<SCHEME, FDS>;
+  __shared__ float sphi[Gm::PX * Gm::PY * Gm::PZ];
+  __shared__ float sE[3][Gm::EX * Gm::EY * Gm::EZ];
+  __shared__ int s_tile, s_chunk;
+  const int total = chunk_start[ntiles];
+  for (int w = blockIdx.x; w < total; w += gridDim.x) {
+    if (threadIdx.x == 0) {
+      int t = find_item(chunk_start, ntiles, w);
+      s_tile = t;
+      s_chunk = w - chunk_start[t];
+    }
+    __syncthreads();
+    const int t = s_tile;
+    const int ox = (t % ntx) * TILE_X, oy = ((t / ntx) % nty) * TILE_Y,
+              oz = (t / (ntx * nty)) * TILE_Z;
+    const int px0 = ox + Gm::SLO - Gm::F, py0 = oy + Gm::SLO - Gm::F, pz0 = oz + Gm::SLO - Gm::F;
+    for (int e = threadIdx.x; e < Gm::PX * Gm::PY * Gm::PZ; e += 256) {
+      const int ix = e % Gm::PX, iy = (e / Gm::PX) % Gm::PY, iz = e / (Gm::PX * Gm::PY);
+      const int gx = wrapc(px0 + ix, nx), gy = wrapc(py0 + iy, ny), gz = wrapc(pz0 + iz, nz);
+      sphi[e] = phi[(int64_t)gx + (int64_t)nx * (gy + (int64_t)ny * gz)];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < Gm::EX * Gm::EY * Gm::EZ; e += 256) {
+      const int ex = e % Gm::EX, ey = (e / Gm::EX) % Gm::EY, ez = e / (Gm::EX * Gm::EY);
+      const int c = (ex + Gm::F) + Gm::PX * ((ey + Gm::F) + Gm::PY * (ez + Gm::F));
+      const int sx = 1, sy = Gm::PX, sz = Gm::PX * Gm::PY;
+      float Ex, Ey, Ez;
+      if constexpr (FDS == PMGPU_TWO_POINT) {
+        Ex = -0.5f * (sphi[c + sx] - sphi[c - sx]);
+        Ey = -0.5f * (sphi[c + sy] - sphi[c - sy]);
+        Ez = -0.5f * (sphi[c + sz] - sphi[c - sz]);
+      } else {
+        Ex = (-1.0f / 12) * (-sphi[c + 2 * sx] + 8 * sphi[c + sx] - 8 * sphi[c - sx] + sphi[c - 2 * sx]);
+        Ey = (-1.0f / 12) * (-sphi[c + 2 * sy] + 8 * sphi[c + sy] - 8 * sphi[c - sy] + sphi[c - 2 * sy]);
+        Ez = (-1.0f / 12) * (-sphi[c + 2 * sz] + 8 * sphi[c + sz] - 8 * sphi[c - sz] + sphi[c - 2 * sz]);
+      }
+      sE[0][e] = Ex;
+      sE[1][e] = Ey;
+      sE[2][e] = Ez;
+    }
+    __syncthreads();
+    const int pbeg = cls_start[t * NCLASS] + s_chunk * INTERP_CHUNK;
+    const int pend = min(pbeg + INTERP_CHUNK, cls_start[t * NCLASS + NCLASS]);
+    for (int i = pbeg + (int)threadIdx.x; i < pend; i += 256) {
+      float4 p = pm[i];
+      const int ax = anchor_of(p.x, SCHEME), ay = anchor_of(p.y, SCHEME), az = anchor_of(p.z, SCHEME);
+      float wx[3], wy[3], wz[3];
+      interp_weights<SCHEME>(p.x - (float)ax, wx);
+      interp_weights<SCHEME>(p.y - (float)ay, wy);
+      interp_weights<SCHEME>(p.z - (float)az, wz);
+      const int lx = wrapc(ax, nx) - ox, ly = wrapc(ay, ny) - oy, lz = wrapc(az, nz) - oz;
+      float sx = 0.f, sy = 0.f, sz = 0.f;
+#pragma unroll
+      for (int a = 0; a < Gm::CNT; ++a) {
+        const float t1 = wx[a];
+#pragma unroll
+        for (int b = 0; b < Gm::CNT; ++b) {
+          const float t2 = t1 * wy[b];
+#pragma unroll
+          for (int c = 0; c < Gm::CNT; ++c) {
+            float t3 = t2 * wz[c];
+            if constexpr (SCHEME == PMGPU_TSC) t3 *= 0.125f;
+            const int e = (lx + a) + Gm::EX * ((ly + b) + Gm::EY * (lz + c));
+            sx += t3 * sE[0][e];
+            sy += t3 * sE[1][e];
+            sz += t3 * sE[2][e];
+          }
+        }
+      }
+      const float3 g = ext_field(ext, p.x, p.y, p.z);
+      sx += g.x;
+      sy += g.y;
+      sz += g.z;
+      if (mode & MODE_ACC) acc[i] = make_float4(sx, sy, sz, 0.f);
+      if (mode & (MODE_KICK | MODE_DRIFT)) {
+        float4 v = vel[i];
+        leapfrog_tail(p, v, sx, sy, sz, mode, kick, box, escape, iteration);
+        vel[i] = v;
+        pm[i] = p;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// E on every cell, for Grid::getField (include/grid.h:26).
+__global__ void k_field(const float* __restrict__ phi, int nx, int ny, int nz, int fds,
+                        float* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t M = (int64_t)nx * ny * nz;
+  if (i >= M) return;
+  const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((int64_t)nx * ny));
+  auto P = [&](int a, int b, int c) {
+    return phi[(int64_t)wrapc(a, nx) + (int64_t)nx * (wrapc(b, ny) + (int64_t)ny * wrapc(c, nz))];
+  };
+  float Ex, Ey, Ez;
+  if (fds == PMGPU_TWO_POINT) {
+    Ex = -0.5f * (P(x + 1, y, z) - P(x - 1, y, z));
+    Ey = -0.5f * (P(x, y + 1, z) - P(x, y - 1, z));
+    Ez = -0.5f * (P(x, y, z + 1) - P(x, y, z - 1));
+  } else {
+    Ex = (-1.0f / 12) * (-P(x + 2, y, z) + 8 * P(x + 1, y, z) - 8 * P(x - 1, y, z) + P(x - 2, y, z));
+    Ey = (-1.0f / 12) * (-P(x, y + 2, z) + 8 * P(x, y + 1, z) - 8 * P(x, y - 1, z) + P(x, y - 2, z));
+    Ez = (-1.0f / 12) * (-P(x, y, z + 2) + 8 * P(x, y, z + 1) - 8 * P(x, y, z - 1) + P(x, y, z - 2));
+  }
+  out[3 * i] = Ex;
+  out[3 * i + 1] = Ey;
+  out[3 * i + 2] = Ez;
+}
+
+// Leapfrog pieces on the sorted state (source/leapfrog.cpp:5-24).
+__global__ void k_leapfrog(float4* __restrict__ pm, float4* __restrict__ vel,
+                           const float4* __restrict__ acc, float4* __restrict__ vint, int64_t n,
+                           int op, float dt, float3 box, int32_t* __restrict__ escape,
+                           int iteration) {
+  if (escape[0] <= iteration) return;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 v = vel[i];
+  if (op == PMGPU_DRIFT) {
+    float4 p = pm[i];
+    p.x += dt * v.x;
+    p.y += dt * v.y;
+    p.z += dt * v.z;
+    pm[i] = p;
+    bool inside = inside_box(p.x, p.y, p.z, box);
+    // iteration < 0: a standalone updatePositions, which never stops a run
+    if (!inside && iteration >= 0) atomicMin(escape, iteration);
+    return;
+  }
+  float4 a = acc[i];
+  if (op == PMGPU_INTEGER_STEP) {
+    vint[i] = make_float4(v.x + 0.5f * dt * a.x, v.y + 0.5f * dt * a.y, v.z + 0.5f * dt * a.z, v.w);
+    return;
+  }
+  const float f = op == PMGPU_HALF_KICK ? 0.5f * dt : dt;
+  v.x += f * a.x;
+  v.y += f * a.y;
+  v.z += f * a.z;
+  vel[i] = v;
+}
+
+__global__ void k_escape(const float4* __restrict__ pm, int64_t n, float3 box,
+                         int32_t* __restrict__ flag) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 p = pm[i];
+  bool inside = inside_box(p.x, p.y, p.z, box);
+  if (!inside) atomicOr(flag, 1);
+}
+
+template <int S, int F>
+void launch_interp(pmgpu_ctx* c, int grid, int mode, float kick, float3 box, ExtField e,
+                   int iteration) {
+  hipLaunchKernelGGL((k_interp<S, F>), dim3(grid), dim3(256), 0, c->stream, c->phi, c->nx, c->ny,
+                     c->nz, c->ntx, c->nty, c->ntiles, c->cls_start, c->int_chunk_start,
+                     c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick, box, e, c->escape,
+                     iteration);
+}
+
+}  // namespace
+
+static ExtField ext_of(pmgpu_ctx* c) {
+  ExtField e{};
+  e.kind = c->prm.ext_kind;
+  e.cx = c->prm.ext_params[0];
+  e.cy = c->prm.ext_params[1];
+  e.cz = c->prm.ext_params[2];
+  e.R = c->prm.ext_params[3];
+  e.M = c->prm.ext_params[4];
+  e.G = c->prm.ext_params[5];
+  e.H = c->prm.H;
+  e.DT = c->prm.DT;
+  return e;
+}
+
+static float3 box_of(pmgpu_ctx* c) {
+  return make_float3(c->prm.box[0], c->prm.box[1], c->prm.box[2]);
+}
+
+void interpolate(pmgpu_ctx* c, int mode, float kick, int iteration) {
+  int tok = stage_begin(c, "interp");
+  int64_t bound = (int64_t)c->ntiles + c->n / INTERP_CHUNK + 1;
+  int grid = (int)std::min<int64_t>(bound, (int64_t)c->num_cu * 8);
+  const ExtField e = ext_of(c);
+  const float3 box = box_of(c);
+  const int s = c->prm.interpolation, f = c->prm.finite_diff;
+  if (s == PMGPU_NGP && f == PMGPU_TWO_POINT) launch_interp<PMGPU_NGP, PMGPU_TWO_POINT>(c, grid, mode, kick, box, e, iteration);
+  else if (s == PMGPU_NGP) launch_interp<PMGPU_NGP, PMGPU_FOUR_POINT>(c, grid, mode, kick, box, e, iteration);
+  else if (s == PMGPU_CIC && f == PMGPU_TWO_POINT) launch_interp<PMGPU_CIC, PMGPU_TWO_POINT>(c, grid, mode, kick, box, e, iteration);
+  else if (s == PMGPU_CIC) launch_interp<PMGPU_CIC, PMGPU_FOUR_POINT>(c, grid, mode, kick, box, e, iteration);
+  else if (f == PMGPU_TWO_POINT) launch_interp<PMGPU_TSC, PMGPU_TWO_POINT>(c, grid, mode, kick, box, e, iteration);
+  else launch_interp<PMGPU_TSC, PMGPU_FOUR_POINT>(c, grid, mode, kick, box, e, iteration);
+  HIP_CHECK(hipGetLastError());
+  stage_end(c, tok);
+}
+
+void field_grid(pmgpu_ctx* c) {
+  hipLaunchKernelGGL(k_field, dim3((unsigned)((c->M + 255) / 256)), dim3(256), 0, c->stream, c->phi,
+                     c->nx, c->ny, c->nz, c->prm.finite_diff, c->field);
+  HIP_CHECK(hipGetLastError());
+}
+
+void leapfrog_op(pmgpu_ctx* c, int op, float dt, int iteration) {
+  int tok = stage_begin(c, "leapfrog");
+  hipLaunchKernelGGL(k_leapfrog, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0, c->stream,
+                     c->pm[c->cur], c->vel[c->cur], c->acc, c->vint, c->n, op, dt, box_of(c),
+                     c->escape, iteration);
+  HIP_CHECK(hipGetLastError());
+  stage_end(c, tok);
+}
+
+void escape_check(pmgpu_ctx* c, int32_t* flag_dev) {
+  hipLaunchKernelGGL(k_escape, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0, c->stream,
+                     c->pm[c->cur], c->n, box_of(c), flag_dev);
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace pmgpu

@@ -1,0 +1,339 @@
+// P3M short-range pass (P3MMethod::calculateShortRangeForces +
+// correctAccelerations, source/p3mMethod.cpp:50-57, 168-322).
+//
+// The reference walks a chaining mesh of cells >= cutoff with linked lists,
+// a 13-cell half shell and per-neighbour accumulation slots on CPU threads.
+// Here:
+//  - particles are counting-sorted by a short-range cell of side
+//    >= cutoff/2 (non-periodic, as the chaining mesh; out-of-box particles
+//    clamp to the edge cells, which never loses a pair because clamping is
+//    1-Lipschitz), into a compact float4 (x, y, z, m) copy;
+//  - one work item = one cell's chunk of 64 i-particles; its j-stream is the
+//    25 contiguous (dz, dy) rows of the 5x5x5 cell neighbourhood.  The four
+//    waves of a workgroup take interleaved 64-particle blocks of that stream
+//    and read each j through the scalar unit (uniform address), so the inner
+//    loop is pure FP32 VALU with scalar operands;
+//  - the four partial sums are added in LDS in fixed wave order, and the
+//    pair force comes from the reference's 500-entry table in LDS
+//    (initSRForceTable, p3mMethod.cpp:275-294; lerp of
+//    shortRangeForceFromTable, p3mMethod.cpp:240-245) or its direct formula
+//    (shortRangeForce, p3mMethod.cpp:220-238).
+// Newton's third law is not used: every ordered pair is evaluated once on
+// the side of its i, which removes all write conflicts.  The mass m_i that
+// the reference multiplies into the pair force and divides out again in
+// correctAccelerations cancels and is never applied.
+#include "pmgpu_internal.h"
+
+#include <cmath>
+
+namespace pmgpu {
+
+namespace {
+
+constexpr int SR_THREADS = 256;
+constexpr int SR_ICHUNK = 64;
+constexpr int CNT_WORK = 1;  // ctx->counters slot of the work counter
+constexpr float kGc = 0.079577471545947668f;  // 1/(4 pi), the code-unit coupling
+
+__device__ __forceinline__ int clampi(int a, int lo, int hi) { return a < lo ? lo : (a > hi ? hi : a); }
+
+__device__ __forceinline__ int sr_cell_1d(float x, float inv_h, int m) {
+  float f = floorf(x * inv_h);
+  f = fminf(fmaxf(f, -1.0f), (float)m);  // NaN -> -1
+  return clampi((int)f, 0, m - 1);
+}
+
+__global__ void k_sr_keys(const float4* __restrict__ pm, int64_t n, float ihx, float ihy, float ihz,
+                          int mx, int my, int mz, int32_t* __restrict__ key,
+                          int32_t* __restrict__ rank, int32_t* __restrict__ count,
+                          const int32_t* __restrict__ escape, int iteration) {
+  if (escape[0] <= iteration) return;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 p = pm[i];
+  int k = sr_cell_1d(p.x, ihx, mx) + mx * (sr_cell_1d(p.y, ihy, my) + my * sr_cell_1d(p.z, ihz, mz));
+  key[i] = k;
+  rank[i] = atomicAdd(&count[k], 1);
+}
+
+__global__ void k_sr_scatter(const float4* __restrict__ pm, int64_t n, const int32_t* __restrict__ key,
+                             const int32_t* __restrict__ rank, const int32_t* __restrict__ start,
+                             float4* __restrict__ sr_pm, int32_t* __restrict__ sr_idx,
+                             const int32_t* __restrict__ escape, int iteration) {
+  if (escape[0] <= iteration) return;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t d = (int64_t)start[key[i]] + rank[i];
+  sr_pm[d] = pm[i];
+  sr_idx[d] = (int32_t)i;
+}
+
+__global__ void k_sr_plan(const int32_t* __restrict__ count, int ncell, int32_t* __restrict__ nchunk,
+                          const int32_t* __restrict__ escape, int iteration) {
+  if (escape[0] <= iteration) return;
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  nchunk[c] = (count[c] + SR_ICHUNK - 1) / SR_ICHUNK;
+}
+
+__device__ __forceinline__ int find_item(const int32_t* __restrict__ starts, int n, int w) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (starts[mid] <= w) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// P3MMethod::referenceForceS1 / S2 (p3mMethod.cpp:194-218).
+__device__ __forceinline__ float reference_force(int cs, float r, float a) {
+  if (cs == PMGPU_S1) {
+    if (r >= a) return kGc / (r * r);
+    float q = r / a;
+    return kGc / (a * a) * (8.0f * q - 9.0f * q * q + 2.0f * q * q * q * q);
+  }
+  float u = 2.0f * r / a;
+  float u2 = u * u, u3 = u2 * u, u4 = u2 * u2, u5 = u4 * u, u6 = u3 * u3;
+  if (u <= 1.0f)
+    return kGc / (35.0f * a * a) * (224.0f * u - 224.0f * u3 + 70.0f * u4 + 48.0f * u5 - 21.0f * u6);
+  if (u <= 2.0f)
+    return kGc / (35.0f * a * a) *
+           (12.0f / u2 - 224.0f + 896.0f * u - 840.0f * u2 + 224.0f * u3 + 70.0f * u4 - 48.0f * u5 +
+            7.0f * u6);
+  return kGc / (r * r);
+}
+
+struct SrParams {
+  int mx, my, mz;
+  float inv_d2, rc2;
+  float a, eps2;
+  int cs;
+};
+
+template <bool TABLE, bool COUNT>
+__global__ __launch_bounds__(SR_THREADS) void k_sr(
+    const float4* __restrict__ sr_pm, const int32_t* __restrict__ sr_start,
+    const int32_t* __restrict__ chunk_start, int ncell, const int32_t* __restrict__ sr_idx,
+    const float2* __restrict__ table_g, SrParams P, float4* __restrict__ pm,
+    float4* __restrict__ vel, float4* __restrict__ acc, int mode, float kick, float3 box,
+    int32_t* __restrict__ counters, unsigned long long* __restrict__ pair_count,
+    int32_t* __restrict__ escape, int iteration) {
+  if (escape[0] <= iteration) return;
+  __shared__ float2 table[SR_TABLE];
+  __shared__ float3 part[SR_THREADS];
+  __shared__ int s_w;
+  for (int i = threadIdx.x; i < SR_TABLE; i += SR_THREADS) table[i] = table_g[i];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int total = chunk_start[ncell];
+  unsigned long long pairs = 0;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_w = atomicAdd(&counters[CNT_WORK], 1);
+    __syncthreads();
+    const int w = s_w;
+    if (w >= total) break;
+    const int c = find_item(chunk_start, ncell, w);
+    const int k = w - chunk_start[c];
+    const int cx = c % P.mx, cy = (c / P.mx) % P.my, cz = c / (P.mx * P.my);
+    const int i = sr_start[c] + k * SR_ICHUNK + lane;
+    const bool valid = i < sr_start[c + 1];
+    const float4 pi = valid ? sr_pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    int blk = 0;
+    const int xlo = max(cx - SR_CELL_DIV, 0), xhi = min(cx + SR_CELL_DIV, P.mx - 1);
+    for (int dz = -SR_CELL_DIV; dz <= SR_CELL_DIV; ++dz) {
+      const int zc = cz + dz;
+      if (zc < 0 || zc >= P.mz) continue;
+      for (int dy = -SR_CELL_DIV; dy <= SR_CELL_DIV; ++dy) {
+        const int yc = cy + dy;
+        if (yc < 0 || yc >= P.my) continue;
+        const int row = P.mx * (yc + P.my * zc);
+        const int js = sr_start[row + xlo], je = sr_start[row + xhi + 1];
+        for (int jb = js; jb < je; jb += 64, ++blk) {
+          if ((blk & 3) != wave) continue;
+          const int jend = min(jb + 64, je);
+          for (int j = jb; j < jend; ++j) {
+            const float4 pj = sr_pm[j];
+            const float dx = pi.x - pj.x, dy_ = pi.y - pj.y, dz_ = pi.z - pj.z;
+            const float r2 = dx * dx + dy_ * dy_ + dz_ * dz_;
+            float F;
+            if constexpr (TABLE) {
+              const float ksi = fminf(r2 * P.inv_d2, 1.0e9f);  // NaN -> 1e9: index stays defined
+              int t = (int)ksi;
+              t = min(t, SR_TABLE_REF - 2);
+              const float2 tv = table[t];
+              F = fmaf(ksi - (float)t, tv.y, tv.x);
+              F = r2 < P.rc2 ? F * pj.w : 0.0f;
+            } else {
+              const float r = sqrtf(r2);
+              const float R = reference_force(P.cs, r, P.a);
+              F = (-kGc / (r2 + P.eps2) + R) / r;
+              F = (r2 < P.rc2 && r2 > 0.0f) ? F * pj.w : 0.0f;
+            }
+            if constexpr (COUNT) pairs += (valid && r2 < P.rc2 && r2 > 0.0f) ? 1ull : 0ull;
+            ax = fmaf(F, dx, ax);
+            ay = fmaf(F, dy_, ay);
+            az = fmaf(F, dz_, az);
+          }
+        }
+      }
+    }
+    part[threadIdx.x] = make_float3(ax, ay, az);
+    __syncthreads();
+    if (wave == 0 && valid && !COUNT) {
+      float3 s = part[lane];
+      for (int q = 1; q < 4; ++q) {
+        const float3 o = part[lane + 64 * q];
+        s.x += o.x;
+        s.y += o.y;
+        s.z += o.z;
+      }
+      const int idx = sr_idx[i];
+      float4 a = acc[idx];
+      a.x += s.x;
+      a.y += s.y;
+      a.z += s.z;
+      if (mode & MODE_ACC) acc[idx] = a;
+      if (mode & (MODE_KICK | MODE_DRIFT)) {
+        float4 v = vel[idx];
+        float4 p = pm[idx];
+        if (mode & MODE_KICK) {
+          v.x += kick * a.x;
+          v.y += kick * a.y;
+          v.z += kick * a.z;
+        }
+        if (mode & MODE_DRIFT) {
+          p.x += v.x;
+          p.y += v.y;
+          p.z += v.z;
+          bool inside = inside_box(p.x, p.y, p.z, box);
+          if (!inside) atomicMin(escape, iteration + 1);
+          pm[idx] = p;
+        }
+        vel[idx] = v;
+      }
+    }
+  }
+  if constexpr (COUNT) {
+    for (int o = 32; o > 0; o >>= 1) pairs += __shfl_down(pairs, o, 64);
+    if (lane == 0) atomicAdd(pair_count, pairs);
+  }
+}
+
+}  // namespace
+
+// P3MMethod::initSRForceTable (p3mMethod.cpp:275-294) in FP32 on the host,
+// laid out as (T[t], T[t+1]-T[t]) pairs, padded to SR_TABLE entries.
+void sr_table_host(float rc, float a, float eps, int cs, std::vector<float>& table, float& d2) {
+  const int cnt = SR_TABLE_REF;
+  d2 = rc * rc / (float)(cnt - 1);
+  const float G = 1.0f / (4.0f * 3.14159265358979323846f);
+  std::vector<float> T(cnt);
+  for (int i = 0; i < cnt; ++i) {
+    float r2 = (float)i * d2;
+    float r = std::sqrt(r2);
+    float R;
+    if (cs == PMGPU_S1) {
+      if (r >= a) R = G / (r * r);
+      else {
+        float q = r / a;
+        R = G / (a * a) * (8.0f * r / a - 9.0f * r * r / (a * a) + 2.0f * q * q * q * q);
+      }
+    } else {
+      float u = 2.0f * r / a;
+      float u2 = u * u, u3 = u2 * u, u4 = u2 * u2, u5 = u4 * u, u6 = u3 * u3;
+      if (u <= 1.0f)
+        R = G / (35.0f * a * a) * (224.0f * u - 224.0f * u3 + 70.0f * u4 + 48.0f * u5 - 21.0f * u6);
+      else if (u <= 2.0f)
+        R = G / (35.0f * a * a) *
+            (12.0f / u2 - 224.0f + 896.0f * u - 840.0f * u2 + 224.0f * u3 + 70.0f * u4 - 48.0f * u5 +
+             7.0f * u6);
+      else R = G / (r * r);
+    }
+    R = -R;
+    float total = -G / (r * r + eps * eps);
+    T[i] = r == 0.0f ? 0.0f : (total - R) / std::sqrt(r * r + eps * eps);
+  }
+  table.assign(2 * SR_TABLE, 0.0f);
+  for (int i = 0; i + 1 < cnt; ++i) {
+    table[2 * i] = T[i];
+    table[2 * i + 1] = T[i + 1] - T[i];
+  }
+}
+
+void p3m_setup(pmgpu_ctx* c) {
+  std::vector<float> tab;
+  float d2 = 0.0f;
+  sr_table_host(c->p3m.cutoff, c->p3m.particle_diameter, c->p3m.softening, c->p3m.cloud_shape, tab,
+                d2);
+  HIP_CHECK(hipMemcpy(c->sr_table, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+  c->sr_inv_d2 = (float)(1.0 / (double)d2);
+}
+
+static SrParams sr_params(pmgpu_ctx* c) {
+  SrParams P;
+  P.mx = c->mcx;
+  P.my = c->mcy;
+  P.mz = c->mcz;
+  P.inv_d2 = c->sr_inv_d2;
+  P.rc2 = c->p3m.cutoff * c->p3m.cutoff;
+  P.a = c->p3m.particle_diameter;
+  P.eps2 = c->p3m.softening * c->p3m.softening;
+  P.cs = c->p3m.cloud_shape;
+  return P;
+}
+
+static void sr_sort(pmgpu_ctx* c, int iteration) {
+  int tok = stage_begin(c, "sr_sort");
+  zero_fill_i32(c, c->sr_count, c->ncell_sr);
+  unsigned g = (unsigned)((c->n + 255) / 256);
+  hipLaunchKernelGGL(k_sr_keys, dim3(g), dim3(256), 0, c->stream, c->pm[c->cur], c->n,
+                     1.0f / c->hcx, 1.0f / c->hcy, 1.0f / c->hcz, c->mcx, c->mcy, c->mcz, c->sr_key,
+                     c->sr_rank, c->sr_count, c->escape, iteration);
+  HIP_CHECK(hipGetLastError());
+  exclusive_scan_i32(c, c->sr_count, c->sr_start, c->ncell_sr);
+  hipLaunchKernelGGL(k_sr_scatter, dim3(g), dim3(256), 0, c->stream, c->pm[c->cur], c->n, c->sr_key,
+                     c->sr_rank, c->sr_start, c->sr_pm, c->sr_idx, c->escape, iteration);
+  HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_sr_plan, dim3((c->ncell_sr + 255) / 256), dim3(256), 0, c->stream,
+                     c->sr_count, c->ncell_sr, c->sr_nchunk, c->escape, iteration);
+  HIP_CHECK(hipGetLastError());
+  exclusive_scan_i32(c, c->sr_nchunk, c->sr_chunk_start, c->ncell_sr);
+  zero_fill_i32(c, c->counters + CNT_WORK, 1);
+  stage_end(c, tok);
+}
+
+template <bool TABLE, bool COUNT>
+static void launch_sr(pmgpu_ctx* c, int mode, float kick, int iteration) {
+  const int grid = c->num_cu * 8;
+  hipLaunchKernelGGL((k_sr<TABLE, COUNT>), dim3(grid), dim3(SR_THREADS), 0, c->stream, c->sr_pm,
+                     c->sr_start, c->sr_chunk_start, c->ncell_sr, c->sr_idx, c->sr_table,
+                     sr_params(c), c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick,
+                     make_float3(c->prm.box[0], c->prm.box[1], c->prm.box[2]), c->counters,
+                     c->pair_count, c->escape, iteration);
+  HIP_CHECK(hipGetLastError());
+}
+
+void p3m_short_range(pmgpu_ctx* c, int mode, float kick, int iteration) {
+  sr_sort(c, iteration);
+  int tok = stage_begin(c, "sr_pairs");
+  if (c->p3m.use_table) launch_sr<true, false>(c, mode, kick, iteration);
+  else launch_sr<false, false>(c, mode, kick, iteration);
+  stage_end(c, tok);
+}
+
+int64_t p3m_count_pairs(pmgpu_ctx* c) {
+  const int it = -2147483647 - 1;
+  sr_sort(c, it);
+  HIP_CHECK(hipMemsetAsync(c->pair_count, 0, sizeof(unsigned long long), c->stream));
+  if (c->p3m.use_table) launch_sr<true, true>(c, 0, 0.0f, it);
+  else launch_sr<false, true>(c, 0, 0.0f, it);
+  unsigned long long h = 0;
+  HIP_CHECK(hipMemcpyAsync(&h, c->pair_count, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  return (int64_t)h;
+}
+
+}  // namespace pmgpu

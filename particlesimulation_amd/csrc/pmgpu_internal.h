@@ -1,0 +1,183 @@
+// Internal declarations shared by the HIP translation units of libpmgpu.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pmgpu.h"
+
+namespace pmgpu {
+
+// ---------------------------------------------------------------------------
+// Mesh tiling used by the deposit and the interpolation.  A tile is
+// TILE_X x TILE_Y x TILE_Z cells; particles are sorted by (tile, boundary
+// class), the class recording per axis whether the anchor cell sits on the
+// tile's low face, inside, or on its high face.  A tile then finds the
+// particles of its 26 neighbours that reach into it as a few contiguous
+// class ranges.
+constexpr int TILE_X = 16;
+constexpr int TILE_Y = 8;
+constexpr int TILE_Z = 4;
+constexpr int TILE_CELLS = TILE_X * TILE_Y * TILE_Z;  // 512
+constexpr int NCLASS = 27;
+// Work-item size of the deposit (virtual particles per work item).
+constexpr int DEPOSIT_CHUNK = 4096;
+constexpr int INTERP_CHUNK = 2048;
+
+// P3M short range: SR cell side >= cutoff / SR_CELL_DIV, stencil +-SR_CELL_DIV.
+constexpr int SR_CELL_DIV = 2;
+constexpr int SR_TABLE = 512;  // padded force table (reference uses 500 entries)
+constexpr int SR_TABLE_REF = 500;
+
+struct Stage {
+  const char* name;
+  double total_ms = 0.0;
+  int64_t launches = 0;
+};
+
+struct Buf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+// Error carrying a status code; thrown inside the library only, converted to
+// a return code at the C ABI.
+struct Error {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg);
+void check_hip(hipError_t e, const char* what);
+#define HIP_CHECK(x) ::pmgpu::check_hip((x), #x)
+
+}  // namespace pmgpu
+
+struct pmgpu_ctx {
+  // configuration
+  pmgpu_pm_params prm{};
+  int nx = 0, ny = 0, nz = 0, nxh = 0;  // nxh = nx/2 + 1
+  int64_t M = 0, K = 0;                  // cells, half-spectrum modes
+  int ntx = 0, nty = 0, ntz = 0, ntiles = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // particles (sorted order, double buffered)
+  int64_t n = 0;
+  float4* pm[2] = {nullptr, nullptr};   // x, y, z, mass
+  float4* vel[2] = {nullptr, nullptr};  // vx, vy, vz, original index (int bits)
+  float4* acc = nullptr;                // ax, ay, az, -
+  float4* vint = nullptr;               // integer-step velocities (lazy)
+  int cur = 0;                          // which buffer holds the state
+  int32_t* key = nullptr;
+  int32_t* rank = nullptr;
+  double mass_abs_sum = 0.0;
+  int fx_shift = 0;                     // fixed-point scale 2^fx_shift
+  bool sorted_valid = false;            // tile sort matches current positions
+
+  // tile sort
+  int32_t* cls_count = nullptr;         // ntiles*27 (+1)
+  int32_t* cls_start = nullptr;         // ntiles*27 + 1
+  int32_t* tile_nchunk = nullptr;       // ntiles (+1)
+  int32_t* tile_chunk_start = nullptr;  // ntiles + 1
+  int32_t* tile_slot = nullptr;         // multi-chunk accumulation slot or -1
+  int32_t* slot_tile = nullptr;         // tile of each used slot
+  int32_t* int_nchunk = nullptr;        // interpolation work items per tile (+1)
+  int32_t* int_chunk_start = nullptr;   // ntiles + 1
+  int4* dep_table = nullptr;            // neighbour (dx,dy,dz,class) ranges of the scheme
+  int dep_table_len = 0;
+  int num_cu = 256;
+  int32_t* counters = nullptr;          // small device counters
+  long long* fx_accum = nullptr;        // multi-chunk tiles x 512 fixed-point sums
+  int64_t fx_slots = 0;
+  int32_t* scan_tmp = nullptr;
+  size_t scan_tmp_elems = 0;
+
+  // grids
+  float* rho = nullptr;      // M
+  float2* spec = nullptr;    // K
+  float* phi = nullptr;      // M
+  float* greens = nullptr;   // K (symmetrised, /M)
+  float* field = nullptr;    // 3M, only on request
+  float2* tw_x = nullptr;    // twiddles for nx/2
+  float2* tw_y = nullptr;
+  float2* tw_z = nullptr;
+  float2* tw_r2c = nullptr;  // exp(-2 pi i k / nx), k in [0, nx/2]
+  bool greens_ready = false;
+  std::vector<float> mlp_weights;  // learned correction (empty: analytic)
+
+  // P3M
+  bool p3m_ready = false;
+  pmgpu_p3m_params p3m{};
+  int mcx = 0, mcy = 0, mcz = 0, ncell_sr = 0;
+  float hcx = 0, hcy = 0, hcz = 0;
+  float2* sr_table = nullptr;           // SR_TABLE entries (T[t], T[t+1]-T[t])
+  float sr_inv_d2 = 0.0f;
+  int32_t* sr_count = nullptr;          // ncell_sr + 1
+  int32_t* sr_start = nullptr;          // ncell_sr + 1
+  int32_t* sr_key = nullptr;            // n
+  int32_t* sr_rank = nullptr;           // n
+  float4* sr_pm = nullptr;              // n, SR-cell order
+  int32_t* sr_idx = nullptr;            // n, index into the tile-sorted arrays
+  int32_t* sr_nchunk = nullptr;         // ncell_sr + 1
+  int32_t* sr_chunk_start = nullptr;    // ncell_sr + 1
+  unsigned long long* pair_count = nullptr;
+
+  // run loop state
+  int32_t* escape = nullptr;   // [0] = first iteration with an escaped particle, or INT_MAX
+  int iter = 0;                // tag of the kernels being enqueued (INT_MIN: standalone call)
+  int run_iter = 0;            // next run-loop iteration number
+
+  // profiling
+  bool profiling = false;
+  std::vector<pmgpu::Stage> stages;
+  std::vector<hipEvent_t> ev_pool;
+  struct Pending {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+};
+
+namespace pmgpu {
+
+// profiling helpers (api.hip)
+int stage_begin(pmgpu_ctx* c, const char* name);
+void stage_end(pmgpu_ctx* c, int token);
+
+// launchers (one per translation unit)
+void zero_fill_i32(pmgpu_ctx* c, int32_t* p, int64_t n);
+void exclusive_scan_i32(pmgpu_ctx* c, const int32_t* in, int32_t* out, int64_t n);
+
+void tile_sort(pmgpu_ctx* c);
+size_t scan_scratch_elems(int64_t n);
+std::vector<int4> deposit_range_table(int scheme);
+void deposit(pmgpu_ctx* c);
+void fft_forward_green_inverse(pmgpu_ctx* c);
+void greens_compute(pmgpu_ctx* c, const float* mlp_dev);
+void interpolate(pmgpu_ctx* c, int mode, float kick, int iteration);
+void field_grid(pmgpu_ctx* c);
+void leapfrog_op(pmgpu_ctx* c, int op, float dt, int iteration);
+void p3m_setup(pmgpu_ctx* c);
+void p3m_short_range(pmgpu_ctx* c, int mode, float kick, int iteration);
+int64_t p3m_count_pairs(pmgpu_ctx* c);
+float* mlp_correction_grid(pmgpu_ctx* c);  // device grid of MLP(k) on the half spectrum (caller frees)
+void escape_check(pmgpu_ctx* c, int32_t* flag_dev);
+void sr_table_host(float rc, float a, float eps, int cs, std::vector<float>& table, float& delta2);
+
+// escapedComputationalBox (source/pmMethod.cpp:146-156): inclusive [0, box];
+// an axis with box <= 0 is not tested (periodic use of the mesh).
+__device__ __forceinline__ bool inside_box(float x, float y, float z, float3 box) {
+  return (box.x <= 0.f || (x >= 0.f && x <= box.x)) && (box.y <= 0.f || (y >= 0.f && y <= box.y)) &&
+         (box.z <= 0.f || (z >= 0.f && z <= box.z));
+}
+
+// interpolate/p3m modes
+constexpr int MODE_ACC = 1;     // write acceleration
+constexpr int MODE_KICK = 2;    // v += kick * a
+constexpr int MODE_DRIFT = 4;   // x += v after the kick (fused next drift)
+
+}  // namespace pmgpu
