@@ -29,6 +29,9 @@ static void dalloc(T** p, int64_t count) {
   if (count <= 0) count = 1;
   HIP_CHECK(hipMalloc((void**)p, sizeof(T) * (size_t)count));
   HIP_CHECK(hipMemset(*p, 0, sizeof(T) * (size_t)count));
+  // the context stream is non-blocking: it does not wait for null-stream
+  // work, so the clear must finish before any kernel can touch the buffer
+  HIP_CHECK(hipDeviceSynchronize());
 }
 
 template <typename T>
