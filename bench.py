@@ -1,0 +1,226 @@
+"""Throughput of the MI355X P3M / PM force step (particle-updates/s).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+One step = one iteration of the reference run loop (source/pmMethod.cpp:
+79-122, source/p3mMethod.cpp:93-154) without I/O: drift, force (deposit,
+FFT x G, field, interpolation, short range for P3M), kick.  Inputs are
+resident in HBM before timing starts; the timed region is exactly K steps
+between two stream synchronisations.  Prints one JSON line on stdout.
+
+The line also carries
+  roofline      the dominant kernel's achieved rate vs the MI355X peak,
+                measured with HIP events on the library's stream,
+  cpu_baseline  the CPU restatement of the reference (oracle/, OpenMP) timed
+                on a bounded sample of the same workload on this host,
+  force_error   RMS and mean relative error of the GPU force against an FP64
+                direct sum on a seeded subsample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-updates/s, P3M 1M on 128^3 at fixed force err; learned vs plain"
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector (with packed FP32)
+PEAK_HBM_GBS = 8000.0
+SR_FLOP_PER_PAIR = 20      # per ordered pair within the cutoff
+GC = 1.0 / (4.0 * np.pi)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def load():
+    import particlesimulation_amd as P
+    if not os.path.exists(P.LIB_PATH):
+        from particlesimulation_amd import build as B
+        B.build_all()
+    P.load_library()
+    return P
+
+
+def make_context(P, w, pos, vel, m):
+    box = (0, 0, 0) if w.periodic else w.grid
+    pm = P.PM(w.grid, interpolation=w.interpolation, finite_diff=w.finite_diff,
+              greens_function=w.greens_function, particle_diameter=w.a_cells, box=box)
+    pm.set_particles(pos, vel, m)
+    pm.init_greens()
+    if w.p3m:
+        pm.p3m_configure(cutoff=w.rc_cells, particle_diameter=w.a_cells, softening=w.softening,
+                         cloud_shape=w.cloud_shape, use_table=True, box=w.grid)
+    return pm
+
+
+def force_error(P, w, pm, pos, m, n_targets=512, seed=7):
+    """e_rms = sqrt(sum|F - F_ds|^2 / sum|F_ds|^2) and the reference's mean relative
+    error (script/pm_accuracy.py:44-48) against an FP64 direct sum (Newtonian,
+    non-periodic) over all particles, on a seeded subsample of targets."""
+    pm.pm_step()
+    if w.p3m:
+        pm.p3m_short_range()
+    _, _, acc = pm.get_particles()
+    rng = np.random.default_rng(seed)
+    t = rng.choice(len(m), size=min(n_targets, len(m)), replace=False)
+    x = pos.astype(np.float64)
+    mm = m.astype(np.float64)
+    ref = np.zeros((len(t), 3))
+    for k, i in enumerate(t):
+        d = x[i] - x
+        r2 = (d * d).sum(1)
+        r2[i] = np.inf
+        ref[k] = -GC * ((mm / (r2 * np.sqrt(r2)))[:, None] * d).sum(0)
+    diff = acc[t].astype(np.float64) - ref
+    e_rms = float(np.sqrt((diff ** 2).sum() / (ref ** 2).sum()))
+    mean_rel = float(np.mean(np.linalg.norm(diff, axis=1) / (np.linalg.norm(ref, axis=1) + 1e-30)))
+    return {"e_rms": e_rms, "mean_rel": mean_rel, "targets": int(len(t)), "reference": "fp64 direct sum"}
+
+
+def cpu_baseline(w, pos, vel, m, pairs_total, budget_s=15.0):
+    """The oracle (CPU restatement of the reference path) on this host."""
+    from oracle import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    O.set_num_threads(threads)
+    grid = w.grid
+    t0 = time.perf_counter()
+    G = O.green_grid(w.greens_function, w.interpolation, w.finite_diff, w.a_cells, grid)
+    t_green = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.pm_accel(pos, m, grid, w.interpolation, w.finite_diff, G, wrap=True)
+    t_pm = time.perf_counter() - t0
+    t_sr, sample = 0.0, "full PM step"
+    if w.p3m:
+        H = w.H
+        box_orig = np.array([w.box] * 3, np.float32)
+        dims = O.p3m_mesh_dims(box_orig, w.rc_cells * H)
+        ncell = dims[0] * dims[1] * dims[2]
+        rng = np.random.default_rng(11)
+        frac, dt, p_s, mask = 0.002, 0.0, 0, None
+        while True:
+            mask = (rng.random(ncell) < frac).astype(np.uint8)
+            t0 = time.perf_counter()
+            _, p_s = O.p3m_short_range(pos, m, box_orig, w.rc_cells * H, H, w.rc_cells, w.a_cells, w.softening,
+                                       w.cloud_shape, use_table=True, ysort=True, cell_mask=mask)
+            dt = time.perf_counter() - t0
+            if (dt > budget_s / 3 and p_s > 0) or frac >= 1.0:
+                break
+            frac = min(1.0, frac * (10.0 if p_s == 0 else max(2.0, (budget_s / 2) / max(dt, 1e-3))))
+        if p_s == 0:
+            raise RuntimeError("short-range sample found no pairs")
+        t_sr = dt * (pairs_total / max(p_s, 1))
+        sample = (f"full PM step ({t_pm:.2f}s) + short range on {mask.sum()} of {ncell} chaining cells "
+                  f"({p_s:.3e} of {pairs_total:.3e} ordered pairs, {dt:.2f}s), scaled by pair count")
+    t_step = t_pm + t_sr
+    return {"value": len(m) / t_step, "unit": "particle-updates/s", "cores": threads, "kind": "port",
+            "sample": sample, "s_per_step": t_step, "green_init_s": t_green}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="p3m_disk_1M_128")
+    ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by "
+              "slab/pencil decomposition, which this build does not run)", file=sys.stderr)
+        sys.exit(2)
+
+    P = load()
+    from particlesimulation_amd import ics
+    w = ics.WORKLOADS[args.workload]
+    log(f"[bench] workload {w.name}: n={w.n} grid={w.grid} p3m={w.p3m}")
+    t0 = time.perf_counter()
+    pos, vel, m = w.code_units(seed=args.seed)
+    log(f"[bench] initial conditions {time.perf_counter() - t0:.1f}s")
+
+    pm = make_context(P, w, pos, vel, m)
+    err = None if args.no_error else force_error(P, w, pm, pos, m)
+    if err:
+        log(f"[bench] force error vs direct sum: {err}")
+    pm.set_particles(pos, vel, m)
+    pm.run(args.warmup, initial=True, p3m=w.p3m)
+    pairs0 = pm.p3m_count_pairs() if w.p3m else 0
+    pm.profile(True)
+    pm.profile_reset()
+    pm.synchronize()
+    t0 = time.perf_counter()
+    pm.run(args.steps, initial=False, p3m=w.p3m, sync=False)
+    pm.synchronize()
+    t1 = time.perf_counter()
+    stages = pm.profile_read()
+    pm.profile(False)
+    done = pm.run(0, initial=False, p3m=w.p3m)  # iterations_done check: 0 requested
+    pairs1 = pm.p3m_count_pairs() if w.p3m else 0
+    pairs = 0.5 * (pairs0 + pairs1)
+    elapsed = t1 - t0
+    value = w.n * args.steps / elapsed
+    log(f"[bench] {args.steps} steps in {elapsed:.4f}s -> {value:.4e} upd/s; escaped={pm.escaped()}")
+
+    per_step = {k: v[0] / max(1, args.steps) for k, v in stages.items()}
+    if w.p3m:
+        ms, cnt = stages["sr_pairs"]
+        avg_s = ms / cnt * 1e-3
+        achieved = SR_FLOP_PER_PAIR * pairs / avg_s / 1e12
+        roofline = {"bound": "valu", "kernel": "k_sr (short-range pairs)", "achieved": achieved,
+                    "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                    "traffic": None, "avg_launch_ms": avg_s * 1e3, "ordered_pairs": pairs,
+                    "flop_per_pair": SR_FLOP_PER_PAIR}
+    else:
+        roofline = None
+    # algorithmic-byte rates of the bandwidth-bound stages (per launch)
+    nx, ny, nz = w.grid
+    M, K, n = nx * ny * nz, (nx // 2 + 1) * ny * nz, w.n
+    alg = {"deposit": 16 * n + 4 * M, "fft_x_r2c": 4 * M + 8 * K, "fft_y_fwd": 16 * K,
+           "fft_z_green": 20 * K, "fft_y_inv": 16 * K, "fft_x_c2r": 8 * K + 4 * M,
+           "interp": 4 * M + (16 + 16 + 16 + 16 + 16) * n}
+    hbm = {}
+    for k, b in alg.items():
+        if k in stages and stages[k][1]:
+            s = stages[k][0] / stages[k][1] * 1e-3
+            hbm[k] = {"ms": s * 1e3, "GB/s": b / s / 1e9, "frac": b / s / 1e9 / PEAK_HBM_GBS}
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(w, pos, vel, m, pairs if w.p3m else 0)
+            log(f"[bench] cpu baseline: {cpu}")
+        except Exception as e:  # the baseline must not hide the GPU measurement
+            log(f"[bench] cpu baseline failed: {e!r}")
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": w.name, "n": w.n, "grid": list(w.grid), "box": w.box,
+                   "interpolation": ["NGP", "CIC", "TSC"][w.interpolation],
+                   "finite_diff": ["TWO_POINT", "FOUR_POINT"][w.finite_diff],
+                   "greens_function": ["DISCRETE_LAPLACIAN", "S1_OPTIMAL", "S2_OPTIMAL", "POOR_MAN"][
+                       w.greens_function],
+                   "a_cells": w.a_cells, "cutoff_cells": w.rc_cells, "dt": w.dt, "seed": args.seed,
+                   "p3m": w.p3m},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "force_error": err,
+        "stage_ms_per_step": per_step,
+        "hbm_stages": hbm,
+        "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
+    }
+    print(json.dumps(out), flush=True)
+    _ = done
+
+
+if __name__ == "__main__":
+    main()
