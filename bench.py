@@ -57,7 +57,7 @@ def make_context(P, w, pos, vel, m):
     pm.init_greens()
     if w.p3m:
         pm.p3m_configure(cutoff=w.rc_cells, particle_diameter=w.a_cells, softening=w.softening,
-                         cloud_shape=w.cloud_shape, use_table=True, box=w.grid)
+                         cloud_shape=w.cloud_shape, use_table=w.sr_table, box=w.grid)
     return pm
 
 
@@ -109,7 +109,7 @@ def cpu_baseline(w, pos, vel, m, pairs_total, budget_s=15.0):
             mask = (rng.random(ncell) < frac).astype(np.uint8)
             t0 = time.perf_counter()
             _, p_s = O.p3m_short_range(pos, m, box_orig, w.rc_cells * H, H, w.rc_cells, w.a_cells, w.softening,
-                                       w.cloud_shape, use_table=True, ysort=True, cell_mask=mask)
+                                       w.cloud_shape, use_table=w.sr_table, ysort=True, cell_mask=mask)
             dt = time.perf_counter() - t0
             if (dt > budget_s / 3 and p_s > 0) or frac >= 1.0:
                 break
@@ -153,7 +153,7 @@ def main():
         log(f"[bench] force error vs direct sum: {err}")
     pm.set_particles(pos, vel, m)
     pm.run(args.warmup, initial=True, p3m=w.p3m)
-    pairs0 = pm.p3m_count_pairs() if w.p3m else 0
+    pairs0, tested0 = pm.p3m_count_pairs(tested=True) if w.p3m else (0, 0)
     pm.profile(True)
     pm.profile_reset()
     pm.synchronize()
@@ -164,8 +164,9 @@ def main():
     stages = pm.profile_read()
     pm.profile(False)
     done = pm.run(0, initial=False, p3m=w.p3m)  # iterations_done check: 0 requested
-    pairs1 = pm.p3m_count_pairs() if w.p3m else 0
+    pairs1, tested1 = pm.p3m_count_pairs(tested=True) if w.p3m else (0, 0)
     pairs = 0.5 * (pairs0 + pairs1)
+    tested = 0.5 * (tested0 + tested1)
     elapsed = t1 - t0
     value = w.n * args.steps / elapsed
     log(f"[bench] {args.steps} steps in {elapsed:.4f}s -> {value:.4e} upd/s; escaped={pm.escaped()}")
@@ -178,7 +179,8 @@ def main():
         roofline = {"bound": "valu", "kernel": "k_sr (short-range pairs)", "achieved": achieved,
                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                     "traffic": None, "avg_launch_ms": avg_s * 1e3, "ordered_pairs": pairs,
-                    "flop_per_pair": SR_FLOP_PER_PAIR}
+                    "tested_pairs": tested, "flop_per_pair": SR_FLOP_PER_PAIR,
+                    "sr_table_mode": w.sr_table}
     else:
         roofline = None
     # algorithmic-byte rates of the bandwidth-bound stages (per launch)

@@ -94,6 +94,12 @@ typedef struct pmgpu_pm_params {
  * cloud shape, useSRForceTable, and the computational box of the chaining
  * mesh.  enableYSorting only changes the reference's summation order and has
  * no counterpart here.
+ * use_table: 0 = the direct formula (shortRangeForce, p3mMethod.cpp:220-238),
+ *            1 = the reference's 500-entry table (useSRForceTable = true),
+ *            2 = an accurate 2048-entry table of T r (r^2 + eps^2), which
+ *                follows the direct formula to ~1e-5 (an extension: the
+ *                reference's table interpolates T ~ r^-3 linearly in r^2 and
+ *                is coarse for close pairs).
  */
 typedef struct pmgpu_p3m_params {
   float cutoff;
@@ -192,9 +198,11 @@ int pmgpu_profile_enable(pmgpu_ctx* ctx, int32_t on);
 int pmgpu_profile_reset(pmgpu_ctx* ctx);
 int pmgpu_profile_read(pmgpu_ctx* ctx, int32_t max, const char** names, double* total_ms,
                        int64_t* launches, int32_t* count);
-/* Ordered pairs with r < cutoff in the current state (not on the step path:
-   used to express the short-range kernel's rate in algorithmic FLOP/s). */
-int pmgpu_p3m_count_pairs(pmgpu_ctx* ctx, int64_t* pairs);
+/* Ordered pairs with 0 < r < cutoff in the current state, and (tested, may be
+   NULL) all candidate pairs the short-range pass visits.  Not on the step
+   path: used to express the short-range kernel's rate in algorithmic
+   FLOP/s. */
+int pmgpu_p3m_count_pairs(pmgpu_ctx* ctx, int64_t* pairs, int64_t* tested);
 
 #ifdef __cplusplus
 }

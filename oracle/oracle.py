@@ -69,6 +69,8 @@ def lib():
         L.orc_reference_force.restype = C.c_float
         L.orc_sr_table.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, _f32p,
                                    C.POINTER(C.c_float)]
+        L.orc_sr_table_accurate.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, _f32p,
+                                            C.POINTER(C.c_float)]
         L.orc_p3m_sr.argtypes = [C.c_int64, _f32p, _f32p, _f32p, C.c_float, C.c_float, C.c_float, C.c_float,
                                  C.c_float, C.c_int, C.c_int, C.c_int, C.c_void_p, _f32p,
                                  C.POINTER(C.c_int64)]
@@ -207,6 +209,13 @@ def sr_table(rc, a, eps, cs, cnt=500):
     return t, d2.value
 
 
+def sr_table_accurate(rc, a, eps, cs, cnt=500):
+    t = np.zeros(cnt, np.float32)
+    d2 = C.c_float(0.0)
+    lib().orc_sr_table_accurate(float(rc), float(a), float(eps), cs, cnt, t, C.byref(d2))
+    return t, d2.value
+
+
 def p3m_mesh_dims(box_orig, rc_orig):
     out = np.zeros(3, np.int32)
     lib().orc_p3m_mesh_dims(np.asarray(box_orig, np.float32), float(rc_orig), out)
@@ -216,7 +225,8 @@ def p3m_mesh_dims(box_orig, rc_orig):
 def p3m_short_range(pos, mass, box_orig, rc_orig, H, rc, a, eps, cs, use_table=True, ysort=True,
                     cell_mask=None):
     """P3MMethod::calculateShortRangeForces + correctAccelerations
-    (source/p3mMethod.cpp:168-192, 50-57) -> (acc_sr (n,3) float32, ordered pairs)."""
+    (source/p3mMethod.cpp:168-192, 50-57) -> (acc_sr (n,3) float32, ordered pairs).
+    use_table: 0/False direct formula, 1/True reference table, 2 accurate table."""
     pos = _pos(pos)
     mass = np.ascontiguousarray(mass, np.float32)
     acc = np.zeros(pos.size, np.float32)

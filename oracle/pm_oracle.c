@@ -598,6 +598,46 @@ int orc_sr_table(float rc, float a, float eps, int cs, int cnt, float* table, fl
   return 0;
 }
 
+/* referenceForceS1/S2 (p3mMethod.cpp:194-218) in double. */
+static double ref_force_d(int cs, double r, double a) {
+  const double G = 1.0 / (4.0 * kPi);
+  if (cs == ORC_S1) {
+    if (r >= a) return G / (r * r);
+    double q = r / a;
+    return G / (a * a) * (8.0 * q - 9.0 * q * q + 2.0 * q * q * q * q);
+  }
+  double u = 2.0 * r / a, u2 = u * u, u3 = u2 * u, u4 = u2 * u2, u5 = u4 * u, u6 = u3 * u3;
+  if (u <= 1.0) return G / (35.0 * a * a) * (224.0 * u - 224.0 * u3 + 70.0 * u4 + 48.0 * u5 - 21.0 * u6);
+  if (u <= 2.0)
+    return G / (35.0 * a * a) *
+           (12.0 / u2 - 224.0 + 896.0 * u - 840.0 * u2 + 224.0 * u3 + 70.0 * u4 - 48.0 * u5 + 7.0 * u6);
+  return G / (r * r);
+}
+
+/*
+ * Accurate short-range table (an extension; use_table == 2): the direct
+ * formula's pair factor T(s) = (-G/(s+eps^2) + R(sqrt s))/sqrt s of
+ * shortRangeForce (p3mMethod.cpp:220-238) times sqrt(s) (s + eps^2),
+ * tabulated on the reference's grid s_i = i*rc^2/(cnt-1).  The product
+ * g = -G + R (s + eps^2) is smooth at s = 0, where T itself diverges (like
+ * s^(-3/2) for eps = 0) and the reference's table (initSRForceTable)
+ * interpolates poorly; the force is
+ * F = m_i m_j lerp(g, s) / (sqrt(s) (s + eps^2)) r_ij.
+ */
+int orc_sr_table_accurate(float rc, float a, float eps, int cs, int cnt, float* table, float* delta2) {
+  float d2 = rc * rc / (float)(cnt - 1);
+  const double G = 1.0 / (4.0 * kPi);
+  for (int i = 0; i < cnt; ++i) {
+    double s = (double)i * (double)d2;
+    double e2 = (double)eps * eps;
+    table[i] = (float)(-G + ref_force_d(cs, sqrt(s), a) * (s + e2));
+  }
+  *delta2 = d2;
+  return 0;
+}
+
+#define ORC_ACCURATE_TABLE 2048
+
 /* P3MMethod::shortRangeForce, p3mMethod.cpp:220-238 (direct formula). */
 static inline void sr_force_direct(const float rij[3], float mi, float mj, int cs, float a,
                                    float eps, float out[3]) {
@@ -658,6 +698,8 @@ static int cmp_yid(const void* pa, const void* pb) {
  * list is in descending index order (head insertion, chainingMesh.cpp:45-58).
  * Every (particle, slot) accumulator has a single writer cell, so the cell
  * loop runs on OpenMP threads exactly as the reference's std::thread pool.
+ * use_table: 0 direct formula, 1 the reference's table, 2 the accurate table
+ * of orc_sr_table_accurate (an extension, same chaining and accumulation).
  * acc_out (n x 3) receives (sum of slots + own) / m.  cell_mask (one byte per
  * chaining cell, NULL = all) restricts the pass to the marked cells; a
  * particle's result is then complete only if all 27 cells around it are
@@ -730,8 +772,10 @@ int orc_p3m_sr(int64_t n, const float* pos, const float* mass, const float box_o
   }
   if (cell_mask) memcpy(active, cell_mask, (size_t)ncell);
   else memset(active, 1, (size_t)ncell);
-  float table[500], d2 = 0.0f;
-  orc_sr_table(rc, a, eps, cs, 500, table, &d2);
+  float table[ORC_ACCURATE_TABLE], d2 = 0.0f;
+  const int tmax = (use_table == 2 ? ORC_ACCURATE_TABLE : 500) - 2;
+  if (use_table == 2) orc_sr_table_accurate(rc, a, eps, cs, ORC_ACCURATE_TABLE, table, &d2);
+  else orc_sr_table(rc, a, eps, cs, 500, table, &d2);
   const float rc2 = rc * rc;
   int64_t pairs = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : pairs)
@@ -758,6 +802,16 @@ int orc_p3m_sr(int64_t n, const float* pos, const float* mass, const float box_o
           float mi = mass[i], mj = mass[j];
           if (!use_table) {
             sr_force_direct(rij, mi, mj, cs, a, eps, f);
+          } else if (use_table == 2) {
+            float ksi = r2 / d2;
+            int t = (int)ksi;
+            if (t > tmax) t = tmax;
+            float g = table[t] + (ksi - t) * (table[t + 1] - table[t]);
+            float q = 1.0f / (sqrtf(r2) * (r2 + eps * eps));
+            float F = mi * mj * g * q;
+            f[0] = F * rij[0];
+            f[1] = F * rij[1];
+            f[2] = F * rij[2];
           } else {
             /* shortRangeForceFromTable, p3mMethod.cpp:240-245; the index is
                clamped to the table (t+1 <= cnt-1), where the reference reads

@@ -127,7 +127,7 @@ def load_library(path: str = LIB_PATH):
     L.pmgpu_profile_reset.argtypes = [vp]
     L.pmgpu_profile_read.argtypes = [vp, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
                                      C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
-    L.pmgpu_p3m_count_pairs.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.pmgpu_p3m_count_pairs.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     _lib = L
     return L
 
@@ -224,13 +224,14 @@ class PM:
     def pm_step(self):
         self._check(self._lib.pmgpu_pm_step(self._ctx))
 
-    def p3m_configure(self, cutoff, particle_diameter, softening=0.0, cloud_shape=S2, use_table=True, box=None):
+    def p3m_configure(self, cutoff, particle_diameter, softening=0.0, cloud_shape=S2, use_table=1, box=None):
+        """use_table: 0 direct formula, 1 (True) the reference's table, 2 the accurate table."""
         q = P3MParams()
         q.cutoff = float(cutoff)
         q.particle_diameter = float(particle_diameter)
         q.softening = float(softening)
         q.cloud_shape = int(cloud_shape)
-        q.use_table = int(bool(use_table))
+        q.use_table = int(use_table)
         box = self.params.box[:] if box is None else box
         q.box[:] = [float(b) for b in box]
         self._check(self._lib.pmgpu_p3m_configure(self._ctx, C.byref(q)))
@@ -238,10 +239,11 @@ class PM:
     def p3m_short_range(self):
         self._check(self._lib.pmgpu_p3m_short_range(self._ctx))
 
-    def p3m_count_pairs(self):
+    def p3m_count_pairs(self, tested=False):
         v = C.c_int64(0)
-        self._check(self._lib.pmgpu_p3m_count_pairs(self._ctx, C.byref(v)))
-        return v.value
+        t = C.c_int64(0)
+        self._check(self._lib.pmgpu_p3m_count_pairs(self._ctx, C.byref(v), C.byref(t)))
+        return (v.value, t.value) if tested else v.value
 
     def leapfrog(self, op, dt=1.0):
         self._check(self._lib.pmgpu_leapfrog(self._ctx, int(op), float(dt)))

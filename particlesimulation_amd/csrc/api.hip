@@ -419,6 +419,7 @@ int pmgpu_p3m_configure(pmgpu_ctx* c, const pmgpu_p3m_params* p) {
   if (!(p->cutoff > 0.0f)) fail(PMGPU_ERR_INVALID_ARGUMENT, "cutoff must be > 0");
   if (p->cloud_shape != PMGPU_S1 && p->cloud_shape != PMGPU_S2)
     fail(PMGPU_ERR_INVALID_ARGUMENT, "not implemented");
+  if (p->use_table < 0 || p->use_table > 2) fail(PMGPU_ERR_INVALID_ARGUMENT, "use_table must be 0, 1 or 2");
   if (!(p->particle_diameter > 0.0f)) fail(PMGPU_ERR_INVALID_ARGUMENT, "particle_diameter must be > 0");
   for (int i = 0; i < 3; ++i)
     if (!(p->box[i] > 0.0f)) fail(PMGPU_ERR_INVALID_ARGUMENT, "box must be > 0");
@@ -441,7 +442,7 @@ int pmgpu_p3m_configure(pmgpu_ctx* c, const pmgpu_p3m_params* p) {
   dalloc(&c->sr_nchunk, c->ncell_sr + 1);
   dalloc(&c->sr_chunk_start, c->ncell_sr + 1);
   dalloc(&c->sr_table, SR_TABLE);
-  dalloc(&c->pair_count, 1);
+  dalloc(&c->pair_count, 2);
   ensure_scan_scratch(c, c->ncell_sr);
   c->p3m_ready = true;
   if (c->n > 0) {
@@ -463,12 +464,12 @@ int pmgpu_p3m_short_range(pmgpu_ctx* c) {
   PMGPU_API_END(c)
 }
 
-int pmgpu_p3m_count_pairs(pmgpu_ctx* c, int64_t* pairs) {
+int pmgpu_p3m_count_pairs(pmgpu_ctx* c, int64_t* pairs, int64_t* tested) {
   PMGPU_API_BEGIN(c)
   require_particles(c);
   if (!c->p3m_ready) fail(PMGPU_ERR_NOT_READY, "call pmgpu_p3m_configure first");
   if (!pairs) fail(PMGPU_ERR_INVALID_ARGUMENT, "pairs is NULL");
-  *pairs = p3m_count_pairs(c);
+  p3m_count_pairs(c, pairs, tested);
   PMGPU_API_END(c)
 }
 

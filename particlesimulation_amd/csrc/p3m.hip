@@ -13,11 +13,8 @@
 //    waves of a workgroup take interleaved 64-particle blocks of that stream
 //    and read each j through the scalar unit (uniform address), so the inner
 //    loop is pure FP32 VALU with scalar operands;
-//  - the four partial sums are added in LDS in fixed wave order, and the
-//    pair force comes from the reference's 500-entry table in LDS
-//    (initSRForceTable, p3mMethod.cpp:275-294; lerp of
-//    shortRangeForceFromTable, p3mMethod.cpp:240-245) or its direct formula
-//    (shortRangeForce, p3mMethod.cpp:220-238).
+//  - the four partial sums are added in LDS in fixed wave order; the pair
+//    force comes from a table in LDS or from the direct formula.
 // Newton's third law is not used: every ordered pair is evaluated once on
 // the side of its i, which removes all write conflicts.  The mass m_i that
 // the reference multiplies into the pair force and divides out again in
@@ -49,11 +46,14 @@ __global__ void k_sr_keys(const float4* __restrict__ pm, int64_t n, float ihx, f
                           const int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float4 p = pm[i];
+  const bool active = i < n;
+  float4 p = active ? pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
   int k = sr_cell_1d(p.x, ihx, mx) + mx * (sr_cell_1d(p.y, ihy, my) + my * sr_cell_1d(p.z, ihz, mz));
-  key[i] = k;
-  rank[i] = atomicAdd(&count[k], 1);
+  const int32_t r = wave_rank(count, k, active);
+  if (active) {
+    key[i] = k;
+    rank[i] = r;
+  }
 }
 
 __global__ void k_sr_scatter(const float4* __restrict__ pm, int64_t n, const int32_t* __restrict__ key,
@@ -109,25 +109,57 @@ struct SrParams {
   float inv_d2, rc2;
   float a, eps2;
   int cs;
+  float tmax_f;  // last table index; entries past the cutoff are zero
 };
 
-template <bool TABLE, bool COUNT>
+// Pair factor F(r^2) such that a_i += m_j F (x_i - x_j).
+//   MODE 0: shortRangeForce (p3mMethod.cpp:220-238), direct formula;
+//   MODE 1: shortRangeForceFromTable (p3mMethod.cpp:240-245) on the
+//           reference's 500-entry table, zero-padded past the cutoff so the
+//           r < rc test is the table range itself;
+//   MODE 2: the accurate table g = T r (s + eps^2), F = g / (r (s + eps^2)).
+template <int MODE, bool SOFT>
+__device__ __forceinline__ float pair_factor(float r2, float mj, const float2* table,
+                                             const SrParams& P) {
+  if constexpr (MODE == 0) {
+    const float r = sqrtf(r2);
+    const float R = reference_force(P.cs, r, P.a);
+    const float F = (-kGc / (r2 + P.eps2) + R) / r;
+    return (r2 < P.rc2 && r2 > 0.0f) ? F * mj : 0.0f;
+  } else {
+    // t = min(floor(ksi), tmax); a NaN r^2 also lands on the zero entry
+    // tmax.  Past the cutoff every entry is (0, 0), so the fraction of an
+    // out-of-range ksi is irrelevant.
+    const float ksi = r2 * P.inv_d2;
+    const int t = (int)fminf(ksi, P.tmax_f);
+    const float2 tv = table[t];
+    float F = fmaf(__builtin_amdgcn_fractf(ksi), tv.y, tv.x) * mj;
+    if constexpr (MODE == 2) {
+      const float q = __frsqrt_rn(fmaxf(r2, 1.0e-12f));  // self pair: finite F times d = 0
+      if constexpr (SOFT) F *= q * __frcp_rn(r2 + P.eps2);
+      else F *= q * q * q;
+    }
+    return F;
+  }
+}
+
+template <int MODE, bool SOFT>
 __global__ __launch_bounds__(SR_THREADS) void k_sr(
     const float4* __restrict__ sr_pm, const int32_t* __restrict__ sr_start,
     const int32_t* __restrict__ chunk_start, int ncell, const int32_t* __restrict__ sr_idx,
     const float2* __restrict__ table_g, SrParams P, float4* __restrict__ pm,
     float4* __restrict__ vel, float4* __restrict__ acc, int mode, float kick, float3 box,
-    int32_t* __restrict__ counters, unsigned long long* __restrict__ pair_count,
-    int32_t* __restrict__ escape, int iteration) {
+    int32_t* __restrict__ counters, int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
-  __shared__ float2 table[SR_TABLE];
+  constexpr int TS = MODE == 2 ? SR_TABLE_ACC : SR_TABLE;
+  __shared__ float2 table[MODE == 0 ? 1 : TS];
   __shared__ float3 part[SR_THREADS];
   __shared__ int s_w;
-  for (int i = threadIdx.x; i < SR_TABLE; i += SR_THREADS) table[i] = table_g[i];
+  if constexpr (MODE != 0)
+    for (int i = threadIdx.x; i < TS; i += SR_THREADS) table[i] = table_g[i];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int total = chunk_start[ncell];
-  unsigned long long pairs = 0;
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) s_w = atomicAdd(&counters[CNT_WORK], 1);
@@ -154,25 +186,12 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
         for (int jb = js; jb < je; jb += 64, ++blk) {
           if ((blk & 3) != wave) continue;
           const int jend = min(jb + 64, je);
+#pragma unroll 4
           for (int j = jb; j < jend; ++j) {
-            const float4 pj = sr_pm[j];
+            const float4 pj = sr_pm[j];  // uniform address: scalar load
             const float dx = pi.x - pj.x, dy_ = pi.y - pj.y, dz_ = pi.z - pj.z;
-            const float r2 = dx * dx + dy_ * dy_ + dz_ * dz_;
-            float F;
-            if constexpr (TABLE) {
-              const float ksi = fminf(r2 * P.inv_d2, 1.0e9f);  // NaN -> 1e9: index stays defined
-              int t = (int)ksi;
-              t = min(t, SR_TABLE_REF - 2);
-              const float2 tv = table[t];
-              F = fmaf(ksi - (float)t, tv.y, tv.x);
-              F = r2 < P.rc2 ? F * pj.w : 0.0f;
-            } else {
-              const float r = sqrtf(r2);
-              const float R = reference_force(P.cs, r, P.a);
-              F = (-kGc / (r2 + P.eps2) + R) / r;
-              F = (r2 < P.rc2 && r2 > 0.0f) ? F * pj.w : 0.0f;
-            }
-            if constexpr (COUNT) pairs += (valid && r2 < P.rc2 && r2 > 0.0f) ? 1ull : 0ull;
+            const float r2 = fmaf(dz_, dz_, fmaf(dy_, dy_, dx * dx));
+            const float F = pair_factor<MODE, SOFT>(r2, pj.w, table, P);
             ax = fmaf(F, dx, ax);
             ay = fmaf(F, dy_, ay);
             az = fmaf(F, dz_, az);
@@ -182,7 +201,7 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
     }
     part[threadIdx.x] = make_float3(ax, ay, az);
     __syncthreads();
-    if (wave == 0 && valid && !COUNT) {
+    if (wave == 0 && valid) {
       float3 s = part[lane];
       for (int q = 1; q < 4; ++q) {
         const float3 o = part[lane + 64 * q];
@@ -208,24 +227,91 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
           p.x += v.x;
           p.y += v.y;
           p.z += v.z;
-          bool inside = inside_box(p.x, p.y, p.z, box);
-          if (!inside) atomicMin(escape, iteration + 1);
+          if (!inside_box(p.x, p.y, p.z, box)) atomicMin(escape, iteration + 1);
           pm[idx] = p;
         }
         vel[idx] = v;
       }
     }
   }
-  if constexpr (COUNT) {
-    for (int o = 32; o > 0; o >>= 1) pairs += __shfl_down(pairs, o, 64);
-    if (lane == 0) atomicAdd(pair_count, pairs);
+}
+
+// Ordered pairs with 0 < r^2 < rc^2 (accepted) and all (i, j) candidates the
+// pass visits (tested), over the same traversal as k_sr.
+__global__ __launch_bounds__(SR_THREADS) void k_sr_count(
+    const float4* __restrict__ sr_pm, const int32_t* __restrict__ sr_start,
+    const int32_t* __restrict__ chunk_start, int ncell, SrParams P,
+    int32_t* __restrict__ counters, unsigned long long* __restrict__ out) {
+  __shared__ int s_w;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int total = chunk_start[ncell];
+  unsigned long long accepted = 0, tested = 0;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_w = atomicAdd(&counters[CNT_WORK], 1);
+    __syncthreads();
+    const int w = s_w;
+    if (w >= total) break;
+    const int c = find_item(chunk_start, ncell, w);
+    const int k = w - chunk_start[c];
+    const int cx = c % P.mx, cy = (c / P.mx) % P.my, cz = c / (P.mx * P.my);
+    const int i = sr_start[c] + k * SR_ICHUNK + lane;
+    const bool valid = i < sr_start[c + 1];
+    const float4 pi = valid ? sr_pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    int blk = 0;
+    const int xlo = max(cx - SR_CELL_DIV, 0), xhi = min(cx + SR_CELL_DIV, P.mx - 1);
+    for (int dz = -SR_CELL_DIV; dz <= SR_CELL_DIV; ++dz) {
+      const int zc = cz + dz;
+      if (zc < 0 || zc >= P.mz) continue;
+      for (int dy = -SR_CELL_DIV; dy <= SR_CELL_DIV; ++dy) {
+        const int yc = cy + dy;
+        if (yc < 0 || yc >= P.my) continue;
+        const int row = P.mx * (yc + P.my * zc);
+        const int js = sr_start[row + xlo], je = sr_start[row + xhi + 1];
+        for (int jb = js; jb < je; jb += 64, ++blk) {
+          if ((blk & 3) != wave) continue;
+          const int jend = min(jb + 64, je);
+          if (valid) tested += (unsigned long long)(jend - jb);
+          for (int j = jb; j < jend; ++j) {
+            const float4 pj = sr_pm[j];
+            const float dx = pi.x - pj.x, dy_ = pi.y - pj.y, dz_ = pi.z - pj.z;
+            const float r2 = fmaf(dz_, dz_, fmaf(dy_, dy_, dx * dx));
+            accepted += (valid && r2 < P.rc2 && r2 > 0.0f) ? 1ull : 0ull;
+          }
+        }
+      }
+    }
   }
+  for (int o = 32; o > 0; o >>= 1) {
+    accepted += __shfl_down(accepted, o, 64);
+    tested += __shfl_down(tested, o, 64);
+  }
+  if (lane == 0) {
+    atomicAdd(&out[0], accepted);
+    atomicAdd(&out[1], tested);
+  }
+}
+
+double ref_force_d(int cs, double r, double a) {
+  const double G = 1.0 / (4.0 * M_PI);
+  if (cs == PMGPU_S1) {
+    if (r >= a) return G / (r * r);
+    double q = r / a;
+    return G / (a * a) * (8.0 * q - 9.0 * q * q + 2.0 * q * q * q * q);
+  }
+  double u = 2.0 * r / a, u2 = u * u, u3 = u2 * u, u4 = u2 * u2, u5 = u4 * u, u6 = u3 * u3;
+  if (u <= 1.0) return G / (35.0 * a * a) * (224.0 * u - 224.0 * u3 + 70.0 * u4 + 48.0 * u5 - 21.0 * u6);
+  if (u <= 2.0)
+    return G / (35.0 * a * a) *
+           (12.0 / u2 - 224.0 + 896.0 * u - 840.0 * u2 + 224.0 * u3 + 70.0 * u4 - 48.0 * u5 + 7.0 * u6);
+  return G / (r * r);
 }
 
 }  // namespace
 
 // P3MMethod::initSRForceTable (p3mMethod.cpp:275-294) in FP32 on the host,
-// laid out as (T[t], T[t+1]-T[t]) pairs, padded to SR_TABLE entries.
+// laid out as (T[t], T[t+1]-T[t]) pairs, zero past the last interval.
 void sr_table_host(float rc, float a, float eps, int cs, std::vector<float>& table, float& d2) {
   const int cnt = SR_TABLE_REF;
   d2 = rc * rc / (float)(cnt - 1);
@@ -263,11 +349,37 @@ void sr_table_host(float rc, float a, float eps, int cs, std::vector<float>& tab
   }
 }
 
+// The accurate table: g(s) = -G + R(sqrt s) (s + eps^2), the direct formula's
+// pair factor times sqrt(s) (s + eps^2), on SR_TABLE_ACC points in s.
+static void sr_table_accurate_host(float rc, float a, float eps, int cs, std::vector<float>& table,
+                                   float& d2) {
+  const int cnt = SR_TABLE_ACC;
+  d2 = rc * rc / (float)(cnt - 1);
+  const double G = 1.0 / (4.0 * M_PI);
+  std::vector<float> g(cnt);
+  for (int i = 0; i < cnt; ++i) {
+    double s = (double)i * (double)d2;
+    g[i] = (float)(-G + ref_force_d(cs, std::sqrt(s), a) * (s + (double)eps * eps));
+  }
+  table.assign(2 * SR_TABLE_ACC, 0.0f);
+  for (int i = 0; i + 1 < cnt; ++i) {
+    table[2 * i] = g[i];
+    table[2 * i + 1] = g[i + 1] - g[i];
+  }
+}
+
 void p3m_setup(pmgpu_ctx* c) {
   std::vector<float> tab;
   float d2 = 0.0f;
-  sr_table_host(c->p3m.cutoff, c->p3m.particle_diameter, c->p3m.softening, c->p3m.cloud_shape, tab,
-                d2);
+  if (c->p3m.use_table == 2) {
+    sr_table_accurate_host(c->p3m.cutoff, c->p3m.particle_diameter, c->p3m.softening,
+                           c->p3m.cloud_shape, tab, d2);
+    c->sr_tmax = SR_TABLE_ACC - 1;
+  } else {
+    sr_table_host(c->p3m.cutoff, c->p3m.particle_diameter, c->p3m.softening, c->p3m.cloud_shape,
+                  tab, d2);
+    c->sr_tmax = SR_TABLE - 1;
+  }
   HIP_CHECK(hipMemcpy(c->sr_table, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
   c->sr_inv_d2 = (float)(1.0 / (double)d2);
 }
@@ -282,6 +394,7 @@ static SrParams sr_params(pmgpu_ctx* c) {
   P.a = c->p3m.particle_diameter;
   P.eps2 = c->p3m.softening * c->p3m.softening;
   P.cs = c->p3m.cloud_shape;
+  P.tmax_f = (float)c->sr_tmax;
   return P;
 }
 
@@ -305,35 +418,44 @@ static void sr_sort(pmgpu_ctx* c, int iteration) {
   stage_end(c, tok);
 }
 
-template <bool TABLE, bool COUNT>
+template <int MODE, bool SOFT>
 static void launch_sr(pmgpu_ctx* c, int mode, float kick, int iteration) {
   const int grid = c->num_cu * 8;
-  hipLaunchKernelGGL((k_sr<TABLE, COUNT>), dim3(grid), dim3(SR_THREADS), 0, c->stream, c->sr_pm,
+  hipLaunchKernelGGL((k_sr<MODE, SOFT>), dim3(grid), dim3(SR_THREADS), 0, c->stream, c->sr_pm,
                      c->sr_start, c->sr_chunk_start, c->ncell_sr, c->sr_idx, c->sr_table,
                      sr_params(c), c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick,
                      make_float3(c->prm.box[0], c->prm.box[1], c->prm.box[2]), c->counters,
-                     c->pair_count, c->escape, iteration);
+                     c->escape, iteration);
   HIP_CHECK(hipGetLastError());
 }
 
 void p3m_short_range(pmgpu_ctx* c, int mode, float kick, int iteration) {
   sr_sort(c, iteration);
   int tok = stage_begin(c, "sr_pairs");
-  if (c->p3m.use_table) launch_sr<true, false>(c, mode, kick, iteration);
-  else launch_sr<false, false>(c, mode, kick, iteration);
+  const bool soft = c->p3m.softening != 0.0f;
+  switch (c->p3m.use_table) {
+    case 0: launch_sr<0, false>(c, mode, kick, iteration); break;
+    case 1: launch_sr<1, false>(c, mode, kick, iteration); break;
+    default:
+      if (soft) launch_sr<2, true>(c, mode, kick, iteration);
+      else launch_sr<2, false>(c, mode, kick, iteration);
+  }
   stage_end(c, tok);
 }
 
-int64_t p3m_count_pairs(pmgpu_ctx* c) {
+void p3m_count_pairs(pmgpu_ctx* c, int64_t* accepted, int64_t* tested) {
   const int it = -2147483647 - 1;
   sr_sort(c, it);
-  HIP_CHECK(hipMemsetAsync(c->pair_count, 0, sizeof(unsigned long long), c->stream));
-  if (c->p3m.use_table) launch_sr<true, true>(c, 0, 0.0f, it);
-  else launch_sr<false, true>(c, 0, 0.0f, it);
-  unsigned long long h = 0;
-  HIP_CHECK(hipMemcpyAsync(&h, c->pair_count, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipMemsetAsync(c->pair_count, 0, 2 * sizeof(unsigned long long), c->stream));
+  hipLaunchKernelGGL(k_sr_count, dim3(c->num_cu * 8), dim3(SR_THREADS), 0, c->stream, c->sr_pm,
+                     c->sr_start, c->sr_chunk_start, c->ncell_sr, sr_params(c), c->counters,
+                     c->pair_count);
+  HIP_CHECK(hipGetLastError());
+  unsigned long long h[2] = {0, 0};
+  HIP_CHECK(hipMemcpyAsync(h, c->pair_count, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
-  return (int64_t)h;
+  *accepted = (int64_t)h[0];
+  if (tested) *tested = (int64_t)h[1];
 }
 
 }  // namespace pmgpu

@@ -29,8 +29,9 @@ constexpr int INTERP_CHUNK = 2048;
 
 // P3M short range: SR cell side >= cutoff / SR_CELL_DIV, stencil +-SR_CELL_DIV.
 constexpr int SR_CELL_DIV = 2;
-constexpr int SR_TABLE = 512;  // padded force table (reference uses 500 entries)
+constexpr int SR_TABLE = 512;  // padded reference table (the reference uses 500 entries)
 constexpr int SR_TABLE_REF = 500;
+constexpr int SR_TABLE_ACC = 2048;  // accurate (r (s+eps^2)-scaled) table
 
 struct Stage {
   const char* name;
@@ -116,6 +117,7 @@ struct pmgpu_ctx {
   float hcx = 0, hcy = 0, hcz = 0;
   float2* sr_table = nullptr;           // SR_TABLE entries (T[t], T[t+1]-T[t])
   float sr_inv_d2 = 0.0f;
+  int sr_tmax = 0;
   int32_t* sr_count = nullptr;          // ncell_sr + 1
   int32_t* sr_start = nullptr;          // ncell_sr + 1
   int32_t* sr_key = nullptr;            // n
@@ -163,10 +165,31 @@ void field_grid(pmgpu_ctx* c);
 void leapfrog_op(pmgpu_ctx* c, int op, float dt, int iteration);
 void p3m_setup(pmgpu_ctx* c);
 void p3m_short_range(pmgpu_ctx* c, int mode, float kick, int iteration);
-int64_t p3m_count_pairs(pmgpu_ctx* c);
+void p3m_count_pairs(pmgpu_ctx* c, int64_t* accepted, int64_t* tested);
 float* mlp_correction_grid(pmgpu_ctx* c);  // device grid of MLP(k) on the half spectrum (caller frees)
 void escape_check(pmgpu_ctx* c, int32_t* flag_dev);
 void sr_table_host(float rc, float a, float eps, int cs, std::vector<float>& table, float& delta2);
+
+// Rank of this lane's key among all lanes of the grid that share it: one
+// integer atomic per distinct key per wavefront (lanes with equal keys are
+// ranked by lane order), so densely populated keys do not serialise 64
+// atomics on one address.
+__device__ __forceinline__ int32_t wave_rank(int32_t* count, int32_t key, bool active) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long pending = __ballot(active);
+  int32_t rank = 0;
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const int32_t lk = __shfl(key, leader);
+    const unsigned long long same = __ballot(active && key == lk) & pending;
+    int32_t base = 0;
+    if (lane == leader) base = atomicAdd(&count[lk], (int32_t)__popcll(same));
+    base = __shfl(base, leader);
+    if (active && key == lk) rank = base + (int32_t)__popcll(same & ((1ull << lane) - 1ull));
+    pending &= ~same;
+  }
+  return rank;
+}
 
 // escapedComputationalBox (source/pmMethod.cpp:146-156): inclusive [0, box];
 // an axis with box <= 0 is not tested (periodic use of the mesh).

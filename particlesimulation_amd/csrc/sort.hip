@@ -124,8 +124,8 @@ __global__ void k_tile_keys(const float4* __restrict__ pm, int64_t n, int nx, in
                             const int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float4 p = pm[i];
+  const bool active = i < n;
+  float4 p = active ? pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
   int ax = wrap_cell(anchor_of(p.x, scheme), nx);
   int ay = wrap_cell(anchor_of(p.y, scheme), ny);
   int az = wrap_cell(anchor_of(p.z, scheme), nz);
@@ -133,8 +133,11 @@ __global__ void k_tile_keys(const float4* __restrict__ pm, int64_t n, int nx, in
   int cls = face_class(ax - tx * TILE_X, TILE_X) + 3 * face_class(ay - ty * TILE_Y, TILE_Y) +
             9 * face_class(az - tz * TILE_Z, TILE_Z);
   int k = (tx + ntx * (ty + nty * tz)) * NCLASS + cls;
-  key[i] = k;
-  rank[i] = atomicAdd(&count[k], 1);
+  const int32_t r = wave_rank(count, k, active);
+  if (active) {
+    key[i] = k;
+    rank[i] = r;
+  }
 }
 
 __global__ void k_tile_scatter(const float4* __restrict__ pm_in, const float4* __restrict__ vel_in,

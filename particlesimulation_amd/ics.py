@@ -133,6 +133,7 @@ class Workload:
     rc_cells: float = 0.0      # short-range cutoff in grid cells
     softening: float = 0.0
     cloud_shape: int = 1       # S2
+    sr_table: int = 2          # short-range pair force: 0 direct, 1 reference table, 2 accurate table
     periodic: bool = False
     kind: str = "plummer"
     extra: dict = field(default_factory=dict)

@@ -160,7 +160,8 @@ def test_external_field_matches_oracle(gpu_lib, oracle):
 # P3M short range
 # --------------------------------------------------------------------------
 
-@pytest.mark.parametrize("cs,use_table,eps", [(1, True, 0.0), (0, True, 0.0), (1, False, 0.0), (1, True, 0.3)])
+@pytest.mark.parametrize("cs,use_table,eps", [(1, 1, 0.0), (0, 1, 0.0), (1, 0, 0.0), (1, 1, 0.3), (1, 2, 0.0),
+                                               (0, 2, 0.3)])
 def test_p3m_matches_oracle(gpu_lib, oracle, cs, use_table, eps):
     grid = (64, 64, 64)
     H = 0.25

@@ -401,6 +401,21 @@ def test_sr_table_matches_direct_formula_away_from_zero(oracle, cs):
     assert np.linalg.norm(table - direct) / np.linalg.norm(direct) < 3e-3
 
 
+@pytest.mark.parametrize("cs,eps", [(0, 0.0), (1, 0.0), (1, 0.4)])
+def test_sr_accurate_table_matches_brute_force(oracle, cs, eps):
+    # the r^3-scaled table follows the direct formula including close pairs
+    r = _rng(14)
+    pos = r.uniform(0.5, 11.5, (400, 3)).astype(np.float32)
+    m = r.uniform(0.5, 1.5, 400).astype(np.float32)
+    rc, a, H = 3.0, 2.5, 0.5
+    box = np.array([6.0, 6.0, 6.0], np.float32)
+    direct, _ = oracle.p3m_short_range(pos, m, box, rc * H, H, rc, a, eps, cs, use_table=0)
+    acc, _ = oracle.p3m_short_range(pos, m, box, rc * H, H, rc, a, eps, cs, use_table=2)
+    assert np.linalg.norm(acc - direct) / np.linalg.norm(direct) < (3e-5 if eps == 0.0 else 1e-3)
+    if eps == 0.0:
+        assert np.linalg.norm(direct - _brute_sr(pos, m, rc, a, cs)) / np.linalg.norm(direct) < 1e-4
+
+
 def test_sr_is_antisymmetric(oracle):
     r = _rng(11)
     pos = r.uniform(0.5, 11.5, (300, 3)).astype(np.float32)
