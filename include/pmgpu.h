@@ -176,6 +176,15 @@ int pmgpu_run(pmgpu_ctx* ctx, int32_t iterations, int32_t initial, int32_t p3m,
 /* escapedComputationalBox (pmMethod.cpp:151-156) of the current state. */
 int pmgpu_escaped(pmgpu_ctx* ctx, int32_t* escaped);
 
+/* GreenDiscreteLaplacian / GreenOptimal / GreenPoorMan
+   (source/greensFunctions.cpp:122-220) at one unfolded mode (kx, ky, kz) in
+   [0, N), evaluated on the host in FP64 with the device kernel's code (NaN
+   for invalid arguments).  Unlike the FP32 reference it is exactly 0 where
+   D(k) = 0 and uses a series where S1/S2Fourier cancel. */
+double pmgpu_greens_value(int32_t greens_function, int32_t interpolation, int32_t finite_diff,
+                          double particle_diameter, int32_t kx, int32_t ky, int32_t kz, int32_t nx,
+                          int32_t ny, int32_t nz);
+
 /* ---- grids (Grid accessors, include/grid.h:18-44) -------------------- */
 #define PMGPU_GRID_DENSITY 0   /* Nx*Ny*Nz floats, x fastest */
 #define PMGPU_GRID_POTENTIAL 1 /* Nx*Ny*Nz floats */

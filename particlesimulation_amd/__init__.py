@@ -49,6 +49,7 @@ EXPORTS = (
     "pmgpu_leapfrog", "pmgpu_get_integer_step_velocities", "pmgpu_run", "pmgpu_escaped",
     "pmgpu_get_grid", "pmgpu_set_greens_correction", "pmgpu_mlp_num_params",
     "pmgpu_profile_enable", "pmgpu_profile_reset", "pmgpu_profile_read", "pmgpu_p3m_count_pairs",
+    "pmgpu_greens_value",
 )
 
 
@@ -128,6 +129,8 @@ def load_library(path: str = LIB_PATH):
     L.pmgpu_profile_read.argtypes = [vp, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
                                      C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
     L.pmgpu_p3m_count_pairs.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.pmgpu_greens_value.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_double] + [C.c_int32] * 6
+    L.pmgpu_greens_value.restype = C.c_double
     _lib = L
     return L
 
@@ -305,6 +308,12 @@ class PM:
         k = C.c_int32(0)
         self._check(self._lib.pmgpu_profile_read(self._ctx, n, names, ms, cnt, C.byref(k)))
         return {names[i].decode(): (ms[i], cnt[i]) for i in range(k.value)}
+
+
+def greens_value(gf, scheme, fds, a, k, grid) -> float:
+    """G(k) of one unfolded mode, FP64, on the host (the device kernel's formulas)."""
+    return float(load_library().pmgpu_greens_value(int(gf), int(scheme), int(fds), float(a), int(k[0]), int(k[1]),
+                                                   int(k[2]), int(grid[0]), int(grid[1]), int(grid[2])))
 
 
 def mlp_num_params() -> int:

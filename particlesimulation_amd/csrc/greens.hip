@@ -20,11 +20,11 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 
-__device__ __forceinline__ double sinc_d(double x) { return x == 0.0 ? 1.0 : sin(x) / x; }
+__host__ __device__ __forceinline__ double sinc_d(double x) { return x == 0.0 ? 1.0 : sin(x) / x; }
 
 // S1Fourier / S2Fourier (greensFunctions.cpp:42-50), with the Taylor series
 // for small u where the closed forms cancel catastrophically.
-__device__ double s_fourier(int cs, double k, double a) {
+__host__ __device__ double s_fourier(int cs, double k, double a) {
   const double u = k * a / 2.0;
   const double u2 = u * u;
   if (fabs(u) < 0.5) {
@@ -50,7 +50,7 @@ __device__ double s_fourier(int cs, double k, double a) {
   return 12.0 / (u2 * u2) * (2.0 - 2.0 * cos(u) - u * sin(u));
 }
 
-__device__ double green_optimal(int is, int fds, int cs, double a, int kx, int ky, int kz, int nx,
+__host__ __device__ double green_optimal(int is, int fds, int cs, double a, int kx, int ky, int kz, int nx,
                                 int ny, int nz) {
   if (kx == 0 && ky == 0 && kz == 0) return 0.0;
   const int ki[3] = {kx, ky, kz}, ni[3] = {nx, ny, nz};
@@ -106,7 +106,7 @@ __device__ double green_optimal(int is, int fds, int cs, double a, int kx, int k
   return num / (dn2 * denom * denom);
 }
 
-__device__ double green_at(int gf, int is, int fds, double a, int kx, int ky, int kz, int nx,
+__host__ __device__ double green_at(int gf, int is, int fds, double a, int kx, int ky, int kz, int nx,
                            int ny, int nz) {
   if (kx == 0 && ky == 0 && kz == 0) return 0.0;
   if (gf == PMGPU_DISCRETE_LAPLACIAN) {
@@ -153,3 +153,13 @@ void greens_compute(pmgpu_ctx* c, const float* corr_dev) {
 }
 
 }  // namespace pmgpu
+
+extern "C" double pmgpu_greens_value(int32_t greens_function, int32_t interpolation, int32_t finite_diff,
+                                     double particle_diameter, int32_t kx, int32_t ky, int32_t kz,
+                                     int32_t nx, int32_t ny, int32_t nz) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || greens_function < 0 || greens_function > 3 || interpolation < 0 ||
+      interpolation > 2 || finite_diff < 0 || finite_diff > 1)
+    return NAN;
+  return pmgpu::green_at(greens_function, interpolation, finite_diff, particle_diameter, kx, ky, kz,
+                         nx, ny, nz);
+}

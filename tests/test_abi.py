@@ -5,6 +5,7 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -59,6 +60,18 @@ def test_create_rejects_unsupported_grid(gpu_lib):
     p.grid[:] = [32, 32, 32]
     p.greens_function = P.S2_OPTIMAL  # needs a particle diameter
     assert L.pmgpu_create(C.byref(p), C.byref(C.c_void_p())) == P.ERR_INVALID_ARGUMENT
+
+
+def test_host_greens_value_matches_oracle(gpu_lib, oracle):
+    # the device kernel's formulas, compiled for the host, against the oracle
+    grid = (32, 16, 64)
+    for gf, scheme, fds in ((0, 1, 0), (1, 2, 0), (2, 2, 1), (3, 0, 0), (2, 1, 0)):
+        G = oracle.green_grid(gf, scheme, fds, 2.5, grid)
+        for k in ((1, 0, 0), (5, 3, 60), (31, 15, 63), (16, 8, 32), (0, 8, 0), (7, 1, 2)):
+            v = gpu_lib.greens_value(gf, scheme, fds, 2.5, k, grid)
+            ref = G[k[2], k[1], k[0]]
+            assert abs(v - ref) <= 1e-12 * max(1.0, abs(ref)), (gf, k, v, ref)
+    assert np.isnan(gpu_lib.greens_value(7, 0, 0, 1.0, (1, 0, 0), grid))
 
 
 def test_null_context_is_rejected(gpu_lib):
