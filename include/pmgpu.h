@@ -185,6 +185,20 @@ double pmgpu_greens_value(int32_t greens_function, int32_t interpolation, int32_
                           double particle_diameter, int32_t kx, int32_t ky, int32_t kz, int32_t nx,
                           int32_t ny, int32_t nz);
 
+/* ---- diagnostics ------------------------------------------------------ */
+/* FP64 device reductions of SimInfo (source/simInfo.cpp:50-118) on the
+   current state, in code units, written to out[PMGPU_DIAG_COUNT]:
+     [0] kinetic energy sum 1/2 m |v|^2 (v: the stored, half-step velocity)
+     [1..3] momentum sum m vi, [4..6] angular momentum sum m x cross vi,
+            where vi are the integer-step velocities of the last
+            pmgpu_leapfrog(PMGPU_INTEGER_STEP), else v
+     [7] sum over cells of rho * phi (the mesh potential energy is
+         1/2 H^3 times this, after unit conversion)
+     [8] total mass, [9..11] sum m x.
+   Bitwise reproducible (fixed reduction order). */
+#define PMGPU_DIAG_COUNT 12
+int pmgpu_diagnostics(pmgpu_ctx* ctx, double* out);
+
 /* ---- grids (Grid accessors, include/grid.h:18-44) -------------------- */
 #define PMGPU_GRID_DENSITY 0   /* Nx*Ny*Nz floats, x fastest */
 #define PMGPU_GRID_POTENTIAL 1 /* Nx*Ny*Nz floats */

@@ -49,7 +49,7 @@ EXPORTS = (
     "pmgpu_leapfrog", "pmgpu_get_integer_step_velocities", "pmgpu_run", "pmgpu_escaped",
     "pmgpu_get_grid", "pmgpu_set_greens_correction", "pmgpu_mlp_num_params",
     "pmgpu_profile_enable", "pmgpu_profile_reset", "pmgpu_profile_read", "pmgpu_p3m_count_pairs",
-    "pmgpu_greens_value",
+    "pmgpu_greens_value", "pmgpu_diagnostics",
 )
 
 
@@ -131,6 +131,7 @@ def load_library(path: str = LIB_PATH):
     L.pmgpu_p3m_count_pairs.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.pmgpu_greens_value.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_double] + [C.c_int32] * 6
     L.pmgpu_greens_value.restype = C.c_double
+    L.pmgpu_diagnostics.argtypes = [vp, C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -269,6 +270,15 @@ class PM:
 
     def synchronize(self):
         self._check(self._lib.pmgpu_synchronize(self._ctx))
+
+    def diagnostics(self):
+        """FP64 device reductions (code units): dict of kinetic, momentum, angular_momentum,
+        rho_phi, mass, mass_moment."""
+        out = (C.c_double * 12)()
+        self._check(self._lib.pmgpu_diagnostics(self._ctx, out))
+        v = list(out)
+        return {"kinetic": v[0], "momentum": v[1:4], "angular_momentum": v[4:7], "rho_phi": v[7],
+                "mass": v[8], "mass_moment": v[9:12]}
 
     # -- grids ------------------------------------------------------------
     def get_grid(self, which):

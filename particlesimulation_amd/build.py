@@ -5,6 +5,7 @@
 Produces, in-tree:
     particlesimulation_amd/libpmgpu.so   HIP kernels + the C ABI of include/pmgpu.h
     particlesimulation_amd/libpmsim.so   C++ drop-in classes (include/pmsim/*.h)
+    tests/dropin/dropin_check            a program using the drop-in classes (test input)
 """
 from __future__ import annotations
 
@@ -20,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 HOST = os.path.join(HERE, "host")
 BUILD = os.path.join(HERE, "_build")
 HIP_SOURCES = ["api.hip", "sort.hip", "deposit.hip", "fft.hip", "greens.hip", "interp.hip", "p3m.hip",
-               "mlp.hip"]
+               "mlp.hip", "diag.hip"]
 ARCH = os.environ.get("PMGPU_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 (v_pk_*_f32) is disabled for all device code.
@@ -72,9 +73,24 @@ def build_pmsim() -> str | None:
     return lib
 
 
+def build_dropin_check() -> str | None:
+    """The C++ program the drop-in tests run (tests/dropin/dropin_check.cpp)."""
+    src = os.path.join(ROOT, "tests", "dropin", "dropin_check.cpp")
+    if not os.path.isfile(src):
+        return None
+    exe = src[:-4]
+    cxx = shutil.which("g++") or "g++"
+    _run([cxx, "-O2", "-std=c++20", "-I", os.path.join(ROOT, "include"), src, "-L", HERE, "-lpmsim",
+          "-lpmgpu", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../particlesimulation_amd", "-o",
+          exe + ".tmp"])
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
 def build_all() -> None:
     build_pmgpu()
     build_pmsim()
+    build_dropin_check()
 
 
 if __name__ == "__main__":

@@ -270,7 +270,7 @@ void pmgpu_destroy(pmgpu_ctx* c) {
   dfree(&c->tw_x); dfree(&c->tw_y); dfree(&c->tw_z); dfree(&c->tw_r2c);
   dfree(&c->sr_table); dfree(&c->sr_count); dfree(&c->sr_start); dfree(&c->sr_key);
   dfree(&c->sr_rank); dfree(&c->sr_pm); dfree(&c->sr_idx); dfree(&c->sr_nchunk);
-  dfree(&c->sr_chunk_start); dfree(&c->pair_count); dfree(&c->escape);
+  dfree(&c->sr_chunk_start); dfree(&c->pair_count); dfree(&c->escape); dfree(&c->diag_scratch);
   for (auto& p : c->pending) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
@@ -441,7 +441,7 @@ int pmgpu_p3m_configure(pmgpu_ctx* c, const pmgpu_p3m_params* p) {
   dalloc(&c->sr_start, c->ncell_sr + 1);
   dalloc(&c->sr_nchunk, c->ncell_sr + 1);
   dalloc(&c->sr_chunk_start, c->ncell_sr + 1);
-  dalloc(&c->sr_table, SR_TABLE);
+  dalloc(&c->sr_table, SR_TABLE_ACC);  // the larger of the two tables
   dalloc(&c->pair_count, 2);
   ensure_scan_scratch(c, c->ncell_sr);
   c->p3m_ready = true;

@@ -115,7 +115,7 @@ struct pmgpu_ctx {
   pmgpu_p3m_params p3m{};
   int mcx = 0, mcy = 0, mcz = 0, ncell_sr = 0;
   float hcx = 0, hcy = 0, hcz = 0;
-  float2* sr_table = nullptr;           // SR_TABLE entries (T[t], T[t+1]-T[t])
+  float2* sr_table = nullptr;           // SR_TABLE_ACC entries (T[t], T[t+1]-T[t])
   float sr_inv_d2 = 0.0f;
   int sr_tmax = 0;
   int32_t* sr_count = nullptr;          // ncell_sr + 1
@@ -127,6 +127,8 @@ struct pmgpu_ctx {
   int32_t* sr_nchunk = nullptr;         // ncell_sr + 1
   int32_t* sr_chunk_start = nullptr;    // ncell_sr + 1
   unsigned long long* pair_count = nullptr;
+
+  double* diag_scratch = nullptr;       // FP64 diagnostics partial sums
 
   // run loop state
   int32_t* escape = nullptr;   // [0] = first iteration with an escaped particle, or INT_MAX

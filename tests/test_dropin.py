@@ -1,0 +1,169 @@
+"""The C++ drop-in classes (include/pmsim/*.h, libpmsim.so).
+
+CPU: the library exports the reference's class API.  GPU: a program written
+against that API (tests/dropin/dropin_check.cpp, modelled on the reference's
+demos) runs PMMethod::run / P3MMethod::run, and its recorded frames, final
+state and diagnostics are checked against the same loop driven through the
+C ABI from Python, step by step (reference source/pmMethod.cpp:62-135,
+source/p3mMethod.cpp:59-166), and against the CPU oracle's leapfrog.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "particlesimulation_amd")
+EXE = os.path.join(ROOT, "tests", "dropin", "dropin_check")
+
+
+def _dynamic_symbols(path):
+    out = subprocess.run(["nm", "-D", "-C", "--defined-only", path], stdout=subprocess.PIPE, text=True,
+                         check=True).stdout
+    return out
+
+
+@pytest.mark.parametrize("sym", [
+    "PMMethod::PMMethod(", "PMMethod::run(", "PMMethod::pmMethodStep()", "PMMethod::getParticles()",
+    "PMMethod::escapedComputationalBox()", "PMMethod::initGreensFunction()",
+    "P3MMethod::P3MMethod(", "P3MMethod::run(", "Grid::Grid(", "Grid::getDensity(", "Grid::getPotential(",
+    "Grid::getField(", "Grid::getGreensFunction(", "Grid::fftDensity()", "StateRecorder::StateRecorder(",
+    "StateRecorder::recordPositions(", "StateRecorder::flush()", "SimInfo::kineticEnergy(",
+    "SimInfo::totalMomentum(", "GreenOptimal(", "sphRadDecrField(",
+])
+def test_libpmsim_exports_reference_api(sym):
+    lib = os.path.join(PKG, "libpmsim.so")
+    if not os.path.exists(lib):
+        pytest.skip("libpmsim.so not built")
+    syms = _dynamic_symbols(lib).replace("[abi:cxx11]", "")
+    assert sym in syms
+
+
+def _write_input(path, pos, vel, mass, steps, p3m, grid, H, DT, G, a, rc):
+    with open(path, "wb") as f:
+        np.array([len(mass), steps, int(p3m), grid], np.int32).tofile(f)
+        np.array([H, DT, G, a, rc], np.float32).tofile(f)
+        pos.astype(np.float32).tofile(f)
+        vel.astype(np.float32).tofile(f)
+        mass.astype(np.float32).tofile(f)
+
+
+def _read_frames(path):
+    raw = np.fromfile(path, np.uint8)
+    n, frames = np.frombuffer(raw[:8].tobytes(), np.int32)
+    data = np.frombuffer(raw[8:].tobytes(), np.float32).reshape(-1, n, 3)
+    return int(n), int(frames), data
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("p3m", [False, True])
+def test_dropin_run_matches_abi_loop(gpu_lib, oracle, tmp_path, p3m):
+    from particlesimulation_amd import ics
+    assert os.path.exists(EXE), "tests/dropin/dropin_check is built by particlesimulation_amd.build"
+    g, steps = 32, 6
+    H, DT, G = 0.25, 0.01, 1.0
+    box = g * H
+    pos, vel, mass = ics.plummer(2000, seed=5, a=0.6, center=(box / 2,) * 3, r_max=0.35 * box)
+    a_orig = rc_orig = 3 * H
+    inp = tmp_path / "in.bin"
+    _write_input(inp, pos, vel, mass, steps, p3m, g, H, DT, G, a_orig, rc_orig)
+    out = tmp_path / "out"
+    out.mkdir()
+    r = subprocess.run([EXE, str(inp), str(out)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0
+
+    # the same loop through the C ABI
+    pc, vc, mc = ics.to_code_units(pos, vel, mass, H, DT, G)
+    pos32 = pos.astype(np.float32)
+    pcode = (pos32 / np.float32(H)).astype(np.float32)  # the drop-in divides in float32
+    assert np.allclose(pcode, pc, rtol=1e-6)
+    pm = gpu_lib.PM((g, g, g), interpolation=2, finite_diff=0, greens_function=2 if p3m else 0,
+                    particle_diameter=3.0, box=(g, g, g))
+    pm.set_particles(pc, vc, mc)
+    pm.init_greens()
+    if p3m:
+        pm.p3m_configure(cutoff=3.0, particle_diameter=3.0, softening=0.0, cloud_shape=1, use_table=1)
+
+    def force():
+        pm.pm_step()
+        if p3m:
+            pm.p3m_short_range()
+
+    force()
+    pm.leapfrog(gpu_lib.HALF_KICK, 1.0)
+    frames, moms = [], []
+    for _ in range(steps + 1):
+        pm.leapfrog(gpu_lib.DRIFT, 1.0)
+        pm.leapfrog(gpu_lib.INTEGER_STEP, 1.0)
+        x, _, _ = pm.get_particles()
+        frames.append(x.astype(np.float64) * H)
+        vi = pm.integer_step_velocities().astype(np.float64) * (H / DT)
+        moms.append((mass[:, None] * vi).sum(0))
+        force()
+        pm.leapfrog(gpu_lib.KICK, 1.0)
+    x_end, v_end, a_end = pm.get_particles()
+
+    n, nframes, rec = _read_frames(out / "positions.dat")
+    assert (n, nframes) == (2000, steps + 1)
+    assert rec.shape == (steps + 1, n, 3)
+    for t in range(steps + 1):
+        assert np.max(np.abs(rec[t] - frames[t])) <= 2e-6 * box, t
+    fin = np.fromfile(out / "final.bin", np.float32).reshape(3, n, 3)
+    # after run() the mirror is in code units, as the reference's particles are
+    assert np.max(np.abs(fin[0] - x_end)) <= 1e-5 * g
+    assert np.max(np.abs(fin[1] - v_end)) <= 1e-6 + 1e-4 * np.abs(v_end).max()
+    assert np.max(np.abs(fin[2] - a_end)) <= 1e-6 + 1e-4 * np.abs(a_end).max()
+
+    mom = np.loadtxt(out / "momentum.txt").reshape(-1, 3)
+    assert mom.shape == (steps + 1, 3)
+    scale = np.abs(mass[:, None] * vel).sum()
+    assert np.max(np.abs(mom - np.array(moms))) <= 1e-5 * scale
+    energy = np.loadtxt(out / "energy.txt").reshape(-1, 2)
+    ke_ref = 0.5 * (mass * ((v_end.astype(np.float64) * H / DT) ** 2).sum(1)).sum()
+    assert energy.shape == (steps + 1, 2)
+    assert np.all(energy[:, 1] > 0) and np.all(energy[:, 0] < 0)  # bound system
+    assert abs(energy[-1, 1] - ke_ref) <= 0.05 * ke_ref  # one kick apart
+
+    # the oracle's leapfrog over the same steps (positions of the last frame)
+    if not p3m:
+        Gk = oracle.green_grid(0, 2, 0, 0.0, (g, g, g))
+        xs, vs = pc.astype(np.float64), vc.astype(np.float64)
+        a = oracle.pm_accel(xs.astype(np.float32), mc, (g, g, g), 2, 0, Gk).astype(np.float64)
+        vs = vs + 0.5 * a
+        for _ in range(steps + 1):
+            xs = xs + vs
+            a = oracle.pm_accel(xs.astype(np.float32), mc, (g, g, g), 2, 0, Gk).astype(np.float64)
+            vs = vs + a
+        assert np.linalg.norm(rec[-1] / H - xs) / np.linalg.norm(xs) < 1e-5
+
+
+@pytest.mark.gpu
+def test_diagnostics_match_numpy(gpu_lib):
+    grid = (32, 32, 32)
+    r = np.random.default_rng(9)
+    n = 5000
+    pos = r.uniform(4, 28, (n, 3)).astype(np.float32)
+    vel = (r.standard_normal((n, 3)) * 0.05).astype(np.float32)
+    m = r.uniform(0.5, 1.5, n).astype(np.float32) / n
+    pm = gpu_lib.PM(grid, interpolation=2, finite_diff=0, greens_function=0, particle_diameter=3.0, box=grid)
+    pm.set_particles(pos, vel, m)
+    pm.init_greens()
+    pm.pm_step()
+    pm.leapfrog(gpu_lib.INTEGER_STEP, 1.0)
+    d = pm.diagnostics()
+    x, v, a = pm.get_particles()
+    u = pm.integer_step_velocities().astype(np.float64)
+    x64, v64, m64 = x.astype(np.float64), v.astype(np.float64), m.astype(np.float64)
+    assert np.isclose(d["kinetic"], 0.5 * (m64 * (v64 ** 2).sum(1)).sum(), rtol=1e-12)
+    assert np.allclose(d["momentum"], (m64[:, None] * u).sum(0), rtol=1e-10, atol=1e-16)
+    assert np.allclose(d["angular_momentum"], (m64[:, None] * np.cross(x64, u)).sum(0), rtol=1e-10, atol=1e-16)
+    rho, phi = pm.density().astype(np.float64), pm.potential().astype(np.float64)
+    assert np.isclose(d["rho_phi"], (rho * phi).sum(), rtol=1e-10)
+    assert d["rho_phi"] < 0
+    assert np.isclose(d["mass"], m64.sum(), rtol=1e-12)
+    assert np.allclose(d["mass_moment"], (m64[:, None] * x64).sum(0), rtol=1e-12)
+    # bitwise reproducible
+    assert pm.diagnostics() == d
