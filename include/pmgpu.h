@@ -227,6 +227,46 @@ int pmgpu_profile_read(pmgpu_ctx* ctx, int32_t max, const char** names, double* 
    FLOP/s. */
 int pmgpu_p3m_count_pairs(pmgpu_ctx* ctx, int64_t* pairs, int64_t* tested);
 
+/* ---- learned correction: standalone MLP evaluation ------------------- */
+/* No counterpart in the reference.  inputs is n x 4 (host); out[m] = y(m).
+   pmgpu_mlp_backward returns the gradient of sum_m up[m] * y(m) with respect
+   to the packed parameters (BF16 MFMA, FP32 accumulation). */
+int pmgpu_mlp_forward(const float* params, int64_t n, const float* inputs, float* out);
+int pmgpu_mlp_backward(const float* params, int64_t n, const float* inputs, const float* up, float* grad);
+
+/* ---- learned correction: training ------------------------------------ */
+/* No counterpart in the reference; the loss is the reference's accuracy
+   measure (script/pm_accuracy.py:44-48) as e_rms^2 = sum|a - t|^2 /
+   sum|t|^2 of the PM force at target particles against given target
+   accelerations.  A trainer owns a grid (params: grid, interpolation,
+   finite_diff, greens_function, particle_diameter; box <= 0, periodic) and
+   a set of snapshots: positions and masses in code units, target indices
+   into them and the targets' accelerations (code units).  The correction
+   multiplies the trainer's analytic G by (1 + y(k)). */
+typedef struct pmgpu_trainer pmgpu_trainer;
+int pmgpu_trainer_create(const pmgpu_pm_params* params, pmgpu_trainer** out);
+void pmgpu_trainer_destroy(pmgpu_trainer* trainer);
+const char* pmgpu_trainer_last_error(const pmgpu_trainer* trainer);
+int pmgpu_trainer_add_snapshot(pmgpu_trainer* trainer, int64_t n, const float* pos, const float* mass,
+                               int64_t n_targets, const int64_t* target_index, const float* target_acc);
+/* Loss for params (NULL = analytic G); grad (NULL to skip) receives dL/dparams,
+   grad_c (NULL to skip) dL/d(1 + y(k)) on the half spectrum [z][y][kx]. */
+int pmgpu_trainer_loss_grad(pmgpu_trainer* trainer, const float* params, double* loss, float* grad,
+                            float* grad_c);
+/* Adam (beta 0.9 / 0.999) for `steps` steps on the joint loss of all
+   trainers (one normaliser over all their targets); params is updated in
+   place; loss_history (may be NULL) receives the loss before each step. */
+int pmgpu_train(pmgpu_trainer** trainers, int32_t n_trainers, float* params, int32_t steps, float lr,
+                float* loss_history);
+
+/* ---- direct-sum forces (training targets and error checks) ------------ */
+/* a_i = -1/(4 pi) sum_j m_j d_ij / (|d_ij|^2 + eps^2)^(3/2), d_ij = x_i - x_j
+   (code units, Plummer softening as source/ppMethod.cpp:23-36 with G' =
+   1/(4 pi)), coincident pairs skipped; FP32 with compensated sums.  Host
+   arrays: pos n x 3, mass n, tpos nt x 3, out nt x 3. */
+int pmgpu_direct_sum(int64_t n, const float* pos, const float* mass, int64_t nt, const float* tpos,
+                     float softening, float* out);
+
 #ifdef __cplusplus
 }
 #endif

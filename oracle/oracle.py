@@ -79,6 +79,10 @@ def lib():
                                      C.c_int, _f64p]
         L.orc_num_threads.restype = C.c_int
         L.orc_set_num_threads.argtypes = [C.c_int]
+        L.orc_mlp_forward.argtypes = [_f64p, C.c_int64, _f64p, _f64p]
+        L.orc_mlp_forward.restype = None
+        L.orc_mlp_backward.argtypes = [_f64p, C.c_int64, _f64p, _f64p, _f64p]
+        L.orc_mlp_backward.restype = None
         _lib = L
     return _lib
 
@@ -254,6 +258,25 @@ def direct_sum(pos, mass, Gc, eps=0.0, kernel=0, targets=None):
     acc = np.zeros(3 * nt, np.float64)
     lib().orc_direct_sum(nt, tptr, len(mass), pos.ravel(), mass, float(Gc), float(eps), int(kernel), acc)
     return acc.reshape(-1, 3)
+
+
+def mlp_forward(params, inputs):
+    """FP64 forward pass of the correction network (csrc/mlp.hip)."""
+    p = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(inputs, np.float64).reshape(-1, 4)
+    y = np.zeros(len(x), np.float64)
+    lib().orc_mlp_forward(p, len(x), x.ravel(), y)
+    return y
+
+
+def mlp_backward(params, inputs, up):
+    """FP64 gradient of sum_m up[m] * y(m) with respect to the parameters."""
+    p = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(inputs, np.float64).reshape(-1, 4)
+    u = np.ascontiguousarray(up, np.float64)
+    g = np.zeros(len(p), np.float64)
+    lib().orc_mlp_backward(p, len(x), x.ravel(), u, g)
+    return g
 
 
 # --------------------------------------------------------------------------

@@ -908,3 +908,93 @@ int orc_direct_sum(int64_t nt, const int64_t* targets, int64_t n, const float* p
   }
   return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * Learned influence-function correction (no reference counterpart): the
+ * 4 -> 64 -> 64 -> 64 -> 1 GELU network of particlesimulation_amd/csrc/mlp.hip
+ * in FP64.  Parameters packed as W1[64][4], b1, W2[64][64], b2, W3[64][64],
+ * b3, w4[64], b4 (8705 values, W[out][in]).  Pinned by closed-form single-path
+ * networks (forward) and central finite differences (backward).
+ * ------------------------------------------------------------------------ */
+#define ORC_HID 64
+enum { ORC_W1 = 0, ORC_B1 = 256, ORC_W2 = 320, ORC_B2 = 4416, ORC_W3 = 4480, ORC_B3 = 8576, ORC_W4 = 8640,
+       ORC_B4 = 8704 };
+
+static inline double orc_gelu(double z) { return 0.5 * z * (1.0 + erf(z * 0.70710678118654752440)); }
+static inline double orc_gelu_d(double z) {
+  return 0.5 * (1.0 + erf(z * 0.70710678118654752440)) + z * 0.39894228040143267794 * exp(-0.5 * z * z);
+}
+
+static double mlp_one(const double* p, const double* x, double z1[ORC_HID], double z2[ORC_HID],
+                      double z3[ORC_HID]) {
+  double h1[ORC_HID], h2[ORC_HID];
+  for (int f = 0; f < ORC_HID; ++f) {
+    double z = p[ORC_B1 + f];
+    for (int i = 0; i < 4; ++i) z += p[ORC_W1 + 4 * f + i] * x[i];
+    z1[f] = z;
+    h1[f] = orc_gelu(z);
+  }
+  for (int f = 0; f < ORC_HID; ++f) {
+    double z = p[ORC_B2 + f];
+    for (int k = 0; k < ORC_HID; ++k) z += p[ORC_W2 + ORC_HID * f + k] * h1[k];
+    z2[f] = z;
+    h2[f] = orc_gelu(z);
+  }
+  double y = p[ORC_B4];
+  for (int f = 0; f < ORC_HID; ++f) {
+    double z = p[ORC_B3 + f];
+    for (int k = 0; k < ORC_HID; ++k) z += p[ORC_W3 + ORC_HID * f + k] * h2[k];
+    z3[f] = z;
+    y += p[ORC_W4 + f] * orc_gelu(z);
+  }
+  return y;
+}
+
+void orc_mlp_forward(const double* p, int64_t n, const double* x, double* y) {
+#pragma omp parallel for schedule(static)
+  for (int64_t m = 0; m < n; ++m) {
+    double z1[ORC_HID], z2[ORC_HID], z3[ORC_HID];
+    y[m] = mlp_one(p, x + 4 * m, z1, z2, z3);
+  }
+}
+
+/* grad = d/dp sum_m up[m] y(m) */
+void orc_mlp_backward(const double* p, int64_t n, const double* x, const double* up, double* grad) {
+  memset(grad, 0, sizeof(double) * 8705);
+  for (int64_t m = 0; m < n; ++m) {
+    double z1[ORC_HID], z2[ORC_HID], z3[ORC_HID], h1[ORC_HID], h2[ORC_HID];
+    double dz3[ORC_HID], dz2[ORC_HID], dz1[ORC_HID];
+    const double* xm = x + 4 * m;
+    mlp_one(p, xm, z1, z2, z3);
+    for (int f = 0; f < ORC_HID; ++f) {
+      h1[f] = orc_gelu(z1[f]);
+      h2[f] = orc_gelu(z2[f]);
+    }
+    const double d = up[m];
+    grad[ORC_B4] += d;
+    for (int f = 0; f < ORC_HID; ++f) {
+      grad[ORC_W4 + f] += d * orc_gelu(z3[f]);
+      dz3[f] = d * p[ORC_W4 + f] * orc_gelu_d(z3[f]);
+      grad[ORC_B3 + f] += dz3[f];
+      for (int k = 0; k < ORC_HID; ++k) grad[ORC_W3 + ORC_HID * f + k] += dz3[f] * h2[k];
+    }
+    for (int k = 0; k < ORC_HID; ++k) {
+      double s = 0.0;
+      for (int f = 0; f < ORC_HID; ++f) s += p[ORC_W3 + ORC_HID * f + k] * dz3[f];
+      dz2[k] = s * orc_gelu_d(z2[k]);
+    }
+    for (int f = 0; f < ORC_HID; ++f) {
+      grad[ORC_B2 + f] += dz2[f];
+      for (int k = 0; k < ORC_HID; ++k) grad[ORC_W2 + ORC_HID * f + k] += dz2[f] * h1[k];
+    }
+    for (int k = 0; k < ORC_HID; ++k) {
+      double s = 0.0;
+      for (int f = 0; f < ORC_HID; ++f) s += p[ORC_W2 + ORC_HID * f + k] * dz2[f];
+      dz1[k] = s * orc_gelu_d(z1[k]);
+    }
+    for (int f = 0; f < ORC_HID; ++f) {
+      grad[ORC_B1 + f] += dz1[f];
+      for (int i = 0; i < 4; ++i) grad[ORC_W1 + 4 * f + i] += dz1[f] * xm[i];
+    }
+  }
+}

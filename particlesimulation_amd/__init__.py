@@ -49,7 +49,9 @@ EXPORTS = (
     "pmgpu_leapfrog", "pmgpu_get_integer_step_velocities", "pmgpu_run", "pmgpu_escaped",
     "pmgpu_get_grid", "pmgpu_set_greens_correction", "pmgpu_mlp_num_params",
     "pmgpu_profile_enable", "pmgpu_profile_reset", "pmgpu_profile_read", "pmgpu_p3m_count_pairs",
-    "pmgpu_greens_value", "pmgpu_diagnostics",
+    "pmgpu_greens_value", "pmgpu_diagnostics", "pmgpu_mlp_forward", "pmgpu_mlp_backward",
+    "pmgpu_trainer_create", "pmgpu_trainer_destroy", "pmgpu_trainer_last_error", "pmgpu_trainer_add_snapshot",
+    "pmgpu_trainer_loss_grad", "pmgpu_train", "pmgpu_direct_sum",
 )
 
 
@@ -132,6 +134,17 @@ def load_library(path: str = LIB_PATH):
     L.pmgpu_greens_value.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_double] + [C.c_int32] * 6
     L.pmgpu_greens_value.restype = C.c_double
     L.pmgpu_diagnostics.argtypes = [vp, C.POINTER(C.c_double)]
+    L.pmgpu_mlp_forward.argtypes = [fp, C.c_int64, fp, fp]
+    L.pmgpu_mlp_backward.argtypes = [fp, C.c_int64, fp, fp, fp]
+    L.pmgpu_trainer_create.argtypes = [C.POINTER(PMParams), C.POINTER(vp)]
+    L.pmgpu_trainer_destroy.argtypes = [vp]
+    L.pmgpu_trainer_destroy.restype = None
+    L.pmgpu_trainer_last_error.argtypes = [vp]
+    L.pmgpu_trainer_last_error.restype = C.c_char_p
+    L.pmgpu_trainer_add_snapshot.argtypes = [vp, C.c_int64, fp, fp, C.c_int64, C.POINTER(C.c_int64), fp]
+    L.pmgpu_trainer_loss_grad.argtypes = [vp, fp, C.POINTER(C.c_double), fp, fp]
+    L.pmgpu_train.argtypes = [C.POINTER(vp), C.c_int32, fp, C.c_int32, C.c_float, fp]
+    L.pmgpu_direct_sum.argtypes = [C.c_int64, fp, fp, C.c_int64, fp, C.c_float, fp]
     _lib = L
     return L
 
@@ -328,3 +341,136 @@ def greens_value(gf, scheme, fds, a, k, grid) -> float:
 
 def mlp_num_params() -> int:
     return int(load_library().pmgpu_mlp_num_params())
+
+
+# ---------------------------------------------------------------------------
+# learned influence-function correction
+# ---------------------------------------------------------------------------
+
+def _raise(rc, msg):
+    raise (InvalidArgument if rc == ERR_INVALID_ARGUMENT else PMGPUError)(rc, msg)
+
+
+def mlp_init(seed=0, scale=1.0):
+    """Initial MLP parameters: He-scaled normal weights, zero biases and a zero
+    output layer, so the correction starts as the analytic G (factor 1)."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(mlp_num_params(), np.float32)
+    o = 0
+    for fan_in, fan_out in ((4, 64), (64, 64), (64, 64)):
+        w = rng.standard_normal((fan_out, fan_in)) * scale * np.sqrt(2.0 / fan_in)
+        p[o:o + fan_in * fan_out] = w.ravel()
+        o += fan_in * fan_out + fan_out
+    return p
+
+
+def mlp_grid_inputs(grid, scheme):
+    """MLP inputs of every half-spectrum mode [z][y][kx] (n x 4)."""
+    nx, ny, nz = grid
+    kx = np.arange(nx // 2 + 1)
+    ky = np.minimum(np.arange(ny), ny - np.arange(ny))
+    kz = np.minimum(np.arange(nz), nz - np.arange(nz))
+    Z, Y, X = np.meshgrid(kz, ky, kx, indexing="ij")
+    x = np.stack([X / (nx // 2), Y / (ny // 2), Z / (nz // 2), np.full(X.shape, float(scheme - CIC))], axis=-1)
+    return x.reshape(-1, 4).astype(np.float32)
+
+
+def mlp_forward(params, inputs):
+    L = load_library()
+    params, inputs = _f32(params), _f32(inputs, (-1, 4))
+    out = np.zeros(len(inputs), np.float32)
+    rc = L.pmgpu_mlp_forward(_fptr(params), len(inputs), _fptr(inputs), _fptr(out))
+    if rc != OK:
+        _raise(rc, "pmgpu_mlp_forward failed")
+    return out
+
+
+def mlp_backward(params, inputs, up):
+    L = load_library()
+    params, inputs, up = _f32(params), _f32(inputs, (-1, 4)), _f32(up)
+    grad = np.zeros(mlp_num_params(), np.float32)
+    rc = L.pmgpu_mlp_backward(_fptr(params), len(inputs), _fptr(inputs), _fptr(up), _fptr(grad))
+    if rc != OK:
+        _raise(rc, "pmgpu_mlp_backward failed")
+    return grad
+
+
+def direct_sum(pos, mass, targets, softening=0.0):
+    """Direct-sum accelerations (code units, G' = 1/(4 pi)) at target positions, on the GPU."""
+    L = load_library()
+    pos, mass, targets = _f32(pos, (-1, 3)), _f32(mass), _f32(targets, (-1, 3))
+    out = np.zeros((len(targets), 3), np.float32)
+    rc = L.pmgpu_direct_sum(len(mass), _fptr(pos), _fptr(mass), len(targets), _fptr(targets), float(softening),
+                            _fptr(out))
+    if rc != OK:
+        _raise(rc, "pmgpu_direct_sum failed")
+    return out
+
+
+class Trainer:
+    """Snapshots of one grid configuration for training the correction."""
+
+    def __init__(self, grid, interpolation=TSC, finite_diff=FOUR_POINT, greens_function=DISCRETE_LAPLACIAN,
+                 particle_diameter=0.0):
+        L = load_library()
+        p = PMParams()
+        p.grid[:] = [int(g) for g in grid]
+        p.interpolation = int(interpolation)
+        p.finite_diff = int(finite_diff)
+        p.greens_function = int(greens_function)
+        p.particle_diameter = float(particle_diameter)
+        p.box[:] = [0.0, 0.0, 0.0]
+        p.H = p.DT = 1.0
+        self._lib = L
+        self.grid = tuple(int(g) for g in grid)
+        self.interpolation = int(interpolation)
+        self._t = C.c_void_p()
+        rc = L.pmgpu_trainer_create(C.byref(p), C.byref(self._t))
+        if rc != OK:
+            _raise(rc, L.pmgpu_last_error(None).decode())
+        self.snapshots = 0
+
+    def _check(self, rc):
+        if rc != OK:
+            _raise(rc, self._lib.pmgpu_trainer_last_error(self._t).decode())
+
+    def add_snapshot(self, pos, mass, target_index, target_acc):
+        pos, mass = _f32(pos, (-1, 3)), _f32(mass)
+        idx = np.ascontiguousarray(target_index, dtype=np.int64)
+        acc = _f32(target_acc, (-1, 3))
+        self._check(self._lib.pmgpu_trainer_add_snapshot(
+            self._t, len(mass), _fptr(pos), _fptr(mass), len(idx), idx.ctypes.data_as(C.POINTER(C.c_int64)),
+            _fptr(acc)))
+        self.snapshots += 1
+
+    def loss_grad(self, params=None, want_grad=True, want_grad_c=False):
+        p = None if params is None else _f32(params)
+        loss = C.c_double(0.0)
+        g = np.zeros(mlp_num_params(), np.float32) if want_grad else None
+        nx, ny, nz = self.grid
+        gc = np.zeros((nz, ny, nx // 2 + 1), np.float32) if want_grad_c else None
+        self._check(self._lib.pmgpu_trainer_loss_grad(self._t, _fptr(p), C.byref(loss), _fptr(g), _fptr(gc)))
+        return loss.value, g, gc
+
+    def close(self):
+        if self._t:
+            self._lib.pmgpu_trainer_destroy(self._t)
+            self._t = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def train(trainers, params, steps, lr=1e-3):
+    """Adam on the joint loss of the trainers; returns (params, loss history)."""
+    L = load_library()
+    p = _f32(params).copy()
+    hist = np.zeros(steps, np.float32)
+    arr = (C.c_void_p * len(trainers))(*[t._t for t in trainers])
+    rc = L.pmgpu_train(arr, len(trainers), _fptr(p), int(steps), float(lr), _fptr(hist))
+    if rc != OK:
+        _raise(rc, L.pmgpu_trainer_last_error(trainers[0]._t).decode())
+    return p, hist

@@ -395,11 +395,9 @@ int pmgpu_update_particles(pmgpu_ctx* c, const float* pos, const float* vel, con
 
 int pmgpu_init_greens(pmgpu_ctx* c) {
   PMGPU_API_BEGIN(c)
-  float* corr = nullptr;
-  if (!c->mlp_weights.empty()) corr = mlp_correction_grid(c);
-  greens_compute(c, corr);
+  greens_compute(c, nullptr);
+  if (!c->mlp_weights.empty()) mlp_apply_to_greens(c);  // G_sym * (1 + y(k))
   HIP_CHECK(hipStreamSynchronize(c->stream));
-  if (corr) HIP_CHECK(hipFree(corr));
   c->greens_ready = true;
   PMGPU_API_END(c)
 }

@@ -248,11 +248,16 @@ __global__ __launch_bounds__(ColTile<L>::THREADS) void k_fft_y(float2* __restric
   }
 }
 
-// z pass: forward along z, multiply by G, inverse along z.
-template <int L>
-__global__ __launch_bounds__(ColTile<L>::THREADS) void k_fft_z_green(
-    float2* __restrict__ spec, const float* __restrict__ greens, int nxh, int ny, int ntile,
-    const float2* __restrict__ tw, const int32_t* __restrict__ escape, int iteration) {
+// z pass.  MODE Z_GREEN: forward along z, multiply by G, inverse along z
+// (the force step: the pencil never leaves LDS).  Z_FWD: forward only.
+// Z_MUL_INV: multiply src by G, inverse along z, write dst (src is kept,
+// so a cached density spectrum can be reused with several G).
+constexpr int Z_GREEN = 0, Z_FWD = 1, Z_MUL_INV = 2;
+
+template <int L, int MODE>
+__global__ __launch_bounds__(ColTile<L>::THREADS) void k_fft_z(
+    const float2* __restrict__ src, float2* __restrict__ dst, const float* __restrict__ greens, int nxh,
+    int ny, int ntile, const float2* __restrict__ tw, const int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
   using CT = ColTile<L>;
   __shared__ float2 buf[L * CT::NC];
@@ -263,20 +268,24 @@ __global__ __launch_bounds__(ColTile<L>::THREADS) void k_fft_z_green(
   const int64_t off = (int64_t)y * nxh + c0;
   for (int e = threadIdx.x; e < L * ncols; e += CT::THREADS) {
     const int n = e / ncols, col = e % ncols;
-    buf[n * CT::NC + col] = spec[off + n * zstride + col];
+    float2 v = src[off + n * zstride + col];
+    if constexpr (MODE == Z_MUL_INV) v = scale2(v, greens[off + n * zstride + col]);
+    buf[n * CT::NC + col] = v;
   }
   __syncthreads();
   const int col = threadIdx.x % CT::NC, t = threadIdx.x / CT::NC;
-  fft_lds<L, false>(buf, 1, CT::NC, col, t, col < ncols, tw);
-  for (int e = threadIdx.x; e < L * ncols; e += CT::THREADS) {
-    const int n = e / ncols, col2 = e % ncols;
-    buf[n * CT::NC + col2] = scale2(buf[n * CT::NC + col2], greens[off + n * zstride + col2]);
+  if constexpr (MODE != Z_MUL_INV) fft_lds<L, false>(buf, 1, CT::NC, col, t, col < ncols, tw);
+  if constexpr (MODE == Z_GREEN) {
+    for (int e = threadIdx.x; e < L * ncols; e += CT::THREADS) {
+      const int n = e / ncols, col2 = e % ncols;
+      buf[n * CT::NC + col2] = scale2(buf[n * CT::NC + col2], greens[off + n * zstride + col2]);
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  fft_lds<L, true>(buf, 1, CT::NC, col, t, col < ncols, tw);
+  if constexpr (MODE != Z_FWD) fft_lds<L, true>(buf, 1, CT::NC, col, t, col < ncols, tw);
   for (int e = threadIdx.x; e < L * ncols; e += CT::THREADS) {
     const int n = e / ncols, col2 = e % ncols;
-    spec[off + n * zstride + col2] = buf[n * CT::NC + col2];
+    dst[off + n * zstride + col2] = buf[n * CT::NC + col2];
   }
 }
 
@@ -296,73 +305,93 @@ int rows_per_block(int L) {
 }
 
 template <int L, bool INV>
-void launch_y(pmgpu_ctx* c) {
+void launch_y(pmgpu_ctx* c, float2* spec) {
   using CT = ColTile<L>;
   const int ntile = (c->nx / 2) / CT::TW;
-  hipLaunchKernelGGL((k_fft_y<L, INV>), dim3(ntile, c->nz), dim3(CT::THREADS), 0, c->stream,
-                     c->spec, c->nxh, ntile, c->tw_y, c->escape, c->iter);
+  hipLaunchKernelGGL((k_fft_y<L, INV>), dim3(ntile, c->nz), dim3(CT::THREADS), 0, c->stream, spec,
+                     c->nxh, ntile, c->tw_y, c->escape, c->iter);
 }
 
-template <int L>
-void launch_z(pmgpu_ctx* c) {
+template <int L, int MODE>
+void launch_z(pmgpu_ctx* c, const float2* src, float2* dst, const float* g) {
   using CT = ColTile<L>;
   const int ntile = (c->nx / 2) / CT::TW;
-  hipLaunchKernelGGL((k_fft_z_green<L>), dim3(ntile, c->ny), dim3(CT::THREADS), 0, c->stream,
-                     c->spec, c->greens, c->nxh, c->ny, ntile, c->tw_z, c->escape, c->iter);
+  hipLaunchKernelGGL((k_fft_z<L, MODE>), dim3(ntile, c->ny), dim3(CT::THREADS), 0, c->stream, src, dst,
+                     g, c->nxh, c->ny, ntile, c->tw_z, c->escape, c->iter);
 }
 
 template <bool INV>
-void y_pass(pmgpu_ctx* c) {
+void y_pass(pmgpu_ctx* c, float2* spec) {
   switch (c->ny) {
-    case 32: launch_y<32, INV>(c); break;
-    case 64: launch_y<64, INV>(c); break;
-    case 128: launch_y<128, INV>(c); break;
-    case 256: launch_y<256, INV>(c); break;
-    case 512: launch_y<512, INV>(c); break;
-    case 1024: launch_y<1024, INV>(c); break;
+    case 32: launch_y<32, INV>(c, spec); break;
+    case 64: launch_y<64, INV>(c, spec); break;
+    case 128: launch_y<128, INV>(c, spec); break;
+    case 256: launch_y<256, INV>(c, spec); break;
+    case 512: launch_y<512, INV>(c, spec); break;
+    case 1024: launch_y<1024, INV>(c, spec); break;
     default: fail(PMGPU_ERR_INVALID_ARGUMENT, "unsupported ny");
   }
   HIP_CHECK(hipGetLastError());
 }
 
-void z_pass(pmgpu_ctx* c) {
+template <int MODE>
+void z_pass(pmgpu_ctx* c, const float2* src, float2* dst, const float* g) {
   switch (c->nz) {
-    case 32: launch_z<32>(c); break;
-    case 64: launch_z<64>(c); break;
-    case 128: launch_z<128>(c); break;
-    case 256: launch_z<256>(c); break;
-    case 512: launch_z<512>(c); break;
-    case 1024: launch_z<1024>(c); break;
+    case 32: launch_z<32, MODE>(c, src, dst, g); break;
+    case 64: launch_z<64, MODE>(c, src, dst, g); break;
+    case 128: launch_z<128, MODE>(c, src, dst, g); break;
+    case 256: launch_z<256, MODE>(c, src, dst, g); break;
+    case 512: launch_z<512, MODE>(c, src, dst, g); break;
+    case 1024: launch_z<1024, MODE>(c, src, dst, g); break;
     default: fail(PMGPU_ERR_INVALID_ARGUMENT, "unsupported nz");
   }
+  HIP_CHECK(hipGetLastError());
+}
+
+void x_r2c(pmgpu_ctx* c, const float* in, float2* spec) {
+  const int L = c->nx / 2;
+  const dim3 gx(c->ny * c->nz / rows_per_block(L));
+  PMGPU_LAUNCH_X(k_fft_x_r2c, L, gx, c->stream, in, spec, c->tw_x, c->tw_r2c, c->escape, c->iter);
+  HIP_CHECK(hipGetLastError());
+}
+
+void x_c2r(pmgpu_ctx* c, const float2* spec, float* out) {
+  const int L = c->nx / 2;
+  const dim3 gx(c->ny * c->nz / rows_per_block(L));
+  PMGPU_LAUNCH_X(k_fft_x_c2r, L, gx, c->stream, spec, out, c->tw_x, c->tw_r2c, c->escape, c->iter);
   HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace
 
 void fft_forward_green_inverse(pmgpu_ctx* c) {
-  const int L = c->nx / 2;
-  const int rows = c->ny * c->nz;
-  const dim3 gx(rows / rows_per_block(L));
   int tok = stage_begin(c, "fft_x_r2c");
-  PMGPU_LAUNCH_X(k_fft_x_r2c, L, gx, c->stream, c->rho, c->spec, c->tw_x, c->tw_r2c, c->escape,
-                 c->iter);
-  HIP_CHECK(hipGetLastError());
+  x_r2c(c, c->rho, c->spec);
   stage_end(c, tok);
   tok = stage_begin(c, "fft_y_fwd");
-  y_pass<false>(c);
+  y_pass<false>(c, c->spec);
   stage_end(c, tok);
   tok = stage_begin(c, "fft_z_green");
-  z_pass(c);
+  z_pass<Z_GREEN>(c, c->spec, c->spec, c->greens);
   stage_end(c, tok);
   tok = stage_begin(c, "fft_y_inv");
-  y_pass<true>(c);
+  y_pass<true>(c, c->spec);
   stage_end(c, tok);
   tok = stage_begin(c, "fft_x_c2r");
-  PMGPU_LAUNCH_X(k_fft_x_c2r, L, gx, c->stream, c->spec, c->phi, c->tw_x, c->tw_r2c, c->escape,
-                 c->iter);
-  HIP_CHECK(hipGetLastError());
+  x_c2r(c, c->spec, c->phi);
   stage_end(c, tok);
+}
+
+void fft_r2c_forward(pmgpu_ctx* c, const float* in, float2* spec) {
+  x_r2c(c, in, spec);
+  y_pass<false>(c, spec);
+  z_pass<Z_FWD>(c, spec, spec, nullptr);
+}
+
+void fft_mul_c2r_inverse(pmgpu_ctx* c, const float2* src, const float* g, float2* work, float* out) {
+  z_pass<Z_MUL_INV>(c, src, work, g);
+  y_pass<true>(c, work);
+  x_c2r(c, work, out);
 }
 
 }  // namespace pmgpu

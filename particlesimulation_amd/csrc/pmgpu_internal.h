@@ -161,6 +161,10 @@ size_t scan_scratch_elems(int64_t n);
 std::vector<int4> deposit_range_table(int scheme);
 void deposit(pmgpu_ctx* c);
 void fft_forward_green_inverse(pmgpu_ctx* c);
+// unnormalised forward R2C of a real grid into a half spectrum (in may be rho)
+void fft_r2c_forward(pmgpu_ctx* c, const float* in, float2* spec);
+// out = unnormalised C2R of (src * g); src is kept, work holds the spectrum
+void fft_mul_c2r_inverse(pmgpu_ctx* c, const float2* src, const float* g, float2* work, float* out);
 void greens_compute(pmgpu_ctx* c, const float* mlp_dev);
 void interpolate(pmgpu_ctx* c, int mode, float kick, int iteration);
 void field_grid(pmgpu_ctx* c);
@@ -168,7 +172,18 @@ void leapfrog_op(pmgpu_ctx* c, int op, float dt, int iteration);
 void p3m_setup(pmgpu_ctx* c);
 void p3m_short_range(pmgpu_ctx* c, int mode, float kick, int iteration);
 void p3m_count_pairs(pmgpu_ctx* c, int64_t* accepted, int64_t* tested);
-float* mlp_correction_grid(pmgpu_ctx* c);  // device grid of MLP(k) on the half spectrum (caller frees)
+// learned correction (mlp.hip)
+struct MlpDevice;
+MlpDevice* mlp_device_create();
+void mlp_device_destroy(MlpDevice* d);
+void mlp_pack(MlpDevice* d, const float* params_dev, hipStream_t s);
+// out[k] = y(k) on the half spectrum, or greens[k] * (1 + y(k)) when greens is given
+void mlp_forward_grid(MlpDevice* d, int nx, int ny, int nz, int scheme, const float* greens, float* out,
+                      hipStream_t s);
+// grad (+)= scale * d/dparams sum_k up[k] y(k)
+void mlp_backward_grid(MlpDevice* d, int nx, int ny, int nz, int scheme, const float* up, float scale,
+                       bool accumulate, float* grad, hipStream_t s);
+void mlp_apply_to_greens(pmgpu_ctx* c);
 void escape_check(pmgpu_ctx* c, int32_t* flag_dev);
 void sr_table_host(float rc, float a, float eps, int cs, std::vector<float>& table, float& delta2);
 

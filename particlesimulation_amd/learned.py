@@ -1,0 +1,130 @@
+"""Learned influence-function correction: snapshots, training and evaluation.
+
+A snapshot is a seeded particle set in code units (grid cells) with a random
+subsample of target particles whose accelerations come from the direct sum
+on the GPU (`direct_sum`, FP32 with compensated sums).  Training fits the
+MLP factor (1 + y(k)) on the analytic G with the GPU's own forward and
+backward kernels (`Trainer`, `train`); the error measure is the reference's
+e_rms against the direct sum (script/pm_accuracy.py:44-48).
+
+    python -m particlesimulation_amd.learned --grid 64 --snapshots 24 --steps 300
+
+writes the trained parameters to an .npy file and prints the held-out error
+with and without the correction.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+import numpy as np
+
+from . import (CIC, DISCRETE_LAPLACIAN, FOUR_POINT, PM, TSC, TWO_POINT, Trainer, direct_sum, mlp_init, train)
+from . import ics
+
+
+def snapshot(grid, n, seed, kind="mixed", target_frac=0.05):
+    """Seeded particles in code units inside a periodic grid, and target indices.
+
+    kind 'plummer': a Plummer sphere (scale radius 1.5-4 cells) at a random
+    centre; 'clumps': uniform background plus Gaussian clumps (the large PM
+    workload's clustering at this grid); 'mixed' alternates by seed."""
+    rng = np.random.default_rng(seed)
+    g = np.array(grid, np.float64)
+    if kind == "mixed":
+        kind = "plummer" if seed % 2 == 0 else "clumps"
+    if kind == "plummer":
+        a = rng.uniform(1.5, 4.0)
+        pos, _, mass = ics.plummer(n, seed=seed, a=a, M=1.0, center=tuple(g / 2 + rng.uniform(-2, 2, 3)),
+                                   r_max=0.3 * g.min())
+    else:
+        n_clumps = int(rng.integers(4, 32))
+        pos, _, mass = ics.uniform_with_clumps(n, seed=seed, box=1.0, clump_frac=0.3, n_clumps=n_clumps,
+                                               sigma=float(rng.uniform(0.005, 0.02)))
+        pos = pos * g
+    pos = np.mod(pos, g).astype(np.float32)
+    mass = (mass / mass.sum()).astype(np.float32)
+    nt = max(1, int(round(target_frac * n)))
+    targets = np.sort(rng.choice(n, size=nt, replace=False))
+    return pos, mass, targets
+
+
+def targets_direct(pos, mass, targets, softening=0.0):
+    return direct_sum(pos, mass, pos[targets], softening)
+
+
+def pm_accel(pos, mass, grid, scheme, fds, gf, params=None, a=0.0):
+    pm = PM(grid, interpolation=scheme, finite_diff=fds, greens_function=gf, particle_diameter=a,
+            box=(0, 0, 0))
+    if params is not None:
+        pm.set_greens_correction(params)
+    pm.set_particles(pos, None, mass)
+    pm.init_greens()
+    pm.pm_step()
+    return pm.get_particles()[2]
+
+
+def e_rms(a, ref):
+    a = np.asarray(a, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.sqrt(((a - ref) ** 2).sum() / (ref ** 2).sum()))
+
+
+def build_trainer(grid, seeds, n, scheme=TSC, fds=FOUR_POINT, gf=DISCRETE_LAPLACIAN, kind="mixed",
+                  target_frac=0.05):
+    tr = Trainer(grid, interpolation=scheme, finite_diff=fds, greens_function=gf)
+    for s in seeds:
+        pos, mass, tgt = snapshot(grid, n, s, kind, target_frac)
+        tr.add_snapshot(pos, mass, tgt, targets_direct(pos, mass, tgt))
+    return tr
+
+
+def heldout_error(grid, seeds, n, params, scheme=TSC, fds=FOUR_POINT, gf=DISCRETE_LAPLACIAN, kind="mixed"):
+    """Pooled e_rms over held-out snapshots, plain and corrected."""
+    num_p = num_l = den = 0.0
+    for s in seeds:
+        pos, mass, tgt = snapshot(grid, n, s, kind)
+        ref = targets_direct(pos, mass, tgt).astype(np.float64)
+        a0 = pm_accel(pos, mass, grid, scheme, fds, gf)[tgt]
+        a1 = pm_accel(pos, mass, grid, scheme, fds, gf, params)[tgt]
+        num_p += ((a0 - ref) ** 2).sum()
+        num_l += ((a1 - ref) ** 2).sum()
+        den += (ref ** 2).sum()
+    return float(np.sqrt(num_p / den)), float(np.sqrt(num_l / den))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--grids", default="64,128", help="training grid sizes")
+    ap.add_argument("--snapshots", type=int, default=100, help="snapshots per grid size")
+    ap.add_argument("--particles", type=int, default=65536)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--scheme", type=int, default=TSC)
+    ap.add_argument("--fds", type=int, default=FOUR_POINT)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", default="learned_correction.npy")
+    args = ap.parse_args(argv)
+    t0 = time.perf_counter()
+    trainers = []
+    for gi, g in enumerate(int(x) for x in args.grids.split(",")):
+        seeds = range(1000 * (gi + 1), 1000 * (gi + 1) + args.snapshots)
+        trainers.append(build_trainer((g, g, g), seeds, args.particles, args.scheme, args.fds))
+    t_data = time.perf_counter() - t0
+    params = mlp_init(args.seed)
+    t0 = time.perf_counter()
+    params, hist = train(trainers, params, args.steps, args.lr)
+    t_train = time.perf_counter() - t0
+    np.save(args.out, params)
+    report = {"train_loss_first": float(hist[0]), "train_loss_last": float(hist[-1]), "data_s": t_data,
+              "train_s": t_train}
+    for g in (int(x) for x in args.grids.split(",")):
+        plain, learned = heldout_error((g, g, g), range(90000, 90008), args.particles, params, args.scheme,
+                                       args.fds)
+        report[f"heldout_{g}"] = {"plain": plain, "learned": learned}
+    print(json.dumps(report))
+
+
+if __name__ == "__main__":
+    main()
