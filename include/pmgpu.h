@@ -263,9 +263,12 @@ int pmgpu_train(pmgpu_trainer** trainers, int32_t n_trainers, float* params, int
 /* a_i = -1/(4 pi) sum_j m_j d_ij / (|d_ij|^2 + eps^2)^(3/2), d_ij = x_i - x_j
    (code units, Plummer softening as source/ppMethod.cpp:23-36 with G' =
    1/(4 pi)), coincident pairs skipped; FP32 with compensated sums.  Host
-   arrays: pos n x 3, mass n, tpos nt x 3, out nt x 3. */
+   arrays: pos n x 3, mass n, tpos nt x 3, out nt x 3.  box NULL: isolated;
+   else the periodic force in a box of box[0..2] (nearest image plus a
+   tabulated Ewald correction on a neutralising background, the boundary
+   condition of the FFT force step). */
 int pmgpu_direct_sum(int64_t n, const float* pos, const float* mass, int64_t nt, const float* tpos,
-                     float softening, float* out);
+                     float softening, const float* box, float* out);
 
 #ifdef __cplusplus
 }

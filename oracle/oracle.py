@@ -79,6 +79,10 @@ def lib():
                                      C.c_int, _f64p]
         L.orc_num_threads.restype = C.c_int
         L.orc_set_num_threads.argtypes = [C.c_int]
+        L.orc_ewald_accel.argtypes = [_f64p, C.c_double, _f64p, _f64p]
+        L.orc_ewald_accel.restype = None
+        L.orc_direct_sum_periodic.argtypes = [C.c_int64, C.c_void_p, C.c_int64, _f32p, _f32p, C.c_double, _f64p,
+                                              _f64p]
         L.orc_mlp_forward.argtypes = [_f64p, C.c_int64, _f64p, _f64p]
         L.orc_mlp_forward.restype = None
         L.orc_mlp_backward.argtypes = [_f64p, C.c_int64, _f64p, _f64p, _f64p]
@@ -257,6 +261,30 @@ def direct_sum(pos, mass, Gc, eps=0.0, kernel=0, targets=None):
         nt, tptr = len(keep), keep.ctypes.data
     acc = np.zeros(3 * nt, np.float64)
     lib().orc_direct_sum(nt, tptr, len(mass), pos.ravel(), mass, float(Gc), float(eps), int(kernel), acc)
+    return acc.reshape(-1, 3)
+
+
+def ewald_accel(box, x, alpha=None):
+    """Periodic acceleration of a unit mass at the origin (G = 1, neutralising background)."""
+    L = np.ascontiguousarray(box, np.float64)
+    a = 2.0 / L.min() if alpha is None else float(alpha)
+    out = np.zeros(3)
+    lib().orc_ewald_accel(L, a, np.ascontiguousarray(x, np.float64), out)
+    return out
+
+
+def direct_sum_periodic(pos, mass, Gc, box, targets=None):
+    """FP64 periodic (Ewald) direct sum on the target subset."""
+    pos = _pos(pos)
+    mass = np.ascontiguousarray(mass, np.float32)
+    if targets is None:
+        nt, tptr = len(mass), None
+    else:
+        keep = np.ascontiguousarray(targets, np.int64)
+        nt, tptr = len(keep), keep.ctypes.data
+    acc = np.zeros(3 * nt, np.float64)
+    lib().orc_direct_sum_periodic(nt, tptr, len(mass), pos.ravel(), mass, float(Gc),
+                                  np.ascontiguousarray(box, np.float64), acc)
     return acc.reshape(-1, 3)
 
 

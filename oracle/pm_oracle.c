@@ -998,3 +998,72 @@ void orc_mlp_backward(const double* p, int64_t n, const double* x, const double*
     }
   }
 }
+
+/* ------------------------------------------------------------------------
+ * Periodic direct sum (no reference counterpart; the reference's PM is
+ * periodic through its FFT, source/FFTWAdapter.cpp, while its direct sum
+ * source/ppMethod.cpp:23-36 is isolated).  Ewald summation of the
+ * acceleration of a unit mass at the origin on a neutralising background,
+ * G = 1:
+ *   a(x) = -sum_n (x-nL)/r^3 [erfc(alpha r) + 2 alpha r/sqrt(pi) e^{-alpha^2 r^2}]
+ *          - (4 pi/V) sum_{k != 0} k/k^2 e^{-k^2/(4 alpha^2)} sin(k.x),
+ *   k = 2 pi (hx/Lx, hy/Ly, hz/Lz).
+ * Pinned by independence of alpha, periodicity, antisymmetry and the
+ * near-field limit -x/|x|^3 + (4 pi / 3V) x of a cubic box.
+ * ------------------------------------------------------------------------ */
+void orc_ewald_accel(const double L[3], double alpha, const double x[3], double out[3]) {
+  double V = L[0] * L[1] * L[2];
+  double a[3] = {0.0, 0.0, 0.0};
+  int R[3], Hm[3];
+  for (int c = 0; c < 3; ++c) {
+    R[c] = (int)ceil(6.0 / (alpha * L[c]) + 0.5);
+    Hm[c] = (int)ceil(sqrt(36.0) * 2.0 * alpha * L[c] / (2.0 * kPi));
+  }
+  for (int i = -R[0]; i <= R[0]; ++i)
+    for (int j = -R[1]; j <= R[1]; ++j)
+      for (int k = -R[2]; k <= R[2]; ++k) {
+        double d[3] = {x[0] - i * L[0], x[1] - j * L[1], x[2] - k * L[2]};
+        double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (r2 == 0.0) continue;
+        double r = sqrt(r2);
+        double f = (erfc(alpha * r) + 2.0 * alpha * r / sqrt(kPi) * exp(-alpha * alpha * r2)) / (r2 * r);
+        for (int c = 0; c < 3; ++c) a[c] -= f * d[c];
+      }
+  for (int i = -Hm[0]; i <= Hm[0]; ++i)
+    for (int j = -Hm[1]; j <= Hm[1]; ++j)
+      for (int k = -Hm[2]; k <= Hm[2]; ++k) {
+        if (i == 0 && j == 0 && k == 0) continue;
+        double kv[3] = {2.0 * kPi * i / L[0], 2.0 * kPi * j / L[1], 2.0 * kPi * k / L[2]};
+        double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+        double e = k2 / (4.0 * alpha * alpha);
+        if (e > 36.0) continue;
+        double f = 4.0 * kPi / V * exp(-e) / k2 * sin(kv[0] * x[0] + kv[1] * x[1] + kv[2] * x[2]);
+        for (int c = 0; c < 3; ++c) a[c] -= f * kv[c];
+      }
+  for (int c = 0; c < 3; ++c) out[c] = a[c];
+}
+
+/* a_t = Gc sum_j m_j a_ewald(x_t - x_j) over all j != t (exact per pair). */
+int orc_direct_sum_periodic(int64_t nt, const int64_t* targets, int64_t n, const float* pos, const float* mass,
+                            double Gc, const double L[3], double* acc) {
+  double Lmin = L[0] < L[1] ? (L[0] < L[2] ? L[0] : L[2]) : (L[1] < L[2] ? L[1] : L[2]);
+  double alpha = 2.0 / Lmin;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int64_t t = 0; t < nt; ++t) {
+    int64_t i = targets ? targets[t] : t;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int64_t j = 0; j < n; ++j) {
+      if (j == i) continue;
+      double d[3], a[3];
+      for (int c = 0; c < 3; ++c) {
+        d[c] = (double)pos[3 * i + c] - (double)pos[3 * j + c];
+        d[c] -= L[c] * floor(d[c] / L[c] + 0.5); /* nearest image; a is periodic */
+      }
+      if (d[0] == 0.0 && d[1] == 0.0 && d[2] == 0.0) continue;
+      orc_ewald_accel(L, alpha, d, a);
+      for (int c = 0; c < 3; ++c) s[c] += (double)mass[j] * a[c];
+    }
+    for (int c = 0; c < 3; ++c) acc[3 * t + c] = Gc * s[c];
+  }
+  return 0;
+}

@@ -144,7 +144,7 @@ def load_library(path: str = LIB_PATH):
     L.pmgpu_trainer_add_snapshot.argtypes = [vp, C.c_int64, fp, fp, C.c_int64, C.POINTER(C.c_int64), fp]
     L.pmgpu_trainer_loss_grad.argtypes = [vp, fp, C.POINTER(C.c_double), fp, fp]
     L.pmgpu_train.argtypes = [C.POINTER(vp), C.c_int32, fp, C.c_int32, C.c_float, fp]
-    L.pmgpu_direct_sum.argtypes = [C.c_int64, fp, fp, C.c_int64, fp, C.c_float, fp]
+    L.pmgpu_direct_sum.argtypes = [C.c_int64, fp, fp, C.c_int64, fp, C.c_float, fp, fp]
     _lib = L
     return L
 
@@ -395,13 +395,15 @@ def mlp_backward(params, inputs, up):
     return grad
 
 
-def direct_sum(pos, mass, targets, softening=0.0):
-    """Direct-sum accelerations (code units, G' = 1/(4 pi)) at target positions, on the GPU."""
+def direct_sum(pos, mass, targets, softening=0.0, box=None):
+    """Direct-sum accelerations (code units, G' = 1/(4 pi)) at target positions, on the GPU;
+    periodic (Ewald) in `box` when given."""
     L = load_library()
     pos, mass, targets = _f32(pos, (-1, 3)), _f32(mass), _f32(targets, (-1, 3))
     out = np.zeros((len(targets), 3), np.float32)
+    b = None if box is None else _f32(box)
     rc = L.pmgpu_direct_sum(len(mass), _fptr(pos), _fptr(mass), len(targets), _fptr(targets), float(softening),
-                            _fptr(out))
+                            _fptr(b), _fptr(out))
     if rc != OK:
         _raise(rc, "pmgpu_direct_sum failed")
     return out

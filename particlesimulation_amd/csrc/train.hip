@@ -26,7 +26,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <array>
+#include <map>
+#include <mutex>
 #include <numeric>
+#include <thread>
 
 namespace pmgpu {
 
@@ -214,7 +218,13 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 
 // Direct sum a_i = -G' sum_j m_j d_ij / (|d_ij|^2 + eps^2)^(3/2), d_ij = x_i - x_j,
 // skipping coincident particles, FP32 with compensated (Neumaier) sums.
+// PERIODIC: d_ij is the nearest image in the box and the Ewald correction
+// (the periodic force minus the nearest-image Newtonian force, for a unit
+// mass on a uniform neutralising background) is added by trilinear
+// interpolation in a table over the positive octant [0, L/2]^3; the
+// correction is odd in the matching component and even in the others.
 constexpr int DS_THREADS = 256;
+constexpr int EW_N = 64;  // table intervals per half box side
 
 __device__ __forceinline__ void neumaier(float& s, float& c, float v) {
   const float t = s + v;
@@ -222,9 +232,37 @@ __device__ __forceinline__ void neumaier(float& s, float& c, float v) {
   s = t;
 }
 
+struct Box {
+  float L[3], invL[3], tab_scale[3];  // tab_scale = EW_N / (L/2)
+};
+
+__device__ __forceinline__ float nearest(float d, float L, float invL) { return d - L * rintf(d * invL); }
+
+__device__ __forceinline__ float3 ewald_corr(const float4* __restrict__ tab, const Box& b, float dx, float dy,
+                                             float dz) {
+  const float ux = fminf(fabsf(dx) * b.tab_scale[0], (float)EW_N - 1e-3f);
+  const float uy = fminf(fabsf(dy) * b.tab_scale[1], (float)EW_N - 1e-3f);
+  const float uz = fminf(fabsf(dz) * b.tab_scale[2], (float)EW_N - 1e-3f);
+  const int ix = (int)ux, iy = (int)uy, iz = (int)uz;
+  const float fx = ux - ix, fy = uy - iy, fz = uz - iz;
+  constexpr int S1 = EW_N + 1, S2 = (EW_N + 1) * (EW_N + 1);
+  const float4* t = tab + ix + S1 * iy + S2 * iz;
+  auto lerp4 = [](float4 a, float4 c, float f) {
+    return make_float4(fmaf(f, c.x - a.x, a.x), fmaf(f, c.y - a.y, a.y), fmaf(f, c.z - a.z, a.z), 0.f);
+  };
+  const float4 c00 = lerp4(t[0], t[1], fx), c10 = lerp4(t[S1], t[S1 + 1], fx);
+  const float4 c01 = lerp4(t[S2], t[S2 + 1], fx), c11 = lerp4(t[S2 + S1], t[S2 + S1 + 1], fx);
+  const float4 c0 = lerp4(c00, c10, fy), c1 = lerp4(c01, c11, fy);
+  const float4 c = lerp4(c0, c1, fz);
+  // odd in its own coordinate: c(-d) = -c(d) componentwise (the tabulated value may be negative)
+  return make_float3(dx < 0.f ? -c.x : c.x, dy < 0.f ? -c.y : c.y, dz < 0.f ? -c.z : c.z);
+}
+
+template <bool PERIODIC>
 __global__ __launch_bounds__(DS_THREADS) void k_direct_sum(const float4* __restrict__ src, int64_t n,
                                                            const float4* __restrict__ tgt, int64_t nt,
-                                                           float eps2, float4* __restrict__ out) {
+                                                           float eps2, Box box, const float4* __restrict__ tab,
+                                                           float4* __restrict__ out) {
   __shared__ float4 sh[DS_THREADS];
   const int64_t i = (int64_t)blockIdx.x * DS_THREADS + threadIdx.x;
   const float4 p = i < nt ? tgt[i] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -237,7 +275,12 @@ __global__ __launch_bounds__(DS_THREADS) void k_direct_sum(const float4* __restr
     float bx = 0.f, by = 0.f, bz = 0.f;  // block partials, then compensated into the total
     for (int k = 0; k < cnt; ++k) {
       const float4 q = sh[k];
-      const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+      float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+      if constexpr (PERIODIC) {
+        dx = nearest(dx, box.L[0], box.invL[0]);
+        dy = nearest(dy, box.L[1], box.invL[1]);
+        dz = nearest(dz, box.L[2], box.invL[2]);
+      }
       const float r2 = dx * dx + dy * dy + dz * dz;
       if (r2 > 0.f) {
         const float s = r2 + eps2;
@@ -246,6 +289,14 @@ __global__ __launch_bounds__(DS_THREADS) void k_direct_sum(const float4* __restr
         bx = fmaf(f, dx, bx);
         by = fmaf(f, dy, by);
         bz = fmaf(f, dz, bz);
+        if constexpr (PERIODIC) {
+          // the correction is an acceleration (G = 1 units); the Newtonian part
+          // above enters with a minus sign below
+          const float3 c = ewald_corr(tab, box, dx, dy, dz);
+          bx = fmaf(-q.w, c.x, bx);
+          by = fmaf(-q.w, c.y, by);
+          bz = fmaf(-q.w, c.z, bz);
+        }
       }
     }
     neumaier(sx, cx, bx);
@@ -254,6 +305,85 @@ __global__ __launch_bounds__(DS_THREADS) void k_direct_sum(const float4* __restr
     __syncthreads();
   }
   if (i < nt) out[i] = make_float4(-kGc * (sx + cx), -kGc * (sy + cy), -kGc * (sz + cz), 0.f);
+}
+
+// Periodic acceleration of a unit mass at the origin on a neutralising
+// background, G = 1 (Ewald summation):
+//   a(x) = -sum_n (x-nL)/r^3 [erfc(alpha r) + 2 alpha r/sqrt(pi) exp(-alpha^2 r^2)]
+//          - (4 pi / V) sum_{k != 0} k/k^2 exp(-k^2 / (4 alpha^2)) sin(k.x)
+// with k = 2 pi (hx/Lx, hy/Ly, hz/Lz); alpha = 2 / min(L); the sums are cut
+// where the terms fall below 1e-12 of the leading one.
+void ewald_accel(const double L[3], const double x[3], double out[3]) {
+  const double Lmin = std::min(L[0], std::min(L[1], L[2]));
+  const double alpha = 2.0 / Lmin, V = L[0] * L[1] * L[2];
+  double a[3] = {0.0, 0.0, 0.0};
+  int R[3], Hm[3];
+  for (int c = 0; c < 3; ++c) {
+    R[c] = (int)std::ceil(5.0 / (alpha * L[c]) + 0.5);
+    Hm[c] = (int)std::ceil(std::sqrt(28.0) * 2.0 * alpha * L[c] / (2.0 * M_PI));
+  }
+  for (int i = -R[0]; i <= R[0]; ++i)
+    for (int j = -R[1]; j <= R[1]; ++j)
+      for (int k = -R[2]; k <= R[2]; ++k) {
+        const double d[3] = {x[0] - i * L[0], x[1] - j * L[1], x[2] - k * L[2]};
+        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (r2 == 0.0) continue;
+        const double r = std::sqrt(r2);
+        const double f = (std::erfc(alpha * r) + 2.0 * alpha * r / std::sqrt(M_PI) * std::exp(-alpha * alpha * r2)) /
+                         (r2 * r);
+        for (int c = 0; c < 3; ++c) a[c] -= f * d[c];
+      }
+  for (int i = -Hm[0]; i <= Hm[0]; ++i)
+    for (int j = -Hm[1]; j <= Hm[1]; ++j)
+      for (int k = -Hm[2]; k <= Hm[2]; ++k) {
+        if (i == 0 && j == 0 && k == 0) continue;
+        const double kv[3] = {2.0 * M_PI * i / L[0], 2.0 * M_PI * j / L[1], 2.0 * M_PI * k / L[2]};
+        const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+        const double e = k2 / (4.0 * alpha * alpha);
+        if (e > 28.0) continue;
+        const double f = 4.0 * M_PI / V * std::exp(-e) / k2 * std::sin(kv[0] * x[0] + kv[1] * x[1] + kv[2] * x[2]);
+        for (int c = 0; c < 3; ++c) a[c] -= f * kv[c];
+      }
+  for (int c = 0; c < 3; ++c) out[c] = a[c];
+}
+
+// Table of the correction a_ewald(x) - (-x/|x|^3) on the octant grid; 0 at x = 0.
+std::vector<float4> ewald_table_compute(const float Lf[3]) {
+  const double L[3] = {Lf[0], Lf[1], Lf[2]};
+  constexpr int S = EW_N + 1;
+  std::vector<float4> tab((size_t)S * S * S);
+  auto work = [&](int z0, int z1) {
+    for (int iz = z0; iz < z1; ++iz)
+      for (int iy = 0; iy < S; ++iy)
+        for (int ix = 0; ix < S; ++ix) {
+          const double x[3] = {0.5 * L[0] * ix / EW_N, 0.5 * L[1] * iy / EW_N, 0.5 * L[2] * iz / EW_N};
+          const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+          float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (r2 > 0.0) {
+            double a[3];
+            ewald_accel(L, x, a);
+            const double r3 = r2 * std::sqrt(r2);
+            v = make_float4((float)(a[0] + x[0] / r3), (float)(a[1] + x[1] / r3), (float)(a[2] + x[2] / r3), 0.f);
+          }
+          tab[ix + (size_t)S * (iy + (size_t)S * iz)] = v;
+        }
+  };
+  const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  for (int t = 0; t < nth; ++t) th.emplace_back(work, S * t / nth, S * (t + 1) / nth);
+  for (auto& t : th) t.join();
+  return tab;
+}
+
+// Tables are cached per box (a training set reuses one box for every snapshot).
+const std::vector<float4>& ewald_table(const float L[3]) {
+  static std::mutex mu;
+  static std::map<std::array<float, 3>, std::vector<float4>> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  const std::array<float, 3> key = {L[0], L[1], L[2]};
+  auto it = cache.find(key);
+  if (it == cache.end()) it = cache.emplace(key, ewald_table_compute(L)).first;
+  return it->second;
 }
 
 }  // namespace
@@ -628,10 +758,12 @@ int pmgpu_train(pmgpu_trainer** trs, int32_t ntr, float* params, int32_t steps, 
 }
 
 int pmgpu_direct_sum(int64_t n, const float* pos, const float* mass, int64_t nt, const float* tpos,
-                     float softening, float* out) {
+                     float softening, const float* box, float* out) {
   if (n < 0 || nt < 0 || (n > 0 && (!pos || !mass)) || (nt > 0 && (!tpos || !out))) return PMGPU_ERR_INVALID_ARGUMENT;
+  const bool periodic = box != nullptr;
+  if (periodic && !(box[0] > 0.f && box[1] > 0.f && box[2] > 0.f)) return PMGPU_ERR_INVALID_ARGUMENT;
   if (nt == 0) return PMGPU_OK;
-  float4 *src = nullptr, *tgt = nullptr, *res = nullptr;
+  float4 *src = nullptr, *tgt = nullptr, *res = nullptr, *tab = nullptr;
   int rc = PMGPU_OK;
   try {
     std::vector<float4> hs(std::max<int64_t>(n, 1)), ht(nt), hr(nt);
@@ -642,8 +774,22 @@ int pmgpu_direct_sum(int64_t n, const float* pos, const float* mass, int64_t nt,
     dmalloc(&res, nt);
     if (n > 0) HIP_CHECK(hipMemcpy(src, hs.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(tgt, ht.data(), sizeof(float4) * nt, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_direct_sum, dim3(blocks(nt, DS_THREADS)), dim3(DS_THREADS), 0, nullptr, src, n, tgt, nt,
-                       softening * softening, res);
+    Box b{};
+    if (periodic) {
+      const std::vector<float4>& t = ewald_table(box);
+      dmalloc(&tab, (int64_t)t.size());
+      HIP_CHECK(hipMemcpy(tab, t.data(), sizeof(float4) * t.size(), hipMemcpyHostToDevice));
+      for (int c = 0; c < 3; ++c) {
+        b.L[c] = box[c];
+        b.invL[c] = 1.0f / box[c];
+        b.tab_scale[c] = (float)EW_N / (0.5f * box[c]);
+      }
+      hipLaunchKernelGGL(k_direct_sum<true>, dim3(blocks(nt, DS_THREADS)), dim3(DS_THREADS), 0, nullptr, src, n,
+                         tgt, nt, softening * softening, b, tab, res);
+    } else {
+      hipLaunchKernelGGL(k_direct_sum<false>, dim3(blocks(nt, DS_THREADS)), dim3(DS_THREADS), 0, nullptr, src, n,
+                         tgt, nt, softening * softening, b, (const float4*)nullptr, res);
+    }
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpy(hr.data(), res, sizeof(float4) * nt, hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < nt; ++i) {
@@ -657,6 +803,7 @@ int pmgpu_direct_sum(int64_t n, const float* pos, const float* mass, int64_t nt,
   dfree(src);
   dfree(tgt);
   dfree(res);
+  dfree(tab);
   return rc;
 }
 

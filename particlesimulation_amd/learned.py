@@ -1,8 +1,10 @@
 """Learned influence-function correction: snapshots, training and evaluation.
 
 A snapshot is a seeded particle set in code units (grid cells) with a random
-subsample of target particles whose accelerations come from the direct sum
-on the GPU (`direct_sum`, FP32 with compensated sums).  Training fits the
+subsample of target particles whose accelerations come from the periodic
+direct sum on the GPU (`direct_sum` with the grid as box: nearest image plus
+Ewald correction, the boundary condition of the FFT force step; FP32 with
+compensated sums).  Training fits the
 MLP factor (1 + y(k)) on the analytic G with the GPU's own forward and
 backward kernels (`Trainer`, `train`); the error measure is the reference's
 e_rms against the direct sum (script/pm_accuracy.py:44-48).
@@ -20,7 +22,7 @@ import time
 
 import numpy as np
 
-from . import (CIC, DISCRETE_LAPLACIAN, FOUR_POINT, PM, TSC, TWO_POINT, Trainer, direct_sum, mlp_init, train)
+from . import DISCRETE_LAPLACIAN, FOUR_POINT, PM, TSC, Trainer, direct_sum, mlp_init, train
 from . import ics
 
 
@@ -28,8 +30,9 @@ def snapshot(grid, n, seed, kind="mixed", target_frac=0.05):
     """Seeded particles in code units inside a periodic grid, and target indices.
 
     kind 'plummer': a Plummer sphere (scale radius 1.5-4 cells) at a random
-    centre; 'clumps': uniform background plus Gaussian clumps (the large PM
-    workload's clustering at this grid); 'mixed' alternates by seed."""
+    centre; 'clumps': 70 % uniform background plus 30 % in 4-32 Gaussian
+    clumps of width 1-4 cells (the clustered PM workload's clumps are 2.6
+    cells wide on its 256^3 grid); 'mixed' alternates by seed."""
     rng = np.random.default_rng(seed)
     g = np.array(grid, np.float64)
     if kind == "mixed":
@@ -40,8 +43,9 @@ def snapshot(grid, n, seed, kind="mixed", target_frac=0.05):
                                    r_max=0.3 * g.min())
     else:
         n_clumps = int(rng.integers(4, 32))
+        sigma_cells = float(rng.uniform(1.0, 4.0))
         pos, _, mass = ics.uniform_with_clumps(n, seed=seed, box=1.0, clump_frac=0.3, n_clumps=n_clumps,
-                                               sigma=float(rng.uniform(0.005, 0.02)))
+                                               sigma=sigma_cells / float(g.min()))
         pos = pos * g
     pos = np.mod(pos, g).astype(np.float32)
     mass = (mass / mass.sum()).astype(np.float32)
@@ -50,8 +54,9 @@ def snapshot(grid, n, seed, kind="mixed", target_frac=0.05):
     return pos, mass, targets
 
 
-def targets_direct(pos, mass, targets, softening=0.0):
-    return direct_sum(pos, mass, pos[targets], softening)
+def targets_direct(pos, mass, targets, grid, softening=0.0):
+    """Periodic direct-sum accelerations of the target particles (GPU)."""
+    return direct_sum(pos, mass, pos[targets], softening, box=grid)
 
 
 def pm_accel(pos, mass, grid, scheme, fds, gf, params=None, a=0.0):
@@ -76,7 +81,7 @@ def build_trainer(grid, seeds, n, scheme=TSC, fds=FOUR_POINT, gf=DISCRETE_LAPLAC
     tr = Trainer(grid, interpolation=scheme, finite_diff=fds, greens_function=gf)
     for s in seeds:
         pos, mass, tgt = snapshot(grid, n, s, kind, target_frac)
-        tr.add_snapshot(pos, mass, tgt, targets_direct(pos, mass, tgt))
+        tr.add_snapshot(pos, mass, tgt, targets_direct(pos, mass, tgt, grid))
     return tr
 
 
@@ -85,7 +90,7 @@ def heldout_error(grid, seeds, n, params, scheme=TSC, fds=FOUR_POINT, gf=DISCRET
     num_p = num_l = den = 0.0
     for s in seeds:
         pos, mass, tgt = snapshot(grid, n, s, kind)
-        ref = targets_direct(pos, mass, tgt).astype(np.float64)
+        ref = targets_direct(pos, mass, tgt, grid).astype(np.float64)
         a0 = pm_accel(pos, mass, grid, scheme, fds, gf)[tgt]
         a1 = pm_accel(pos, mass, grid, scheme, fds, gf, params)[tgt]
         num_p += ((a0 - ref) ** 2).sum()
