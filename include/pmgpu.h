@@ -213,6 +213,43 @@ int pmgpu_get_grid(pmgpu_ctx* ctx, int32_t which, float* out);
 int pmgpu_set_greens_correction(pmgpu_ctx* ctx, const float* weights);
 int64_t pmgpu_mlp_num_params(void);
 
+/* ---- ensembles -------------------------------------------------------- */
+/* No counterpart in the reference.  nsim independent simulations (each with
+   its own pmgpu_pm_params, and pmgpu_p3m_params when use_p3m[i] != 0),
+   stepped together by one HIP graph launch per two run-loop iterations.
+   lanes (1..16) is the number of concurrent chains in the graph.  prepare
+   computes the Green's functions, runs the initial force, half kick and
+   first drift, autotunes the launch knobs per class of member when asked,
+   and captures the graphs.  Steps must be even.  Members are periodic
+   (escapes are recorded but do not stop a member). */
+typedef struct pmgpu_batch pmgpu_batch;
+int pmgpu_batch_create(int32_t nsim, const pmgpu_pm_params* params, const pmgpu_p3m_params* p3m,
+                       const int32_t* use_p3m, int32_t lanes, pmgpu_batch** out);
+void pmgpu_batch_destroy(pmgpu_batch* batch);
+const char* pmgpu_batch_last_error(const pmgpu_batch* batch);
+int pmgpu_batch_size(const pmgpu_batch* batch);
+int pmgpu_batch_set_particles(pmgpu_batch* batch, int32_t sim, int64_t n, const float* pos, const float* vel,
+                              const float* mass);
+int pmgpu_batch_prepare(pmgpu_batch* batch, int32_t autotune);
+int pmgpu_batch_step(pmgpu_batch* batch, int32_t steps);
+int pmgpu_batch_synchronize(pmgpu_batch* batch);
+/* wall milliseconds of `steps` ensemble steps (events on the ensemble stream) */
+int pmgpu_batch_time(pmgpu_batch* batch, int32_t steps, double* ms);
+/* per-member step latency (each member's own graph, alone), ms[nsim] */
+int pmgpu_batch_member_latency(pmgpu_batch* batch, int32_t reps, double* ms);
+int pmgpu_batch_get_particles(pmgpu_batch* batch, int32_t sim, float* pos, float* vel, float* acc);
+int pmgpu_batch_get_tuning(pmgpu_batch* batch, int32_t sim, int32_t* knobs);
+
+/* ---- launch tuning ------------------------------------------------------ */
+/* No counterpart in the reference.  knobs[0] particles per deposit work item
+   (1024..16384, default 4096), knobs[1] particles per interpolation work item
+   (256..16384, default 2048), knobs[2] workgroups per CU of the persistent
+   deposit and interpolation kernels (1..32, default 8).  Results do not
+   depend on them (the deposit's fixed-point sums are order-independent). */
+#define PMGPU_TUNING_KNOBS 3
+int pmgpu_set_tuning(pmgpu_ctx* ctx, const int32_t* knobs);
+int pmgpu_get_tuning(pmgpu_ctx* ctx, int32_t* knobs);
+
 /* ---- profiling -------------------------------------------------------- */
 /* When enabled, every kernel of the force step is bracketed by HIP events
    on the context's stream; pmgpu_profile_read returns per-stage totals

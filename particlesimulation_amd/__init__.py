@@ -51,7 +51,10 @@ EXPORTS = (
     "pmgpu_profile_enable", "pmgpu_profile_reset", "pmgpu_profile_read", "pmgpu_p3m_count_pairs",
     "pmgpu_greens_value", "pmgpu_diagnostics", "pmgpu_mlp_forward", "pmgpu_mlp_backward",
     "pmgpu_trainer_create", "pmgpu_trainer_destroy", "pmgpu_trainer_last_error", "pmgpu_trainer_add_snapshot",
-    "pmgpu_trainer_loss_grad", "pmgpu_train", "pmgpu_direct_sum",
+    "pmgpu_trainer_loss_grad", "pmgpu_train", "pmgpu_direct_sum", "pmgpu_set_tuning", "pmgpu_get_tuning",
+    "pmgpu_batch_create", "pmgpu_batch_destroy", "pmgpu_batch_last_error", "pmgpu_batch_size",
+    "pmgpu_batch_set_particles", "pmgpu_batch_prepare", "pmgpu_batch_step", "pmgpu_batch_synchronize",
+    "pmgpu_batch_time", "pmgpu_batch_member_latency", "pmgpu_batch_get_particles", "pmgpu_batch_get_tuning",
 )
 
 
@@ -134,6 +137,23 @@ def load_library(path: str = LIB_PATH):
     L.pmgpu_greens_value.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_double] + [C.c_int32] * 6
     L.pmgpu_greens_value.restype = C.c_double
     L.pmgpu_diagnostics.argtypes = [vp, C.POINTER(C.c_double)]
+    L.pmgpu_batch_create.argtypes = [C.c_int32, C.POINTER(PMParams), C.POINTER(P3MParams), C.POINTER(C.c_int32),
+                                     C.c_int32, C.POINTER(vp)]
+    L.pmgpu_batch_destroy.argtypes = [vp]
+    L.pmgpu_batch_destroy.restype = None
+    L.pmgpu_batch_last_error.argtypes = [vp]
+    L.pmgpu_batch_last_error.restype = C.c_char_p
+    L.pmgpu_batch_size.argtypes = [vp]
+    L.pmgpu_batch_set_particles.argtypes = [vp, C.c_int32, C.c_int64, fp, fp, fp]
+    L.pmgpu_batch_prepare.argtypes = [vp, C.c_int32]
+    L.pmgpu_batch_step.argtypes = [vp, C.c_int32]
+    L.pmgpu_batch_synchronize.argtypes = [vp]
+    L.pmgpu_batch_time.argtypes = [vp, C.c_int32, C.POINTER(C.c_double)]
+    L.pmgpu_batch_member_latency.argtypes = [vp, C.c_int32, C.POINTER(C.c_double)]
+    L.pmgpu_batch_get_particles.argtypes = [vp, C.c_int32, fp, fp, fp]
+    L.pmgpu_batch_get_tuning.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
+    L.pmgpu_set_tuning.argtypes = [vp, C.POINTER(C.c_int32)]
+    L.pmgpu_get_tuning.argtypes = [vp, C.POINTER(C.c_int32)]
     L.pmgpu_mlp_forward.argtypes = [fp, C.c_int64, fp, fp]
     L.pmgpu_mlp_backward.argtypes = [fp, C.c_int64, fp, fp, fp]
     L.pmgpu_trainer_create.argtypes = [C.POINTER(PMParams), C.POINTER(vp)]
@@ -316,6 +336,16 @@ class PM:
         w = None if weights is None else _f32(weights)
         self._check(self._lib.pmgpu_set_greens_correction(self._ctx, _fptr(w)))
 
+    # -- launch tuning ----------------------------------------------------
+    def set_tuning(self, deposit_chunk=4096, interp_chunk=2048, blocks_per_cu=8):
+        k = (C.c_int32 * 3)(int(deposit_chunk), int(interp_chunk), int(blocks_per_cu))
+        self._check(self._lib.pmgpu_set_tuning(self._ctx, k))
+
+    def get_tuning(self):
+        k = (C.c_int32 * 3)()
+        self._check(self._lib.pmgpu_get_tuning(self._ctx, k))
+        return tuple(k)
+
     # -- profiling --------------------------------------------------------
     def profile(self, on=True):
         self._check(self._lib.pmgpu_profile_enable(self._ctx, int(bool(on))))
@@ -476,3 +506,103 @@ def train(trainers, params, steps, lr=1e-3):
     if rc != OK:
         _raise(rc, L.pmgpu_trainer_last_error(trainers[0]._t).decode())
     return p, hist
+
+
+# ---------------------------------------------------------------------------
+# ensembles
+# ---------------------------------------------------------------------------
+
+def pm_params(grid, interpolation=CIC, finite_diff=TWO_POINT, greens_function=DISCRETE_LAPLACIAN,
+              particle_diameter=0.0, box=(0.0, 0.0, 0.0)):
+    p = PMParams()
+    p.grid[:] = [int(g) for g in grid]
+    p.interpolation = int(interpolation)
+    p.finite_diff = int(finite_diff)
+    p.greens_function = int(greens_function)
+    p.particle_diameter = float(particle_diameter)
+    p.box[:] = [float(b) for b in box]
+    p.ext_kind = EXT_NONE
+    p.H = p.DT = 1.0
+    return p
+
+
+def p3m_params(cutoff, particle_diameter, softening=0.0, cloud_shape=S2, use_table=2, box=(0.0, 0.0, 0.0)):
+    q = P3MParams()
+    q.cutoff = float(cutoff)
+    q.particle_diameter = float(particle_diameter)
+    q.softening = float(softening)
+    q.cloud_shape = int(cloud_shape)
+    q.use_table = int(use_table)
+    q.box[:] = [float(b) for b in box]
+    return q
+
+
+class Batch:
+    """Independent simulations stepped together by one graph launch per two steps."""
+
+    def __init__(self, members, lanes=8):
+        """members: list of (PMParams, P3MParams or None)."""
+        L = load_library()
+        n = len(members)
+        pa = (PMParams * n)(*[m[0] for m in members])
+        qa = (P3MParams * n)(*[m[1] if m[1] is not None else P3MParams() for m in members])
+        use = (C.c_int32 * n)(*[1 if m[1] is not None else 0 for m in members])
+        self._lib = L
+        self._b = C.c_void_p()
+        rc = L.pmgpu_batch_create(n, pa, qa, use, int(lanes), C.byref(self._b))
+        if rc != OK:
+            _raise(rc, "pmgpu_batch_create failed")
+        self.size = n
+        self.grids = [tuple(m[0].grid) for m in members]
+
+    def _check(self, rc):
+        if rc != OK:
+            _raise(rc, self._lib.pmgpu_batch_last_error(self._b).decode())
+
+    def set_particles(self, i, pos, vel, mass):
+        pos, mass = _f32(pos, (-1, 3)), _f32(mass)
+        vel = None if vel is None else _f32(vel, (-1, 3))
+        self._check(self._lib.pmgpu_batch_set_particles(self._b, int(i), len(mass), _fptr(pos), _fptr(vel),
+                                                        _fptr(mass)))
+
+    def prepare(self, autotune=True):
+        self._check(self._lib.pmgpu_batch_prepare(self._b, int(bool(autotune))))
+
+    def step(self, steps):
+        self._check(self._lib.pmgpu_batch_step(self._b, int(steps)))
+
+    def synchronize(self):
+        self._check(self._lib.pmgpu_batch_synchronize(self._b))
+
+    def time(self, steps):
+        ms = C.c_double(0.0)
+        self._check(self._lib.pmgpu_batch_time(self._b, int(steps), C.byref(ms)))
+        return ms.value
+
+    def member_latency(self, reps=5):
+        out = (C.c_double * self.size)()
+        self._check(self._lib.pmgpu_batch_member_latency(self._b, int(reps), out))
+        return np.array(out[:])
+
+    def get_particles(self, i, n):
+        pos = np.zeros((n, 3), np.float32)
+        vel = np.zeros((n, 3), np.float32)
+        acc = np.zeros((n, 3), np.float32)
+        self._check(self._lib.pmgpu_batch_get_particles(self._b, int(i), _fptr(pos), _fptr(vel), _fptr(acc)))
+        return pos, vel, acc
+
+    def tuning(self, i):
+        k = (C.c_int32 * 3)()
+        self._check(self._lib.pmgpu_batch_get_tuning(self._b, int(i), k))
+        return tuple(k)
+
+    def close(self):
+        if self._b:
+            self._lib.pmgpu_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

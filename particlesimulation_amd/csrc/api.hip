@@ -132,7 +132,7 @@ static void alloc_particle_arrays(pmgpu_ctx* c, int64_t n) {
   dalloc(&c->key, n);
   dalloc(&c->rank, n);
   dfree(&c->vint);
-  c->fx_slots = 8 * n / DEPOSIT_CHUNK + 2;
+  c->fx_slots = 8 * n / DEPOSIT_CHUNK_MIN + 2;  // enough for any tunable chunk size
   dalloc(&c->fx_accum, c->fx_slots * TILE_CELLS);
   dalloc(&c->slot_tile, c->fx_slots);
   if (c->p3m_ready) {
@@ -579,6 +579,30 @@ int pmgpu_get_grid(pmgpu_ctx* c, int32_t which, float* out) {
     default:
       fail(PMGPU_ERR_INVALID_ARGUMENT, "unknown grid");
   }
+  PMGPU_API_END(c)
+}
+
+int pmgpu_set_tuning(pmgpu_ctx* c, const int32_t* knobs) {
+  PMGPU_API_BEGIN(c)
+  if (!knobs) fail(PMGPU_ERR_INVALID_ARGUMENT, "knobs is NULL");
+  const int dep = knobs[0], itp = knobs[1], bpc = knobs[2];
+  if (dep < DEPOSIT_CHUNK_MIN || dep > DEPOSIT_CHUNK_MAX)
+    fail(PMGPU_ERR_INVALID_ARGUMENT, "deposit chunk out of range");
+  if (itp < INTERP_CHUNK_MIN || itp > INTERP_CHUNK_MAX)
+    fail(PMGPU_ERR_INVALID_ARGUMENT, "interpolation chunk out of range");
+  if (bpc < 1 || bpc > BLOCKS_PER_CU_MAX) fail(PMGPU_ERR_INVALID_ARGUMENT, "blocks per CU out of range");
+  c->dep_chunk = dep;
+  c->int_chunk = itp;
+  c->blocks_per_cu = bpc;
+  PMGPU_API_END(c)
+}
+
+int pmgpu_get_tuning(pmgpu_ctx* c, int32_t* knobs) {
+  PMGPU_API_BEGIN(c)
+  if (!knobs) fail(PMGPU_ERR_INVALID_ARGUMENT, "knobs is NULL");
+  knobs[0] = c->dep_chunk;
+  knobs[1] = c->int_chunk;
+  knobs[2] = c->blocks_per_cu;
   PMGPU_API_END(c)
 }
 

@@ -4,7 +4,7 @@
 // Each tile (TILE_X x TILE_Y x TILE_Z cells) gathers every particle whose
 // assignment stencil reaches into it: its own particles (one contiguous
 // range) plus the boundary-class ranges of its 26 neighbours.  That virtual
-// list is cut into work items of DEPOSIT_CHUNK particles.  A work item sums
+// list is cut into work items of dep_chunk particles (tunable).  A work item sums
 // its contributions in LDS as 64-bit fixed point (integer adds are
 // associative, so the sum does not depend on thread or atomic order), then
 //  - a tile with one work item writes its 512 cells straight to the density
@@ -87,8 +87,8 @@ __global__ void k_deposit_plan(int ntiles, int ntx, int nty, int ntz, int nrange
                                const int32_t* __restrict__ cls_start,
                                int32_t* __restrict__ dep_nchunk, int32_t* __restrict__ int_nchunk,
                                int32_t* __restrict__ slot, int32_t* __restrict__ slot_tile,
-                               int32_t* __restrict__ counters, int64_t max_slots,
-                               const int32_t* __restrict__ escape, int iteration) {
+                               int32_t* __restrict__ counters, int64_t max_slots, int dep_chunk,
+                               int int_chunk, const int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
@@ -100,10 +100,10 @@ __global__ void k_deposit_plan(int ntiles, int ntx, int nty, int ntz, int nrange
     if (r == 0) own = l;
     total += l;
   }
-  int nch = (int)((total + DEPOSIT_CHUNK - 1) / DEPOSIT_CHUNK);
+  int nch = (int)((total + dep_chunk - 1) / dep_chunk);
   if (nch < 1) nch = 1;
   dep_nchunk[t] = nch;
-  int_nchunk[t] = (own + INTERP_CHUNK - 1) / INTERP_CHUNK;
+  int_nchunk[t] = (own + int_chunk - 1) / int_chunk;
   int s = -1;
   if (nch > 1) {
     s = atomicAdd(&counters[CNT_SLOTS], 1);
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit(
     int ntiles, int scheme, int nranges, const int4* __restrict__ table,
     const int32_t* __restrict__ cls_start, const int32_t* __restrict__ chunk_start,
     const int32_t* __restrict__ slot, long long* __restrict__ fx_accum, int shift,
-    float* __restrict__ rho, const int32_t* __restrict__ escape, int iteration) {
+    float* __restrict__ rho, int dep_chunk, const int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
   __shared__ unsigned long long acc[TILE_CELLS];
   __shared__ int32_t rstart[MAX_RANGES], rpre[MAX_RANGES + 1];
@@ -163,8 +163,8 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit(
     __syncthreads();
     const int ox = (t % ntx) * TILE_X, oy = ((t / ntx) % nty) * TILE_Y,
               oz = (t / (ntx * nty)) * TILE_Z;
-    const int vbeg = s_chunk * DEPOSIT_CHUNK;
-    const int vend = min(vbeg + DEPOSIT_CHUNK, rpre[nranges]);
+    const int vbeg = s_chunk * dep_chunk;
+    const int vend = min(vbeg + dep_chunk, rpre[nranges]);
     for (int v = vbeg + (int)threadIdx.x; v < vend; v += DEP_THREADS) {
       // range containing virtual index v
       int a = 0, b = nranges;
@@ -282,19 +282,19 @@ void deposit(pmgpu_ctx* c) {
   hipLaunchKernelGGL(k_deposit_plan, dim3((c->ntiles + 255) / 256), dim3(256), 0, c->stream,
                      c->ntiles, c->ntx, c->nty, c->ntz, nranges, c->dep_table, c->cls_start,
                      c->tile_nchunk, c->int_nchunk, c->tile_slot, c->slot_tile, c->counters,
-                     c->fx_slots, c->escape, c->iter);
+                     c->fx_slots, c->dep_chunk, c->int_chunk, c->escape, c->iter);
   HIP_CHECK(hipGetLastError());
   exclusive_scan_i32(c, c->tile_nchunk, c->tile_chunk_start, c->ntiles);
   exclusive_scan_i32(c, c->int_nchunk, c->int_chunk_start, c->ntiles);
   stage_end(c, tok);
 
   tok = stage_begin(c, "deposit");
-  int64_t bound = (int64_t)c->ntiles + 8 * c->n / DEPOSIT_CHUNK + 1;
-  int grid = (int)std::min<int64_t>(bound, (int64_t)c->num_cu * 8);
+  int64_t bound = (int64_t)c->ntiles + 8 * c->n / c->dep_chunk + 1;
+  int grid = (int)std::min<int64_t>(bound, (int64_t)c->num_cu * c->blocks_per_cu);
   hipLaunchKernelGGL(k_deposit, dim3(grid), dim3(DEP_THREADS), 0, c->stream, c->pm[c->cur], c->nx,
                      c->ny, c->nz, c->ntx, c->nty, c->ntz, c->ntiles, c->prm.interpolation, nranges,
                      c->dep_table, c->cls_start, c->tile_chunk_start, c->tile_slot, c->fx_accum,
-                     c->fx_shift, c->rho, c->escape, c->iter);
+                     c->fx_shift, c->rho, c->dep_chunk, c->escape, c->iter);
   HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_deposit_finalize, dim3(256), dim3(256), 0, c->stream, c->nx, c->ny, c->ntx,
                      c->nty, c->fx_accum, c->slot_tile, c->counters, c->fx_slots, c->fx_shift,

@@ -3,7 +3,7 @@
 // next drift (source/leapfrog.cpp:16-24).
 //
 // One work item = one tile's own particles (a contiguous range of the
-// tile-sorted arrays, cut into INTERP_CHUNK pieces).  The work item stages
+// tile-sorted arrays, cut into int_chunk pieces, tunable).  The work item stages
 // the potential of the tile plus the stencil and finite-difference halo in
 // LDS, forms E = -grad(phi) on the stencil region in LDS (2- or 4-point,
 // pmMethod.cpp:355-365), and interpolates with the deposit's weights.  The
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void k_interp(
     const float* __restrict__ phi, int nx, int ny, int nz, int ntx, int nty, int ntiles,
     const int32_t* __restrict__ cls_start, const int32_t* __restrict__ chunk_start,
     float4* __restrict__ pm, float4* __restrict__ vel, float4* __restrict__ acc, int mode,
-    float kick, float3 box, ExtField ext, int32_t* __restrict__ escape, int iteration) {
+    float kick, float3 box, ExtField ext, int int_chunk, int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
   using Gm = Geo<SCHEME, FDS>;
   __shared__ float sphi[Gm::PX * Gm::PY * Gm::PZ];
@@ -142,8 +142,8 @@ __global__ __launch_bounds__(256) void k_interp(
       sE[2][e] = Ez;
     }
     __syncthreads();
-    const int pbeg = cls_start[t * NCLASS] + s_chunk * INTERP_CHUNK;
-    const int pend = min(pbeg + INTERP_CHUNK, cls_start[t * NCLASS + NCLASS]);
+    const int pbeg = cls_start[t * NCLASS] + s_chunk * int_chunk;
+    const int pend = min(pbeg + int_chunk, cls_start[t * NCLASS + NCLASS]);
     for (int i = pbeg + (int)threadIdx.x; i < pend; i += 256) {
       float4 p = pm[i];
       const int ax = anchor_of(p.x, SCHEME), ay = anchor_of(p.y, SCHEME), az = anchor_of(p.z, SCHEME);
@@ -257,7 +257,7 @@ void launch_interp(pmgpu_ctx* c, int grid, int mode, float kick, float3 box, Ext
                    int iteration) {
   hipLaunchKernelGGL((k_interp<S, F>), dim3(grid), dim3(256), 0, c->stream, c->phi, c->nx, c->ny,
                      c->nz, c->ntx, c->nty, c->ntiles, c->cls_start, c->int_chunk_start,
-                     c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick, box, e, c->escape,
+                     c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick, box, e, c->int_chunk, c->escape,
                      iteration);
 }
 
@@ -283,8 +283,8 @@ static float3 box_of(pmgpu_ctx* c) {
 
 void interpolate(pmgpu_ctx* c, int mode, float kick, int iteration) {
   int tok = stage_begin(c, "interp");
-  int64_t bound = (int64_t)c->ntiles + c->n / INTERP_CHUNK + 1;
-  int grid = (int)std::min<int64_t>(bound, (int64_t)c->num_cu * 8);
+  int64_t bound = (int64_t)c->ntiles + c->n / c->int_chunk + 1;
+  int grid = (int)std::min<int64_t>(bound, (int64_t)c->num_cu * c->blocks_per_cu);
   const ExtField e = ext_of(c);
   const float3 box = box_of(c);
   const int s = c->prm.interpolation, f = c->prm.finite_diff;

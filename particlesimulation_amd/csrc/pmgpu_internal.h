@@ -24,8 +24,11 @@ constexpr int TILE_Z = 4;
 constexpr int TILE_CELLS = TILE_X * TILE_Y * TILE_Z;  // 512
 constexpr int NCLASS = 27;
 // Work-item size of the deposit (virtual particles per work item).
-constexpr int DEPOSIT_CHUNK = 4096;
-constexpr int INTERP_CHUNK = 2048;
+// work-item sizes (particles) of the deposit and interpolation passes: defaults
+// and limits of the tunable values (pmgpu_set_tuning)
+constexpr int DEPOSIT_CHUNK = 4096, DEPOSIT_CHUNK_MIN = 1024, DEPOSIT_CHUNK_MAX = 16384;
+constexpr int INTERP_CHUNK = 2048, INTERP_CHUNK_MIN = 256, INTERP_CHUNK_MAX = 16384;
+constexpr int BLOCKS_PER_CU = 8, BLOCKS_PER_CU_MAX = 32;
 
 // P3M short range: SR cell side >= cutoff / SR_CELL_DIV, stencil +-SR_CELL_DIV.
 constexpr int SR_CELL_DIV = 2;
@@ -91,6 +94,9 @@ struct pmgpu_ctx {
   int4* dep_table = nullptr;            // neighbour (dx,dy,dz,class) ranges of the scheme
   int dep_table_len = 0;
   int num_cu = 256;
+  int dep_chunk = pmgpu::DEPOSIT_CHUNK;  // tunables (pmgpu_set_tuning)
+  int int_chunk = pmgpu::INTERP_CHUNK;
+  int blocks_per_cu = pmgpu::BLOCKS_PER_CU;
   int32_t* counters = nullptr;          // small device counters
   long long* fx_accum = nullptr;        // multi-chunk tiles x 512 fixed-point sums
   int64_t fx_slots = 0;

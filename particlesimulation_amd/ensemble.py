@@ -1,0 +1,100 @@
+"""Batched ensembles of independent PM / P3M simulations (one graph launch per
+two steps of every member).
+
+The ensemble: N log-uniform in [4096, 262144], grids drawn from {32^3, 64^3,
+128^3}, half Plummer spheres and half exponential disks (the generators of
+`ics`), CIC and TSC alternating; members with at most 32768 particles on TSC
+run P3M (S2 optimal G, a = r_c = 3 cells), the rest PM with the discrete
+Laplacian.  All members are periodic in a box of their grid.
+
+    python -m particlesimulation_amd.ensemble --members 256 --steps 20
+
+prints aggregate particle-updates/s, the members' step latency p50/p99 and
+the sequential (one member after another, each by its own graph) time.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+import numpy as np
+
+from . import (CIC, DISCRETE_LAPLACIAN, S2, S2_OPTIMAL, TSC, TWO_POINT, Batch, ics, p3m_params, pm_params)
+
+
+def member(i, rng):
+    n = int(round(np.exp(rng.uniform(np.log(4096), np.log(262144)))))
+    g = int(rng.choice([32, 64, 128]))
+    scheme = CIC if i % 2 == 0 else TSC
+    kind = "plummer" if i % 4 < 2 else "disk"
+    L = 8.0 if kind == "plummer" else 16.0
+    H = L / g
+    c = L / 2
+    seed = 1000 + i
+    if kind == "plummer":
+        pos, vel, m = ics.plummer(n, seed, a=1.0, M=1.0, center=(c, c, c), r_max=0.45 * L)
+    else:
+        pos, vel, m = ics.exponential_disk(n, seed, center=(c, c, c), r_max=0.4 * L)
+    pos, vel, m = ics.to_code_units(pos, vel, m, H, 0.01, 1.0)
+    pos = np.mod(pos, g).astype(np.float32)
+    p3m = scheme == TSC and n <= 32768
+    if p3m:
+        prm = pm_params((g, g, g), TSC, TWO_POINT, S2_OPTIMAL, particle_diameter=3.0)
+        q = p3m_params(3.0, 3.0, 0.0, S2, use_table=2)
+    else:
+        prm = pm_params((g, g, g), scheme, TWO_POINT, DISCRETE_LAPLACIAN)
+        q = None
+    return {"n": n, "grid": g, "scheme": scheme, "kind": kind, "p3m": p3m, "params": prm, "p3m_params": q,
+            "pos": pos, "vel": vel, "mass": m}
+
+
+def make_ensemble(members=256, seed=42):
+    rng = np.random.default_rng(seed)
+    return [member(i, rng) for i in range(members)]
+
+
+def build(ens, lanes=8, autotune=True):
+    b = Batch([(e["params"], e["p3m_params"]) for e in ens], lanes=lanes)
+    for i, e in enumerate(ens):
+        b.set_particles(i, e["pos"], e["vel"], e["mass"])
+    b.prepare(autotune)
+    return b
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--lanes", type=int, default=8)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--seed", type=int, default=42)
+    args = ap.parse_args(argv)
+    t0 = time.perf_counter()
+    ens = make_ensemble(args.members, args.seed)
+    t_ic = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b = build(ens, args.lanes, not args.no_autotune)
+    t_prep = time.perf_counter() - t0
+    b.step(2)  # warm
+    b.synchronize()
+    ms = b.time(args.steps)
+    total_n = sum(e["n"] for e in ens)
+    lat = b.member_latency(3)
+    out = {
+        "metric": "ensemble particle-updates/s", "members": args.members, "steps": args.steps,
+        "lanes": args.lanes, "autotune": not args.no_autotune,
+        "particles": total_n, "p3m_members": int(sum(e["p3m"] for e in ens)),
+        "ms_per_step": ms / args.steps, "value": total_n * args.steps / (ms * 1e-3),
+        "member_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                              "max": float(lat.max())},
+        "sequential_ms_per_step": float(lat.sum()),
+        "batch_speedup_vs_sequential": float(lat.sum() / (ms / args.steps)),
+        "setup_s": {"initial_conditions": t_ic, "prepare": t_prep},
+        "tuning_classes": len({b.tuning(i) for i in range(len(ens))}),
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

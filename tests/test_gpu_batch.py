@@ -1,0 +1,92 @@
+"""Ensembles stepped by one graph launch: every member matches the same
+simulation run alone through the C ABI (pmgpu_run), bitwise for PM members
+(deposit sums are order-independent, interpolation is per particle) and to
+FP32 rounding for P3M members (the short-range sum order follows the cell
+sort's order within a cell)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _members(P):
+    from particlesimulation_amd import ics
+    specs = [(32, P.CIC, False, 3000, "plummer"), (64, P.TSC, False, 20000, "plummer"),
+             (32, P.TSC, True, 4000, "plummer"), (64, P.CIC, False, 9000, "disk"),
+             (128, P.TSC, False, 30000, "disk"), (64, P.TSC, True, 6000, "disk")]
+    out = []
+    for i, (g, scheme, p3m, n, kind) in enumerate(specs):
+        L = 8.0 if kind == "plummer" else 16.0
+        c = L / 2
+        if kind == "plummer":
+            pos, vel, m = ics.plummer(n, 100 + i, a=1.0, center=(c, c, c), r_max=0.45 * L)
+        else:
+            pos, vel, m = ics.exponential_disk(n, 100 + i, center=(c, c, c), r_max=0.4 * L)
+        pos, vel, m = ics.to_code_units(pos, vel, m, L / g, 0.01, 1.0)
+        pos = np.mod(pos, g).astype(np.float32)
+        gf = P.S2_OPTIMAL if p3m else P.DISCRETE_LAPLACIAN
+        prm = P.pm_params((g, g, g), scheme, P.TWO_POINT, gf, particle_diameter=3.0 if p3m else 0.0)
+        q = P.p3m_params(3.0, 3.0, 0.0, P.S2, use_table=2) if p3m else None
+        out.append(dict(g=g, scheme=scheme, p3m=p3m, gf=gf, prm=prm, q=q, pos=pos, vel=vel, m=m))
+    return out
+
+
+@pytest.mark.parametrize("lanes,autotune", [(4, True), (1, False)])
+def test_batch_members_match_single_runs(gpu_lib, lanes, autotune):
+    P = gpu_lib
+    mem = _members(P)
+    b = P.Batch([(e["prm"], e["q"]) for e in mem], lanes=lanes)
+    for i, e in enumerate(mem):
+        b.set_particles(i, e["pos"], e["vel"], e["m"])
+    b.prepare(autotune)
+    b.step(4)
+    for i, e in enumerate(mem):
+        x, v, _ = b.get_particles(i, len(e["m"]))
+        pm = P.PM((e["g"],) * 3, interpolation=e["scheme"], finite_diff=P.TWO_POINT, greens_function=e["gf"],
+                  particle_diameter=3.0 if e["p3m"] else 0.0, box=(0, 0, 0))
+        pm.set_particles(e["pos"], e["vel"], e["m"])
+        pm.init_greens()
+        if e["p3m"]:
+            pm.p3m_configure(cutoff=3.0, particle_diameter=3.0, softening=0.0, cloud_shape=P.S2, use_table=2,
+                             box=(0, 0, 0))
+        # prepare = initial force, half kick, first drift; 4 batch steps = 4 more
+        # iterations whose drift of the next step is fused
+        assert pm.run(4, initial=True, p3m=e["p3m"]) == 4
+        pm.leapfrog(P.DRIFT, 1.0)
+        rx, rv, _ = pm.get_particles()
+        if e["p3m"]:
+            assert np.max(np.abs(x - rx)) <= 1e-5 * e["g"], i
+            assert np.linalg.norm(v - rv) <= 1e-4 * np.linalg.norm(rv), i
+        else:
+            assert np.array_equal(x, rx), i
+            assert np.array_equal(v, rv), i
+        knobs = b.tuning(i)
+        assert len(knobs) == 3 and all(k > 0 for k in knobs)
+    ms = b.time(2)
+    lat = b.member_latency(2)
+    assert ms > 0 and np.all(lat > 0)
+    b.close()
+
+
+def test_tuning_does_not_change_results(gpu_lib):
+    P = gpu_lib
+    rng = np.random.default_rng(3)
+    g = 64
+    pos = np.mod(g / 2 + rng.standard_normal((50000, 3)) * 5, g).astype(np.float32)
+    m = np.full(50000, 1.0 / 50000, np.float32)
+    ref = None
+    for knobs in ((4096, 2048, 8), (1024, 512, 2), (16384, 16384, 32), (2048, 256, 1)):
+        pm = P.PM((g, g, g), interpolation=P.TSC, finite_diff=P.FOUR_POINT, greens_function=P.DISCRETE_LAPLACIAN,
+                  box=(0, 0, 0))
+        pm.set_tuning(*knobs)
+        assert pm.get_tuning() == knobs
+        pm.set_particles(pos, None, m)
+        pm.init_greens()
+        pm.pm_step()
+        out = (pm.density(), pm.get_particles()[2])
+        if ref is None:
+            ref = out
+        else:
+            assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]), knobs
+    with pytest.raises(P.InvalidArgument):
+        pm.set_tuning(512, 2048, 8)
