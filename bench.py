@@ -133,6 +133,8 @@ def main():
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--sr-div", type=int, default=None,
+                    help="short-range cell division (cells of side >= cutoff/div); default: the library's")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by "
@@ -148,6 +150,10 @@ def main():
     log(f"[bench] initial conditions {time.perf_counter() - t0:.1f}s")
 
     pm = make_context(P, w, pos, vel, m)
+    if args.sr_div is not None:
+        k = list(pm.get_tuning())
+        k[3] = args.sr_div
+        pm.set_tuning(*k)
     err = None if args.no_error else force_error(P, w, pm, pos, m)
     if err:
         log(f"[bench] force error vs direct sum: {err}")
@@ -180,7 +186,7 @@ def main():
                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                     "traffic": None, "avg_launch_ms": avg_s * 1e3, "ordered_pairs": pairs,
                     "tested_pairs": tested, "flop_per_pair": SR_FLOP_PER_PAIR,
-                    "sr_table_mode": w.sr_table}
+                    "sr_table_mode": w.sr_table, "sr_cell_division": pm.get_tuning()[3]}
     else:
         roofline = None
     # algorithmic-byte rates of the bandwidth-bound stages (per launch)

@@ -244,9 +244,12 @@ int pmgpu_batch_get_tuning(pmgpu_batch* batch, int32_t sim, int32_t* knobs);
 /* No counterpart in the reference.  knobs[0] particles per deposit work item
    (1024..16384, default 4096), knobs[1] particles per interpolation work item
    (256..16384, default 2048), knobs[2] workgroups per CU of the persistent
-   deposit and interpolation kernels (1..32, default 8).  Results do not
-   depend on them (the deposit's fixed-point sums are order-independent). */
-#define PMGPU_TUNING_KNOBS 3
+   deposit and interpolation kernels (1..32, default 8), knobs[3] the
+   short-range cell division: cells of side >= cutoff / knobs[3] and a
+   (2 knobs[3] + 1)^3 neighbourhood (2..4, default 2).  The PM results do
+   not depend on them (the deposit's fixed-point sums are order-independent);
+   the short-range sum order, and so its rounding, follows knobs[3]. */
+#define PMGPU_TUNING_KNOBS 4
 int pmgpu_set_tuning(pmgpu_ctx* ctx, const int32_t* knobs);
 int pmgpu_get_tuning(pmgpu_ctx* ctx, int32_t* knobs);
 

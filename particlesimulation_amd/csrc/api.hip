@@ -143,6 +143,27 @@ static void alloc_particle_arrays(pmgpu_ctx* c, int64_t n) {
   }
 }
 
+// The short-range cell grid: side >= cutoff / sr_div over the particle box.
+void sr_cells_setup(pmgpu_ctx* c) {
+  const double side = (double)c->p3m.cutoff / c->sr_div * (1.0 + 1e-5);
+  int m[3];
+  for (int i = 0; i < 3; ++i) m[i] = std::max(1, (int)std::floor((double)c->p3m.box[i] / side));
+  if ((int64_t)m[0] * m[1] * m[2] > 64ll * 1024 * 1024)
+    fail(PMGPU_ERR_UNSUPPORTED, "cutoff too small for the box (too many short-range cells)");
+  c->mcx = m[0];
+  c->mcy = m[1];
+  c->mcz = m[2];
+  c->hcx = c->p3m.box[0] / m[0];
+  c->hcy = c->p3m.box[1] / m[1];
+  c->hcz = c->p3m.box[2] / m[2];
+  c->ncell_sr = m[0] * m[1] * m[2];
+  dalloc(&c->sr_count, c->ncell_sr + 1);
+  dalloc(&c->sr_start, c->ncell_sr + 1);
+  dalloc(&c->sr_nchunk, c->ncell_sr + 1);
+  dalloc(&c->sr_chunk_start, c->ncell_sr + 1);
+  ensure_scan_scratch(c, c->ncell_sr);
+}
+
 static void set_shift(pmgpu_ctx* c) {
   // every cell sum of the deposit is bounded by 1.5 * sum|m| (the largest
   // weight product of any scheme is 1.125^3 < 1.5); keep it below 2^62
@@ -423,25 +444,9 @@ int pmgpu_p3m_configure(pmgpu_ctx* c, const pmgpu_p3m_params* p) {
     if (!(p->box[i] > 0.0f)) fail(PMGPU_ERR_INVALID_ARGUMENT, "box must be > 0");
   HIP_CHECK(hipStreamSynchronize(c->stream));
   c->p3m = *p;
-  const double side = 0.5 * (double)p->cutoff * (1.0 + 1e-5);
-  int m[3];
-  for (int i = 0; i < 3; ++i) m[i] = std::max(1, (int)std::floor((double)p->box[i] / side));
-  if ((int64_t)m[0] * m[1] * m[2] > 64ll * 1024 * 1024)
-    fail(PMGPU_ERR_UNSUPPORTED, "cutoff too small for the box (too many short-range cells)");
-  c->mcx = m[0];
-  c->mcy = m[1];
-  c->mcz = m[2];
-  c->hcx = p->box[0] / m[0];
-  c->hcy = p->box[1] / m[1];
-  c->hcz = p->box[2] / m[2];
-  c->ncell_sr = m[0] * m[1] * m[2];
-  dalloc(&c->sr_count, c->ncell_sr + 1);
-  dalloc(&c->sr_start, c->ncell_sr + 1);
-  dalloc(&c->sr_nchunk, c->ncell_sr + 1);
-  dalloc(&c->sr_chunk_start, c->ncell_sr + 1);
+  sr_cells_setup(c);
   dalloc(&c->sr_table, SR_TABLE_ACC);  // the larger of the two tables
   dalloc(&c->pair_count, 2);
-  ensure_scan_scratch(c, c->ncell_sr);
   c->p3m_ready = true;
   if (c->n > 0) {
     dalloc(&c->sr_key, c->n);
@@ -585,15 +590,23 @@ int pmgpu_get_grid(pmgpu_ctx* c, int32_t which, float* out) {
 int pmgpu_set_tuning(pmgpu_ctx* c, const int32_t* knobs) {
   PMGPU_API_BEGIN(c)
   if (!knobs) fail(PMGPU_ERR_INVALID_ARGUMENT, "knobs is NULL");
-  const int dep = knobs[0], itp = knobs[1], bpc = knobs[2];
+  const int dep = knobs[0], itp = knobs[1], bpc = knobs[2], div = knobs[3];
   if (dep < DEPOSIT_CHUNK_MIN || dep > DEPOSIT_CHUNK_MAX)
     fail(PMGPU_ERR_INVALID_ARGUMENT, "deposit chunk out of range");
   if (itp < INTERP_CHUNK_MIN || itp > INTERP_CHUNK_MAX)
     fail(PMGPU_ERR_INVALID_ARGUMENT, "interpolation chunk out of range");
   if (bpc < 1 || bpc > BLOCKS_PER_CU_MAX) fail(PMGPU_ERR_INVALID_ARGUMENT, "blocks per CU out of range");
+  if (div < SR_CELL_DIV || div > SR_CELL_DIV_MAX) fail(PMGPU_ERR_INVALID_ARGUMENT, "short-range cell division out of range");
   c->dep_chunk = dep;
   c->int_chunk = itp;
   c->blocks_per_cu = bpc;
+  if (div != c->sr_div) {
+    c->sr_div = div;
+    if (c->p3m_ready) {
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+      sr_cells_setup(c);
+    }
+  }
   PMGPU_API_END(c)
 }
 
@@ -603,6 +616,7 @@ int pmgpu_get_tuning(pmgpu_ctx* c, int32_t* knobs) {
   knobs[0] = c->dep_chunk;
   knobs[1] = c->int_chunk;
   knobs[2] = c->blocks_per_cu;
+  knobs[3] = c->sr_div;
   PMGPU_API_END(c)
 }
 

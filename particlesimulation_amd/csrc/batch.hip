@@ -77,31 +77,42 @@ float time_member(pmgpu_ctx* c, bool p3m, int reps) {
   return ms / reps;
 }
 
-// Coordinate descent over the three knobs, starting from the defaults.
-void autotune_member(pmgpu_ctx* c, bool p3m, int32_t best[3]) {
-  const std::vector<int> cand[3] = {{1024, 2048, 4096, 8192}, {512, 1024, 2048, 4096}, {2, 4, 8, 16}};
+void apply_knobs(pmgpu_ctx* c, const int32_t k[4]) {
+  c->dep_chunk = k[0];
+  c->int_chunk = k[1];
+  c->blocks_per_cu = k[2];
+  if (k[3] != c->sr_div) {
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->sr_div = k[3];
+    if (c->p3m_ready) sr_cells_setup(c);
+  }
+}
+
+// Coordinate descent over the knobs (the short-range division only for P3M
+// members), starting from the defaults.
+void autotune_member(pmgpu_ctx* c, bool p3m, int32_t best[4]) {
+  const std::vector<int> cand[4] = {{1024, 2048, 4096, 8192}, {512, 1024, 2048, 4096}, {2, 4, 8, 16}, {2, 3, 4}};
   best[0] = c->dep_chunk;
   best[1] = c->int_chunk;
   best[2] = c->blocks_per_cu;
+  best[3] = c->sr_div;
   float best_ms = time_member(c, p3m, 3);
-  for (int k : {2, 0, 1}) {
+  std::vector<int> order = {2, 0, 1};
+  if (p3m) order.insert(order.begin(), 3);
+  for (int k : order) {
     for (int v : cand[k]) {
       if (v == best[k]) continue;
-      int32_t trial[3] = {best[0], best[1], best[2]};
+      int32_t trial[4] = {best[0], best[1], best[2], best[3]};
       trial[k] = v;
-      c->dep_chunk = trial[0];
-      c->int_chunk = trial[1];
-      c->blocks_per_cu = trial[2];
+      apply_knobs(c, trial);
       const float ms = time_member(c, p3m, 3);
       if (ms < 0.97f * best_ms) {
         best_ms = ms;
-        std::copy(trial, trial + 3, best);
+        std::copy(trial, trial + 4, best);
       }
     }
   }
-  c->dep_chunk = best[0];
-  c->int_chunk = best[1];
-  c->blocks_per_cu = best[2];
+  apply_knobs(c, best);
 }
 
 }  // namespace
@@ -214,22 +225,19 @@ int pmgpu_batch_prepare(pmgpu_batch* b, int32_t autotune) {
       auto it = rep.find(key);
       if (it == rep.end() || b->sims[it->second]->n < c->n) rep[key] = i;
     }
-    std::map<std::tuple<int, int, int, int, int, int>, std::array<int32_t, 3>> best;
+    std::map<std::tuple<int, int, int, int, int, int>, std::array<int32_t, 4>> best;
     for (auto& kv : rep) {
       pmgpu_ctx* c = b->sims[kv.second];
       c->iter = INT_MIN;
-      int32_t k[3];
+      int32_t k[4];
       autotune_member(c, b->use_p3m[kv.second] != 0, k);
-      best[kv.first] = {k[0], k[1], k[2]};
+      best[kv.first] = {k[0], k[1], k[2], k[3]};
     }
     for (int i = 0; i < n; ++i) {
       pmgpu_ctx* c = b->sims[i];
       auto key = std::make_tuple(c->nx, c->ny, c->nz, c->prm.interpolation, b->use_p3m[i],
                                  (int)std::floor(std::log2((double)c->n)));
-      const auto& k = best[key];
-      c->dep_chunk = k[0];
-      c->int_chunk = k[1];
-      c->blocks_per_cu = k[2];
+      apply_knobs(c, best[key].data());
     }
   }
   // initial force, half kick and first drift, on each member's own stream

@@ -5,11 +5,12 @@
 // a 13-cell half shell and per-neighbour accumulation slots on CPU threads.
 // Here:
 //  - particles are counting-sorted by a short-range cell of side
-//    >= cutoff/2 (non-periodic, as the chaining mesh; out-of-box particles
+//    >= cutoff/div (div = 2 by default, tunable; non-periodic, as the chaining mesh; out-of-box particles
 //    clamp to the edge cells, which never loses a pair because clamping is
 //    1-Lipschitz), into a compact float4 (x, y, z, m) copy;
 //  - one work item = one cell's chunk of 64 i-particles; its j-stream is the
-//    25 contiguous (dz, dy) rows of the 5x5x5 cell neighbourhood.  The four
+//    (2 div + 1)^2 contiguous (dz, dy) rows of the (2 div + 1)^3 cell
+//    neighbourhood.  The four
 //    waves of a workgroup take interleaved 64-particle blocks of that stream
 //    and read each j through the scalar unit (uniform address), so the inner
 //    loop is pure FP32 VALU with scalar operands;
@@ -105,7 +106,7 @@ __device__ __forceinline__ float reference_force(int cs, float r, float a) {
 }
 
 struct SrParams {
-  int mx, my, mz;
+  int mx, my, mz, div;
   float inv_d2, rc2;
   float a, eps2;
   int cs;
@@ -174,11 +175,11 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
     const float4 pi = valid ? sr_pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     float ax = 0.f, ay = 0.f, az = 0.f;
     int blk = 0;
-    const int xlo = max(cx - SR_CELL_DIV, 0), xhi = min(cx + SR_CELL_DIV, P.mx - 1);
-    for (int dz = -SR_CELL_DIV; dz <= SR_CELL_DIV; ++dz) {
+    const int xlo = max(cx - P.div, 0), xhi = min(cx + P.div, P.mx - 1);
+    for (int dz = -P.div; dz <= P.div; ++dz) {
       const int zc = cz + dz;
       if (zc < 0 || zc >= P.mz) continue;
-      for (int dy = -SR_CELL_DIV; dy <= SR_CELL_DIV; ++dy) {
+      for (int dy = -P.div; dy <= P.div; ++dy) {
         const int yc = cy + dy;
         if (yc < 0 || yc >= P.my) continue;
         const int row = P.mx * (yc + P.my * zc);
@@ -260,11 +261,11 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr_count(
     const bool valid = i < sr_start[c + 1];
     const float4 pi = valid ? sr_pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     int blk = 0;
-    const int xlo = max(cx - SR_CELL_DIV, 0), xhi = min(cx + SR_CELL_DIV, P.mx - 1);
-    for (int dz = -SR_CELL_DIV; dz <= SR_CELL_DIV; ++dz) {
+    const int xlo = max(cx - P.div, 0), xhi = min(cx + P.div, P.mx - 1);
+    for (int dz = -P.div; dz <= P.div; ++dz) {
       const int zc = cz + dz;
       if (zc < 0 || zc >= P.mz) continue;
-      for (int dy = -SR_CELL_DIV; dy <= SR_CELL_DIV; ++dy) {
+      for (int dy = -P.div; dy <= P.div; ++dy) {
         const int yc = cy + dy;
         if (yc < 0 || yc >= P.my) continue;
         const int row = P.mx * (yc + P.my * zc);
@@ -389,6 +390,7 @@ static SrParams sr_params(pmgpu_ctx* c) {
   P.mx = c->mcx;
   P.my = c->mcy;
   P.mz = c->mcz;
+  P.div = c->sr_div;
   P.inv_d2 = c->sr_inv_d2;
   P.rc2 = c->p3m.cutoff * c->p3m.cutoff;
   P.a = c->p3m.particle_diameter;

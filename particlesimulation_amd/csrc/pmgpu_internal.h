@@ -30,8 +30,8 @@ constexpr int DEPOSIT_CHUNK = 4096, DEPOSIT_CHUNK_MIN = 1024, DEPOSIT_CHUNK_MAX 
 constexpr int INTERP_CHUNK = 2048, INTERP_CHUNK_MIN = 256, INTERP_CHUNK_MAX = 16384;
 constexpr int BLOCKS_PER_CU = 8, BLOCKS_PER_CU_MAX = 32;
 
-// P3M short range: SR cell side >= cutoff / SR_CELL_DIV, stencil +-SR_CELL_DIV.
-constexpr int SR_CELL_DIV = 2;
+// P3M short range: cell side >= cutoff / div, stencil +-div cells (div tunable)
+constexpr int SR_CELL_DIV = 2, SR_CELL_DIV_MAX = 4;
 constexpr int SR_TABLE = 512;  // padded reference table (the reference uses 500 entries)
 constexpr int SR_TABLE_REF = 500;
 constexpr int SR_TABLE_ACC = 2048;  // accurate (r (s+eps^2)-scaled) table
@@ -97,6 +97,7 @@ struct pmgpu_ctx {
   int dep_chunk = pmgpu::DEPOSIT_CHUNK;  // tunables (pmgpu_set_tuning)
   int int_chunk = pmgpu::INTERP_CHUNK;
   int blocks_per_cu = pmgpu::BLOCKS_PER_CU;
+  int sr_div = pmgpu::SR_CELL_DIV;
   int32_t* counters = nullptr;          // small device counters
   long long* fx_accum = nullptr;        // multi-chunk tiles x 512 fixed-point sums
   int64_t fx_slots = 0;
@@ -176,6 +177,7 @@ void interpolate(pmgpu_ctx* c, int mode, float kick, int iteration);
 void field_grid(pmgpu_ctx* c);
 void leapfrog_op(pmgpu_ctx* c, int op, float dt, int iteration);
 void p3m_setup(pmgpu_ctx* c);
+void sr_cells_setup(pmgpu_ctx* c);  // short-range cell grid for c->p3m and c->sr_div
 void p3m_short_range(pmgpu_ctx* c, int mode, float kick, int iteration);
 void p3m_count_pairs(pmgpu_ctx* c, int64_t* accepted, int64_t* tested);
 // learned correction (mlp.hip)

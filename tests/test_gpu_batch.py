@@ -26,7 +26,7 @@ def _members(P):
         pos = np.mod(pos, g).astype(np.float32)
         gf = P.S2_OPTIMAL if p3m else P.DISCRETE_LAPLACIAN
         prm = P.pm_params((g, g, g), scheme, P.TWO_POINT, gf, particle_diameter=3.0 if p3m else 0.0)
-        q = P.p3m_params(3.0, 3.0, 0.0, P.S2, use_table=2) if p3m else None
+        q = P.p3m_params(3.0, 3.0, 0.0, P.S2, use_table=2, box=(g, g, g)) if p3m else None
         out.append(dict(g=g, scheme=scheme, p3m=p3m, gf=gf, prm=prm, q=q, pos=pos, vel=vel, m=m))
     return out
 
@@ -48,7 +48,7 @@ def test_batch_members_match_single_runs(gpu_lib, lanes, autotune):
         pm.init_greens()
         if e["p3m"]:
             pm.p3m_configure(cutoff=3.0, particle_diameter=3.0, softening=0.0, cloud_shape=P.S2, use_table=2,
-                             box=(0, 0, 0))
+                             box=(e["g"],) * 3)
         # prepare = initial force, half kick, first drift; 4 batch steps = 4 more
         # iterations whose drift of the next step is fused
         assert pm.run(4, initial=True, p3m=e["p3m"]) == 4
@@ -61,7 +61,7 @@ def test_batch_members_match_single_runs(gpu_lib, lanes, autotune):
             assert np.array_equal(x, rx), i
             assert np.array_equal(v, rv), i
         knobs = b.tuning(i)
-        assert len(knobs) == 3 and all(k > 0 for k in knobs)
+        assert len(knobs) == 4 and all(k > 0 for k in knobs)
     ms = b.time(2)
     lat = b.member_latency(2)
     assert ms > 0 and np.all(lat > 0)
@@ -75,7 +75,7 @@ def test_tuning_does_not_change_results(gpu_lib):
     pos = np.mod(g / 2 + rng.standard_normal((50000, 3)) * 5, g).astype(np.float32)
     m = np.full(50000, 1.0 / 50000, np.float32)
     ref = None
-    for knobs in ((4096, 2048, 8), (1024, 512, 2), (16384, 16384, 32), (2048, 256, 1)):
+    for knobs in ((4096, 2048, 8, 2), (1024, 512, 2, 2), (16384, 16384, 32, 3), (2048, 256, 1, 4)):
         pm = P.PM((g, g, g), interpolation=P.TSC, finite_diff=P.FOUR_POINT, greens_function=P.DISCRETE_LAPLACIAN,
                   box=(0, 0, 0))
         pm.set_tuning(*knobs)
@@ -89,4 +89,4 @@ def test_tuning_does_not_change_results(gpu_lib):
         else:
             assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]), knobs
     with pytest.raises(P.InvalidArgument):
-        pm.set_tuning(512, 2048, 8)
+        pm.set_tuning(512, 2048, 8, 2)
