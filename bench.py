@@ -63,26 +63,33 @@ def make_context(P, w, pos, vel, m):
 
 def force_error(P, w, pm, pos, m, n_targets=512, seed=7):
     """e_rms = sqrt(sum|F - F_ds|^2 / sum|F_ds|^2) and the reference's mean relative
-    error (script/pm_accuracy.py:44-48) against an FP64 direct sum (Newtonian,
-    non-periodic) over all particles, on a seeded subsample of targets."""
+    error (script/pm_accuracy.py:44-48) against a direct sum over all particles, on a
+    seeded subsample of targets.  Isolated workloads: FP64 Newtonian direct sum (numpy).
+    Periodic workloads: the GPU's periodic direct sum (nearest image + Ewald
+    correction, FP32 with compensated sums; checked against the FP64 oracle)."""
     pm.pm_step()
     if w.p3m:
         pm.p3m_short_range()
     _, _, acc = pm.get_particles()
     rng = np.random.default_rng(seed)
     t = rng.choice(len(m), size=min(n_targets, len(m)), replace=False)
-    x = pos.astype(np.float64)
-    mm = m.astype(np.float64)
-    ref = np.zeros((len(t), 3))
-    for k, i in enumerate(t):
-        d = x[i] - x
-        r2 = (d * d).sum(1)
-        r2[i] = np.inf
-        ref[k] = -GC * ((mm / (r2 * np.sqrt(r2)))[:, None] * d).sum(0)
+    if w.periodic:
+        ref = P.direct_sum(pos, m, pos[t], box=w.grid).astype(np.float64)
+        kind = "gpu fp32 periodic (Ewald) direct sum"
+    else:
+        x = pos.astype(np.float64)
+        mm = m.astype(np.float64)
+        ref = np.zeros((len(t), 3))
+        for k, i in enumerate(t):
+            d = x[i] - x
+            r2 = (d * d).sum(1)
+            r2[i] = np.inf
+            ref[k] = -GC * ((mm / (r2 * np.sqrt(r2)))[:, None] * d).sum(0)
+        kind = "fp64 direct sum"
     diff = acc[t].astype(np.float64) - ref
     e_rms = float(np.sqrt((diff ** 2).sum() / (ref ** 2).sum()))
     mean_rel = float(np.mean(np.linalg.norm(diff, axis=1) / (np.linalg.norm(ref, axis=1) + 1e-30)))
-    return {"e_rms": e_rms, "mean_rel": mean_rel, "targets": int(len(t)), "reference": "fp64 direct sum"}
+    return {"e_rms": e_rms, "mean_rel": mean_rel, "targets": int(len(t)), "reference": kind}
 
 
 def cpu_baseline(w, pos, vel, m, pairs_total, budget_s=15.0):

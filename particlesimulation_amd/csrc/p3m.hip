@@ -9,8 +9,9 @@
 //    clamp to the edge cells, which never loses a pair because clamping is
 //    1-Lipschitz), into a compact float4 (x, y, z, m) copy;
 //  - one work item = one cell's chunk of 64 i-particles; its j-stream is the
-//    (2 div + 1)^2 contiguous (dz, dy) rows of the (2 div + 1)^3 cell
-//    neighbourhood.  The four
+//    contiguous (dz, dy) rows of the (2 div + 1)^3 cell neighbourhood, each
+//    trimmed to the x cells that can hold a pair within the cutoff (corner
+//    rows drop out for div >= 3).  The four
 //    waves of a workgroup take interleaved 64-particle blocks of that stream
 //    and read each j through the scalar unit (uniform address), so the inner
 //    loop is pure FP32 VALU with scalar operands;
@@ -107,6 +108,8 @@ __device__ __forceinline__ float reference_force(int cs, float r, float a) {
 
 struct SrParams {
   int mx, my, mz, div;
+  float hx, hy, hz;  // cell sides
+  float rc2m;        // rc^2 (1 + 1e-4): row trimming bound with a rounding margin
   float inv_d2, rc2;
   float a, eps2;
   int cs;
@@ -144,6 +147,25 @@ __device__ __forceinline__ float pair_factor(float r2, float mj, const float2* t
   }
 }
 
+// Row (dy, dz) of the neighbourhood and its x half-width kx: cells whose
+// smallest possible separation from the i cell, (|offset| - 1) * side per
+// axis, is below the cutoff (with a rounding margin).  The bound holds for
+// particles clamped into edge cells too (clamping only shrinks offsets), so
+// trimming never drops a pair within the cutoff.
+__device__ __forceinline__ bool row_extent(const SrParams& P, int dy, int dz, int* kx) {
+  const float gy = (float)max(abs(dy) - 1, 0) * P.hy, gz = (float)max(abs(dz) - 1, 0) * P.hz;
+  const float rem = P.rc2m - gy * gy - gz * gz;
+  if (!(rem > 0.0f)) return false;
+  int k = P.div;
+  while (k > 1) {
+    const float gx = (float)(k - 1) * P.hx;
+    if (gx * gx < rem) break;
+    --k;
+  }
+  *kx = k;
+  return true;
+}
+
 template <int MODE, bool SOFT>
 __global__ __launch_bounds__(SR_THREADS) void k_sr(
     const float4* __restrict__ sr_pm, const int32_t* __restrict__ sr_start,
@@ -175,13 +197,15 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
     const float4 pi = valid ? sr_pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     float ax = 0.f, ay = 0.f, az = 0.f;
     int blk = 0;
-    const int xlo = max(cx - P.div, 0), xhi = min(cx + P.div, P.mx - 1);
     for (int dz = -P.div; dz <= P.div; ++dz) {
       const int zc = cz + dz;
       if (zc < 0 || zc >= P.mz) continue;
       for (int dy = -P.div; dy <= P.div; ++dy) {
         const int yc = cy + dy;
         if (yc < 0 || yc >= P.my) continue;
+        int kx;
+        if (!row_extent(P, dy, dz, &kx)) continue;
+        const int xlo = max(cx - kx, 0), xhi = min(cx + kx, P.mx - 1);
         const int row = P.mx * (yc + P.my * zc);
         const int js = sr_start[row + xlo], je = sr_start[row + xhi + 1];
         for (int jb = js; jb < je; jb += 64, ++blk) {
@@ -261,13 +285,15 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr_count(
     const bool valid = i < sr_start[c + 1];
     const float4 pi = valid ? sr_pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     int blk = 0;
-    const int xlo = max(cx - P.div, 0), xhi = min(cx + P.div, P.mx - 1);
     for (int dz = -P.div; dz <= P.div; ++dz) {
       const int zc = cz + dz;
       if (zc < 0 || zc >= P.mz) continue;
       for (int dy = -P.div; dy <= P.div; ++dy) {
         const int yc = cy + dy;
         if (yc < 0 || yc >= P.my) continue;
+        int kx;
+        if (!row_extent(P, dy, dz, &kx)) continue;
+        const int xlo = max(cx - kx, 0), xhi = min(cx + kx, P.mx - 1);
         const int row = P.mx * (yc + P.my * zc);
         const int js = sr_start[row + xlo], je = sr_start[row + xhi + 1];
         for (int jb = js; jb < je; jb += 64, ++blk) {
@@ -391,6 +417,10 @@ static SrParams sr_params(pmgpu_ctx* c) {
   P.my = c->mcy;
   P.mz = c->mcz;
   P.div = c->sr_div;
+  P.hx = c->hcx;
+  P.hy = c->hcy;
+  P.hz = c->hcz;
+  P.rc2m = c->p3m.cutoff * c->p3m.cutoff * (1.0f + 1e-4f);
   P.inv_d2 = c->sr_inv_d2;
   P.rc2 = c->p3m.cutoff * c->p3m.cutoff;
   P.a = c->p3m.particle_diameter;

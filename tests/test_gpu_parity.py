@@ -186,6 +186,37 @@ def test_p3m_matches_oracle(gpu_lib, oracle, cs, use_table, eps):
     assert abs(pm.p3m_count_pairs() - pairs) <= 2 + 1e-5 * pairs
 
 
+@pytest.mark.parametrize("div", [3, 4])
+def test_p3m_cell_division_keeps_the_pair_set(gpu_lib, oracle, div):
+    # finer short-range cells with trimmed rows visit fewer candidates but must
+    # accept exactly the same pairs and give the same force
+    grid = (64, 64, 64)
+    H = 0.25
+    pos, vel, m = _particles(6000, grid, 31, clustered=True)
+    pm = _make(gpu_lib, grid, 2, 0, 2, a=3.0)
+    pm.set_particles(pos, vel, m)
+    pm.init_greens()
+    pm.p3m_configure(cutoff=3.0, particle_diameter=3.0, cloud_shape=1, use_table=2)
+    pairs2, tested2 = pm.p3m_count_pairs(tested=True)
+    pm.pm_step()
+    pm.p3m_short_range()
+    a2 = pm.get_particles()[2]
+    k = list(pm.get_tuning())
+    k[3] = div
+    pm.set_tuning(*k)
+    pairs_d, tested_d = pm.p3m_count_pairs(tested=True)
+    assert pairs_d == pairs2
+    assert tested_d < tested2
+    pm.pm_step()
+    pm.p3m_short_range()
+    a_d = pm.get_particles()[2]
+    assert _rel(a_d, a2) <= 1e-6
+    G = oracle.green_grid(2, 2, 0, 3.0, grid)
+    ref_sr, _ = oracle.p3m_short_range(pos, m, np.array(grid, np.float32) * H, 3 * H, H, 3.0, 3.0, 0.0, 1,
+                                       use_table=2)
+    assert _rel(a_d, oracle.pm_accel(pos, m, grid, 2, 0, G) + ref_sr) <= FORCE_TOL
+
+
 def test_p3m_short_range_conserves_momentum_at_scale(gpu_lib):
     grid = (128, 128, 128)
     pos, vel, m = _particles(200000, grid, 40, clustered=True)
