@@ -246,10 +246,14 @@ int pmgpu_batch_get_tuning(pmgpu_batch* batch, int32_t sim, int32_t* knobs);
    (256..16384, default 2048), knobs[2] workgroups per CU of the persistent
    deposit and interpolation kernels (1..32, default 8), knobs[3] the
    short-range cell division: cells of side >= cutoff / knobs[3] and a
-   (2 knobs[3] + 1)^3 neighbourhood (2..4, default 2).  The PM results do
+   (2 knobs[3] + 1)^3 neighbourhood (2..4; unless set here, p3m_configure
+   picks the largest whose cell grid has at most 16M cells), knobs[4] the
+   interpolation variant: 1 tiles staging the potential in LDS, 2 a gather
+   from an E grid, 0 automatic (the gather below 256 particles per tile);
+   both give the same result.  The PM results do
    not depend on them (the deposit's fixed-point sums are order-independent);
    the short-range sum order, and so its rounding, follows knobs[3]. */
-#define PMGPU_TUNING_KNOBS 4
+#define PMGPU_TUNING_KNOBS 5
 int pmgpu_set_tuning(pmgpu_ctx* ctx, const int32_t* knobs);
 int pmgpu_get_tuning(pmgpu_ctx* ctx, int32_t* knobs);
 

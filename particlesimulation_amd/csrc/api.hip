@@ -287,7 +287,7 @@ void pmgpu_destroy(pmgpu_ctx* c) {
   dfree(&c->cls_count); dfree(&c->cls_start); dfree(&c->tile_nchunk); dfree(&c->tile_chunk_start);
   dfree(&c->tile_slot); dfree(&c->slot_tile); dfree(&c->int_nchunk); dfree(&c->int_chunk_start);
   dfree(&c->dep_table); dfree(&c->counters); dfree(&c->fx_accum); dfree(&c->scan_tmp);
-  dfree(&c->rho); dfree(&c->spec); dfree(&c->phi); dfree(&c->greens); dfree(&c->field);
+  dfree(&c->rho); dfree(&c->spec); dfree(&c->phi); dfree(&c->greens); dfree(&c->field); dfree(&c->efield);
   dfree(&c->tw_x); dfree(&c->tw_y); dfree(&c->tw_z); dfree(&c->tw_r2c);
   dfree(&c->sr_table); dfree(&c->sr_count); dfree(&c->sr_start); dfree(&c->sr_key);
   dfree(&c->sr_rank); dfree(&c->sr_pm); dfree(&c->sr_idx); dfree(&c->sr_nchunk);
@@ -444,6 +444,19 @@ int pmgpu_p3m_configure(pmgpu_ctx* c, const pmgpu_p3m_params* p) {
     if (!(p->box[i] > 0.0f)) fail(PMGPU_ERR_INVALID_ARGUMENT, "box must be > 0");
   HIP_CHECK(hipStreamSynchronize(c->stream));
   c->p3m = *p;
+  if (!c->sr_div_set) {
+    // the finest division whose cell grid stays modest (finer cells visit
+    // fewer candidate pairs; the pair set is the same for every division)
+    c->sr_div = SR_CELL_DIV;
+    for (int d = SR_CELL_DIV_MAX; d > SR_CELL_DIV; --d) {
+      double cells = 1.0;
+      for (int i = 0; i < 3; ++i) cells *= std::floor((double)p->box[i] / ((double)p->cutoff / d * (1.0 + 1e-5)));
+      if (cells <= 16.0 * 1024 * 1024) {
+        c->sr_div = d;
+        break;
+      }
+    }
+  }
   sr_cells_setup(c);
   dalloc(&c->sr_table, SR_TABLE_ACC);  // the larger of the two tables
   dalloc(&c->pair_count, 2);
@@ -590,16 +603,19 @@ int pmgpu_get_grid(pmgpu_ctx* c, int32_t which, float* out) {
 int pmgpu_set_tuning(pmgpu_ctx* c, const int32_t* knobs) {
   PMGPU_API_BEGIN(c)
   if (!knobs) fail(PMGPU_ERR_INVALID_ARGUMENT, "knobs is NULL");
-  const int dep = knobs[0], itp = knobs[1], bpc = knobs[2], div = knobs[3];
+  const int dep = knobs[0], itp = knobs[1], bpc = knobs[2], div = knobs[3], ik = knobs[4];
   if (dep < DEPOSIT_CHUNK_MIN || dep > DEPOSIT_CHUNK_MAX)
     fail(PMGPU_ERR_INVALID_ARGUMENT, "deposit chunk out of range");
   if (itp < INTERP_CHUNK_MIN || itp > INTERP_CHUNK_MAX)
     fail(PMGPU_ERR_INVALID_ARGUMENT, "interpolation chunk out of range");
   if (bpc < 1 || bpc > BLOCKS_PER_CU_MAX) fail(PMGPU_ERR_INVALID_ARGUMENT, "blocks per CU out of range");
   if (div < SR_CELL_DIV || div > SR_CELL_DIV_MAX) fail(PMGPU_ERR_INVALID_ARGUMENT, "short-range cell division out of range");
+  if (ik < 0 || ik > 2) fail(PMGPU_ERR_INVALID_ARGUMENT, "interpolation variant must be 0, 1 or 2");
+  c->interp_kernel = ik;
   c->dep_chunk = dep;
   c->int_chunk = itp;
   c->blocks_per_cu = bpc;
+  c->sr_div_set = true;
   if (div != c->sr_div) {
     c->sr_div = div;
     if (c->p3m_ready) {
@@ -617,6 +633,7 @@ int pmgpu_get_tuning(pmgpu_ctx* c, int32_t* knobs) {
   knobs[1] = c->int_chunk;
   knobs[2] = c->blocks_per_cu;
   knobs[3] = c->sr_div;
+  knobs[4] = c->interp_kernel;
   PMGPU_API_END(c)
 }
 

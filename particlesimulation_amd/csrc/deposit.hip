@@ -29,6 +29,7 @@ constexpr int MAX_RANGES = 99;
 constexpr int CNT_SLOTS = 0;
 
 __device__ __forceinline__ int wrapc(int a, int n) {
+  if ((unsigned)a < (unsigned)n) return a;  // the common case: no division
   a %= n;
   return a < 0 ? a + n : a;
 }

@@ -16,6 +16,7 @@ namespace pmgpu {
 namespace {
 
 __device__ __forceinline__ int wrapc(int a, int n) {
+  if ((unsigned)a < (unsigned)n) return a;  // the common case: no division
   a %= n;
   return a < 0 ? a + n : a;
 }
@@ -211,6 +212,92 @@ __global__ void k_field(const float* __restrict__ phi, int nx, int ny, int nz, i
   out[3 * i + 2] = Ez;
 }
 
+// E = -grad(phi) on every cell as float4 (x, y, z, 0), for the gather
+// variant of the interpolation.
+template <int FDS>
+__global__ void k_field4(const float* __restrict__ phi, int nx, int ny, int nz, float4* __restrict__ out,
+                         const int32_t* __restrict__ escape, int iteration) {
+  if (escape[0] <= iteration) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t M = (int64_t)nx * ny * nz;
+  if (i >= M) return;
+  const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((int64_t)nx * ny));
+  const int64_t sy = nx, sz = (int64_t)nx * ny;
+  auto X = [&](int d) { return phi[i - x + wrapc(x + d, nx)]; };
+  auto Y = [&](int d) { return phi[i + (wrapc(y + d, ny) - y) * sy]; };
+  auto Z = [&](int d) { return phi[i + (wrapc(z + d, nz) - z) * sz]; };
+  float4 e;
+  if constexpr (FDS == PMGPU_TWO_POINT) {
+    e.x = -0.5f * (X(1) - X(-1));
+    e.y = -0.5f * (Y(1) - Y(-1));
+    e.z = -0.5f * (Z(1) - Z(-1));
+  } else {
+    e.x = (-1.0f / 12) * (-X(2) + 8 * X(1) - 8 * X(-1) + X(-2));
+    e.y = (-1.0f / 12) * (-Y(2) + 8 * Y(1) - 8 * Y(-1) + Y(-2));
+    e.z = (-1.0f / 12) * (-Z(2) + 8 * Z(1) - 8 * Z(-1) + Z(-2));
+  }
+  e.w = 0.0f;
+  out[i] = e;
+}
+
+// Interpolation as a gather from the E grid, one particle per thread (the
+// particles are tile-sorted, so neighbouring threads read neighbouring
+// cells through the caches).  Same weights, operation order and leapfrog
+// tail as k_interp; no per-tile staging, so sparse tiles cost nothing extra.
+template <int SCHEME>
+__global__ __launch_bounds__(256) void k_interp_gather(
+    const float4* __restrict__ E, int nx, int ny, int nz, int64_t n, float4* __restrict__ pm,
+    float4* __restrict__ vel, float4* __restrict__ acc, int mode, float kick, float3 box, ExtField ext,
+    int32_t* __restrict__ escape, int iteration) {
+  if (escape[0] <= iteration) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  constexpr int SLO = SCHEME == PMGPU_TSC ? -1 : 0;
+  constexpr int CNT = SCHEME == PMGPU_TSC ? 3 : (SCHEME == PMGPU_CIC ? 2 : 1);
+  float4 p = pm[i];
+  const int ax = anchor_of(p.x, SCHEME), ay = anchor_of(p.y, SCHEME), az = anchor_of(p.z, SCHEME);
+  float wx[3], wy[3], wz[3];
+  interp_weights<SCHEME>(p.x - (float)ax, wx);
+  interp_weights<SCHEME>(p.y - (float)ay, wy);
+  interp_weights<SCHEME>(p.z - (float)az, wz);
+  int gx[3], gy[3], gz[3];
+#pragma unroll
+  for (int a = 0; a < CNT; ++a) {
+    gx[a] = wrapc(ax + SLO + a, nx);
+    gy[a] = wrapc(ay + SLO + a, ny);
+    gz[a] = wrapc(az + SLO + a, nz);
+  }
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+#pragma unroll
+  for (int a = 0; a < CNT; ++a) {
+    const float t1 = wx[a];
+#pragma unroll
+    for (int b = 0; b < CNT; ++b) {
+      const float t2 = t1 * wy[b];
+#pragma unroll
+      for (int c = 0; c < CNT; ++c) {
+        float t3 = t2 * wz[c];
+        if constexpr (SCHEME == PMGPU_TSC) t3 *= 0.125f;
+        const float4 e = E[(int64_t)gx[a] + (int64_t)nx * (gy[b] + (int64_t)ny * gz[c])];
+        sx += t3 * e.x;
+        sy += t3 * e.y;
+        sz += t3 * e.z;
+      }
+    }
+  }
+  const float3 g = ext_field(ext, p.x, p.y, p.z);
+  sx += g.x;
+  sy += g.y;
+  sz += g.z;
+  if (mode & MODE_ACC) acc[i] = make_float4(sx, sy, sz, 0.f);
+  if (mode & (MODE_KICK | MODE_DRIFT)) {
+    float4 v = vel[i];
+    leapfrog_tail(p, v, sx, sy, sz, mode, kick, box, escape, iteration);
+    vel[i] = v;
+    pm[i] = p;
+  }
+}
+
 // Leapfrog pieces on the sorted state (source/leapfrog.cpp:5-24).
 __global__ void k_leapfrog(float4* __restrict__ pm, float4* __restrict__ vel,
                            const float4* __restrict__ acc, float4* __restrict__ vint, int64_t n,
@@ -281,8 +368,39 @@ static float3 box_of(pmgpu_ctx* c) {
   return make_float3(c->prm.box[0], c->prm.box[1], c->prm.box[2]);
 }
 
+// The gather variant: E on every cell, then one particle per thread.
+static void interpolate_gather(pmgpu_ctx* c, int mode, float kick, int iteration) {
+  if (!c->efield) HIP_CHECK(hipMalloc((void**)&c->efield, sizeof(float4) * (size_t)c->M));
+  const unsigned gm = (unsigned)((c->M + 255) / 256), gp = (unsigned)((c->n + 255) / 256);
+  if (c->prm.finite_diff == PMGPU_TWO_POINT)
+    hipLaunchKernelGGL(k_field4<PMGPU_TWO_POINT>, dim3(gm), dim3(256), 0, c->stream, c->phi, c->nx, c->ny, c->nz,
+                       c->efield, c->escape, iteration);
+  else
+    hipLaunchKernelGGL(k_field4<PMGPU_FOUR_POINT>, dim3(gm), dim3(256), 0, c->stream, c->phi, c->nx, c->ny, c->nz,
+                       c->efield, c->escape, iteration);
+  HIP_CHECK(hipGetLastError());
+  const ExtField e = ext_of(c);
+  const float3 box = box_of(c);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(gp), dim3(256), 0, c->stream, (const float4*)c->efield, c->nx, c->ny, c->nz, c->n,
+                       c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick, box, e, c->escape, iteration);
+  };
+  if (c->prm.interpolation == PMGPU_NGP) go(k_interp_gather<PMGPU_NGP>);
+  else if (c->prm.interpolation == PMGPU_CIC) go(k_interp_gather<PMGPU_CIC>);
+  else go(k_interp_gather<PMGPU_TSC>);
+  HIP_CHECK(hipGetLastError());
+}
+
 void interpolate(pmgpu_ctx* c, int mode, float kick, int iteration) {
   int tok = stage_begin(c, "interp");
+  // auto: the gather variant when tiles are sparse (the tiled variant's
+  // per-tile staging of the potential and field then dominates)
+  const bool gather = c->interp_kernel == 2 || (c->interp_kernel == 0 && c->n < (int64_t)INTERP_GATHER_BELOW * c->ntiles);
+  if (gather) {
+    interpolate_gather(c, mode, kick, iteration);
+    stage_end(c, tok);
+    return;
+  }
   int64_t bound = (int64_t)c->ntiles + c->n / c->int_chunk + 1;
   int grid = (int)std::min<int64_t>(bound, (int64_t)c->num_cu * c->blocks_per_cu);
   const ExtField e = ext_of(c);

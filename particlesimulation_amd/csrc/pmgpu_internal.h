@@ -29,6 +29,8 @@ constexpr int NCLASS = 27;
 constexpr int DEPOSIT_CHUNK = 4096, DEPOSIT_CHUNK_MIN = 1024, DEPOSIT_CHUNK_MAX = 16384;
 constexpr int INTERP_CHUNK = 2048, INTERP_CHUNK_MIN = 256, INTERP_CHUNK_MAX = 16384;
 constexpr int BLOCKS_PER_CU = 8, BLOCKS_PER_CU_MAX = 32;
+// automatic interpolation variant: the gather variant below this many particles per tile
+constexpr int INTERP_GATHER_BELOW = 256;
 
 // P3M short range: cell side >= cutoff / div, stencil +-div cells (div tunable)
 constexpr int SR_CELL_DIV = 2, SR_CELL_DIV_MAX = 4;
@@ -98,6 +100,9 @@ struct pmgpu_ctx {
   int int_chunk = pmgpu::INTERP_CHUNK;
   int blocks_per_cu = pmgpu::BLOCKS_PER_CU;
   int sr_div = pmgpu::SR_CELL_DIV;
+  bool sr_div_set = false;              // set explicitly (else chosen at p3m_configure)
+  int interp_kernel = 0;                // 0 auto, 1 tiled (LDS halo), 2 gather from an E grid
+  float4* efield = nullptr;             // M, the gather variant's E grid
   int32_t* counters = nullptr;          // small device counters
   long long* fx_accum = nullptr;        // multi-chunk tiles x 512 fixed-point sums
   int64_t fx_slots = 0;

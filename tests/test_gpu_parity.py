@@ -196,7 +196,9 @@ def test_p3m_cell_division_keeps_the_pair_set(gpu_lib, oracle, div):
     pm = _make(gpu_lib, grid, 2, 0, 2, a=3.0)
     pm.set_particles(pos, vel, m)
     pm.init_greens()
+    pm.set_tuning(sr_div=2)  # explicit: p3m_configure would otherwise pick the finest division
     pm.p3m_configure(cutoff=3.0, particle_diameter=3.0, cloud_shape=1, use_table=2)
+    assert pm.get_tuning()[3] == 2
     pairs2, tested2 = pm.p3m_count_pairs(tested=True)
     pm.pm_step()
     pm.p3m_short_range()
