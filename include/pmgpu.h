@@ -246,7 +246,7 @@ int pmgpu_batch_get_tuning(pmgpu_batch* batch, int32_t sim, int32_t* knobs);
    (256..16384, default 2048), knobs[2] workgroups per CU of the persistent
    deposit and interpolation kernels (1..32, default 8), knobs[3] the
    short-range cell division: cells of side >= cutoff / knobs[3] and a
-   (2 knobs[3] + 1)^3 neighbourhood (2..4; unless set here, p3m_configure
+   (2 knobs[3] + 1)^3 neighbourhood (2..6; unless set here, p3m_configure
    picks the largest whose cell grid has at most 16M cells), knobs[4] the
    interpolation variant: 1 tiles staging the potential in LDS, 2 a gather
    from an E grid, 0 automatic (the gather below 256 particles per tile);
@@ -270,6 +270,9 @@ int pmgpu_profile_read(pmgpu_ctx* ctx, int32_t max, const char** names, double* 
    path: used to express the short-range kernel's rate in algorithmic
    FLOP/s. */
 int pmgpu_p3m_count_pairs(pmgpu_ctx* ctx, int64_t* pairs, int64_t* tested);
+/* stats[0] accepted pairs, stats[1] tested pairs, stats[2] lane-slots issued
+   (64 per visited j, idle lanes of partly filled i chunks included). */
+int pmgpu_p3m_pair_stats(pmgpu_ctx* ctx, int64_t* stats);
 
 /* ---- learned correction: standalone MLP evaluation ------------------- */
 /* No counterpart in the reference.  inputs is n x 4 (host); out[m] = y(m).

@@ -51,7 +51,7 @@ EXPORTS = (
     "pmgpu_profile_enable", "pmgpu_profile_reset", "pmgpu_profile_read", "pmgpu_p3m_count_pairs",
     "pmgpu_greens_value", "pmgpu_diagnostics", "pmgpu_mlp_forward", "pmgpu_mlp_backward",
     "pmgpu_trainer_create", "pmgpu_trainer_destroy", "pmgpu_trainer_last_error", "pmgpu_trainer_add_snapshot",
-    "pmgpu_trainer_loss_grad", "pmgpu_train", "pmgpu_direct_sum", "pmgpu_set_tuning", "pmgpu_get_tuning",
+    "pmgpu_trainer_loss_grad", "pmgpu_train", "pmgpu_direct_sum", "pmgpu_set_tuning", "pmgpu_get_tuning", "pmgpu_p3m_pair_stats",
     "pmgpu_batch_create", "pmgpu_batch_destroy", "pmgpu_batch_last_error", "pmgpu_batch_size",
     "pmgpu_batch_set_particles", "pmgpu_batch_prepare", "pmgpu_batch_step", "pmgpu_batch_synchronize",
     "pmgpu_batch_time", "pmgpu_batch_member_latency", "pmgpu_batch_get_particles", "pmgpu_batch_get_tuning",
@@ -134,6 +134,7 @@ def load_library(path: str = LIB_PATH):
     L.pmgpu_profile_read.argtypes = [vp, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
                                      C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
     L.pmgpu_p3m_count_pairs.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.pmgpu_p3m_pair_stats.argtypes = [vp, C.POINTER(C.c_int64)]
     L.pmgpu_greens_value.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_double] + [C.c_int32] * 6
     L.pmgpu_greens_value.restype = C.c_double
     L.pmgpu_diagnostics.argtypes = [vp, C.POINTER(C.c_double)]
@@ -281,6 +282,12 @@ class PM:
         t = C.c_int64(0)
         self._check(self._lib.pmgpu_p3m_count_pairs(self._ctx, C.byref(v), C.byref(t)))
         return (v.value, t.value) if tested else v.value
+
+    def p3m_pair_stats(self):
+        """(accepted pairs, tested pairs, lane-slots issued) of the short-range pass."""
+        s = (C.c_int64 * 3)()
+        self._check(self._lib.pmgpu_p3m_pair_stats(self._ctx, s))
+        return tuple(s)
 
     def leapfrog(self, op, dt=1.0):
         self._check(self._lib.pmgpu_leapfrog(self._ctx, int(op), float(dt)))

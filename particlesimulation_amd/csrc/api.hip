@@ -135,7 +135,11 @@ static void alloc_particle_arrays(pmgpu_ctx* c, int64_t n) {
   c->fx_slots = 8 * n / DEPOSIT_CHUNK_MIN + 2;  // enough for any tunable chunk size
   dalloc(&c->fx_accum, c->fx_slots * TILE_CELLS);
   dalloc(&c->slot_tile, c->fx_slots);
+  // work-item owner tables, sized for the smallest tunable work items
+  dalloc(&c->dep_owner, (int64_t)c->ntiles + 8 * n / DEPOSIT_CHUNK_MIN + 2);
+  dalloc(&c->int_owner, (int64_t)c->ntiles + n / INTERP_CHUNK_MIN + 2);
   if (c->p3m_ready) {
+    dalloc(&c->sr_owner, (int64_t)c->ncell_sr + n / 64 + 2);
     dalloc(&c->sr_key, n);
     dalloc(&c->sr_rank, n);
     dalloc(&c->sr_pm, n);
@@ -162,6 +166,7 @@ void sr_cells_setup(pmgpu_ctx* c) {
   dalloc(&c->sr_nchunk, c->ncell_sr + 1);
   dalloc(&c->sr_chunk_start, c->ncell_sr + 1);
   ensure_scan_scratch(c, c->ncell_sr);
+  if (c->n > 0) dalloc(&c->sr_owner, (int64_t)c->ncell_sr + c->n / 64 + 2);
 }
 
 static void set_shift(pmgpu_ctx* c) {
@@ -287,6 +292,7 @@ void pmgpu_destroy(pmgpu_ctx* c) {
   dfree(&c->cls_count); dfree(&c->cls_start); dfree(&c->tile_nchunk); dfree(&c->tile_chunk_start);
   dfree(&c->tile_slot); dfree(&c->slot_tile); dfree(&c->int_nchunk); dfree(&c->int_chunk_start);
   dfree(&c->dep_table); dfree(&c->counters); dfree(&c->fx_accum); dfree(&c->scan_tmp);
+  dfree(&c->dep_owner); dfree(&c->int_owner); dfree(&c->sr_owner);
   dfree(&c->rho); dfree(&c->spec); dfree(&c->phi); dfree(&c->greens); dfree(&c->field); dfree(&c->efield);
   dfree(&c->tw_x); dfree(&c->tw_y); dfree(&c->tw_z); dfree(&c->tw_r2c);
   dfree(&c->sr_table); dfree(&c->sr_count); dfree(&c->sr_start); dfree(&c->sr_key);
@@ -459,7 +465,7 @@ int pmgpu_p3m_configure(pmgpu_ctx* c, const pmgpu_p3m_params* p) {
   }
   sr_cells_setup(c);
   dalloc(&c->sr_table, SR_TABLE_ACC);  // the larger of the two tables
-  dalloc(&c->pair_count, 2);
+  dalloc(&c->pair_count, 3);
   c->p3m_ready = true;
   if (c->n > 0) {
     dalloc(&c->sr_key, c->n);
@@ -485,7 +491,16 @@ int pmgpu_p3m_count_pairs(pmgpu_ctx* c, int64_t* pairs, int64_t* tested) {
   require_particles(c);
   if (!c->p3m_ready) fail(PMGPU_ERR_NOT_READY, "call pmgpu_p3m_configure first");
   if (!pairs) fail(PMGPU_ERR_INVALID_ARGUMENT, "pairs is NULL");
-  p3m_count_pairs(c, pairs, tested);
+  p3m_count_pairs(c, pairs, tested, nullptr);
+  PMGPU_API_END(c)
+}
+
+int pmgpu_p3m_pair_stats(pmgpu_ctx* c, int64_t* stats) {
+  PMGPU_API_BEGIN(c)
+  require_particles(c);
+  if (!c->p3m_ready) fail(PMGPU_ERR_NOT_READY, "call pmgpu_p3m_configure first");
+  if (!stats) fail(PMGPU_ERR_INVALID_ARGUMENT, "stats is NULL");
+  p3m_count_pairs(c, &stats[0], &stats[1], &stats[2]);
   PMGPU_API_END(c)
 }
 

@@ -114,35 +114,35 @@ __global__ void k_deposit_plan(int ntiles, int ntx, int nty, int ntz, int nrange
   slot[t] = s;
 }
 
-__device__ __forceinline__ int find_item(const int32_t* __restrict__ starts, int n, int w) {
-  // largest t with starts[t] <= w (starts has n+1 entries, non-decreasing)
-  int lo = 0, hi = n;
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (starts[mid] <= w) lo = mid;
-    else hi = mid;
-  }
-  return lo;
+// Fixed-point value round(x * 2^shift) of a float contribution, through FP64
+// (exact scaling; FP64 runs at the FP32 rate here): hi/lo 32-bit halves.
+__device__ __forceinline__ unsigned long long to_fixed(float x, int shift) {
+  const double d = rint(ldexp((double)x, shift));
+  const double hi = floor(d * 0x1p-32);
+  const double lo = fma(hi, -0x1p32, d);  // in [0, 2^32), exact
+  const unsigned int h = (unsigned int)(int)hi, l = (unsigned int)lo;
+  return ((unsigned long long)h << 32) | l;
 }
 
+template <int SCHEME>
 __global__ __launch_bounds__(DEP_THREADS) void k_deposit(
     const float4* __restrict__ pm, int nx, int ny, int nz, int ntx, int nty, int ntz,
-    int ntiles, int scheme, int nranges, const int4* __restrict__ table,
+    int ntiles, int nranges, const int4* __restrict__ table,
     const int32_t* __restrict__ cls_start, const int32_t* __restrict__ chunk_start,
-    const int32_t* __restrict__ slot, long long* __restrict__ fx_accum, int shift,
-    float* __restrict__ rho, int dep_chunk, const int32_t* __restrict__ escape, int iteration) {
+    const int32_t* __restrict__ owner, const int32_t* __restrict__ slot, long long* __restrict__ fx_accum,
+    int shift, float* __restrict__ rho, int dep_chunk, const int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
   __shared__ unsigned long long acc[TILE_CELLS];
   __shared__ int32_t rstart[MAX_RANGES], rpre[MAX_RANGES + 1];
   __shared__ int s_tile, s_chunk, s_nchunk;
 
   const int total = chunk_start[ntiles];
-  const int lo = scheme == PMGPU_TSC ? -1 : 0;
-  const int cnt = scheme == PMGPU_TSC ? 3 : (scheme == PMGPU_CIC ? 2 : 1);
+  constexpr int LO = SCHEME == PMGPU_TSC ? -1 : 0;
+  constexpr int CNT = SCHEME == PMGPU_TSC ? 3 : (SCHEME == PMGPU_CIC ? 2 : 1);
 
   for (int w = blockIdx.x; w < total; w += gridDim.x) {
     if (threadIdx.x == 0) {
-      int t = find_item(chunk_start, ntiles, w);
+      const int t = owner[w];
       s_tile = t;
       s_chunk = w - chunk_start[t];
       s_nchunk = chunk_start[t + 1] - chunk_start[t];
@@ -157,51 +157,73 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit(
       rpre[threadIdx.x + 1] = l;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      rpre[0] = 0;
-      for (int r = 1; r <= nranges; ++r) rpre[r] += rpre[r - 1];
+    if (threadIdx.x < 64) {  // inclusive scan of the range lengths by the first wave
+      const int lane = threadIdx.x;
+      int carry = 0;
+      for (int base = 0; base < nranges; base += 64) {
+        const int r = base + lane;
+        int v = r < nranges ? rpre[r + 1] : 0;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const int u = __shfl_up(v, o, 64);
+          if (lane >= o) v += u;
+        }
+        if (r < nranges) rpre[r + 1] = carry + v;
+        carry += __shfl(v, 63, 64);
+      }
+      if (lane == 0) rpre[0] = 0;
     }
     __syncthreads();
     const int ox = (t % ntx) * TILE_X, oy = ((t / ntx) % nty) * TILE_Y,
               oz = (t / (ntx * nty)) * TILE_Z;
     const int vbeg = s_chunk * dep_chunk;
     const int vend = min(vbeg + dep_chunk, rpre[nranges]);
-    for (int v = vbeg + (int)threadIdx.x; v < vend; v += DEP_THREADS) {
-      // range containing virtual index v
-      int a = 0, b = nranges;
+    // range of this thread's first virtual index; later ones advance linearly
+    int a = 0;
+    {
+      const int v0 = vbeg + (int)threadIdx.x;
+      int b = nranges;
       while (b - a > 1) {
-        int mid = (a + b) >> 1;
-        if (rpre[mid] <= v) a = mid;
+        const int mid = (a + b) >> 1;
+        if (rpre[mid] <= v0) a = mid;
         else b = mid;
       }
+    }
+    for (int v = vbeg + (int)threadIdx.x; v < vend; v += DEP_THREADS) {
+      while (rpre[a + 1] <= v) ++a;
       const float4 p = pm[rstart[a] + (v - rpre[a])];
-      const int ax = anchor_of(p.x, scheme), ay = anchor_of(p.y, scheme),
-                az = anchor_of(p.z, scheme);
+      const int ax = anchor_of(p.x, SCHEME), ay = anchor_of(p.y, SCHEME), az = anchor_of(p.z, SCHEME);
       float wx[3], wy[3], wz[3];
-      axis_weights(scheme, p.x - (float)ax, wx);
-      axis_weights(scheme, p.y - (float)ay, wy);
-      axis_weights(scheme, p.z - (float)az, wz);
-      const float m = scheme == PMGPU_TSC ? p.w / 8.0f : p.w;
-      // local coordinates of the stencil's first cell relative to the tile,
-      // after periodic wrap
-      int bx = wrapc(ax + lo, nx) - ox, by = wrapc(ay + lo, ny) - oy, bz = wrapc(az + lo, nz) - oz;
-      for (int i = 0; i < cnt; ++i) {
-        int lx = bx + i;
-        if (lx >= nx - ox) lx -= nx;  // wrapped past the grid end
-        if (lx < 0 || lx >= TILE_X) continue;
+      axis_weights(SCHEME, p.x - (float)ax, wx);
+      axis_weights(SCHEME, p.y - (float)ay, wy);
+      axis_weights(SCHEME, p.z - (float)az, wz);
+      const float m = SCHEME == PMGPU_TSC ? p.w / 8.0f : p.w;
+      // local coordinates of the stencil cells relative to the tile, after
+      // periodic wrap, and whether each lies in the tile
+      int lx[3], ly[3], lz[3];
+      bool okx[3], oky[3], okz[3];
+#pragma unroll
+      for (int i = 0; i < CNT; ++i) {
+        lx[i] = wrapc(ax + LO + i, nx) - ox;
+        ly[i] = wrapc(ay + LO + i, ny) - oy;
+        lz[i] = wrapc(az + LO + i, nz) - oz;
+        okx[i] = (unsigned)lx[i] < (unsigned)TILE_X;
+        oky[i] = (unsigned)ly[i] < (unsigned)TILE_Y;
+        okz[i] = (unsigned)lz[i] < (unsigned)TILE_Z;
+      }
+#pragma unroll
+      for (int i = 0; i < CNT; ++i) {
+        if (!okx[i]) continue;
         const float t1 = m * wx[i];
-        for (int j = 0; j < cnt; ++j) {
-          int ly = by + j;
-          if (ly >= ny - oy) ly -= ny;
-          if (ly < 0 || ly >= TILE_Y) continue;
+#pragma unroll
+        for (int j = 0; j < CNT; ++j) {
+          if (!oky[j]) continue;
           const float t2 = t1 * wy[j];
-          for (int k = 0; k < cnt; ++k) {
-            int lz = bz + k;
-            if (lz >= nz - oz) lz -= nz;
-            if (lz < 0 || lz >= TILE_Z) continue;
+#pragma unroll
+          for (int k = 0; k < CNT; ++k) {
+            if (!okz[k]) continue;
             const float t3 = t2 * wz[k];
-            long long q = __float2ll_rn(ldexpf(t3, shift));
-            atomicAdd(&acc[lx + TILE_X * (ly + TILE_Y * lz)], (unsigned long long)q);
+            atomicAdd(&acc[lx[i] + TILE_X * (ly[j] + TILE_Y * lz[k])], to_fixed(t3, shift));
           }
         }
       }
@@ -287,15 +309,23 @@ void deposit(pmgpu_ctx* c) {
   HIP_CHECK(hipGetLastError());
   exclusive_scan_i32(c, c->tile_nchunk, c->tile_chunk_start, c->ntiles);
   exclusive_scan_i32(c, c->int_nchunk, c->int_chunk_start, c->ntiles);
+  expand_owner(c, c->tile_chunk_start, c->ntiles, c->dep_owner, c->iter);
+  expand_owner(c, c->int_chunk_start, c->ntiles, c->int_owner, c->iter);
   stage_end(c, tok);
 
   tok = stage_begin(c, "deposit");
   int64_t bound = (int64_t)c->ntiles + 8 * c->n / c->dep_chunk + 1;
   int grid = (int)std::min<int64_t>(bound, (int64_t)c->num_cu * c->blocks_per_cu);
-  hipLaunchKernelGGL(k_deposit, dim3(grid), dim3(DEP_THREADS), 0, c->stream, c->pm[c->cur], c->nx,
-                     c->ny, c->nz, c->ntx, c->nty, c->ntz, c->ntiles, c->prm.interpolation, nranges,
-                     c->dep_table, c->cls_start, c->tile_chunk_start, c->tile_slot, c->fx_accum,
-                     c->fx_shift, c->rho, c->dep_chunk, c->escape, c->iter);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(DEP_THREADS), 0, c->stream, c->pm[c->cur], c->nx, c->ny, c->nz,
+                       c->ntx, c->nty, c->ntz, c->ntiles, nranges, c->dep_table, c->cls_start,
+                       c->tile_chunk_start, c->dep_owner, c->tile_slot, c->fx_accum, c->fx_shift, c->rho,
+                       c->dep_chunk,
+                       c->escape, c->iter);
+  };
+  if (c->prm.interpolation == PMGPU_TSC) go(k_deposit<PMGPU_TSC>);
+  else if (c->prm.interpolation == PMGPU_CIC) go(k_deposit<PMGPU_CIC>);
+  else go(k_deposit<PMGPU_NGP>);
   HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_deposit_finalize, dim3(256), dim3(256), 0, c->stream, c->nx, c->ny, c->ntx,
                      c->nty, c->fx_accum, c->slot_tile, c->counters, c->fx_slots, c->fx_shift,

@@ -27,16 +27,6 @@ __device__ __forceinline__ int anchor_of(float x, int scheme) {
   return (int)f;
 }
 
-__device__ __forceinline__ int find_item(const int32_t* __restrict__ starts, int n, int w) {
-  int lo = 0, hi = n;
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (starts[mid] <= w) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-
 struct ExtField {
   int kind;
   float cx, cy, cz, R, M, G, H, DT;
@@ -99,7 +89,7 @@ template <int SCHEME, int FDS>
 __global__ __launch_bounds__(256) void k_interp(
     const float* __restrict__ phi, int nx, int ny, int nz, int ntx, int nty, int ntiles,
     const int32_t* __restrict__ cls_start, const int32_t* __restrict__ chunk_start,
-    float4* __restrict__ pm, float4* __restrict__ vel, float4* __restrict__ acc, int mode,
+    const int32_t* __restrict__ owner, float4* __restrict__ pm, float4* __restrict__ vel, float4* __restrict__ acc, int mode,
     float kick, float3 box, ExtField ext, int int_chunk, int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
   using Gm = Geo<SCHEME, FDS>;
@@ -109,7 +99,7 @@ __global__ __launch_bounds__(256) void k_interp(
   const int total = chunk_start[ntiles];
   for (int w = blockIdx.x; w < total; w += gridDim.x) {
     if (threadIdx.x == 0) {
-      int t = find_item(chunk_start, ntiles, w);
+      const int t = owner[w];
       s_tile = t;
       s_chunk = w - chunk_start[t];
     }
@@ -343,7 +333,7 @@ template <int S, int F>
 void launch_interp(pmgpu_ctx* c, int grid, int mode, float kick, float3 box, ExtField e,
                    int iteration) {
   hipLaunchKernelGGL((k_interp<S, F>), dim3(grid), dim3(256), 0, c->stream, c->phi, c->nx, c->ny,
-                     c->nz, c->ntx, c->nty, c->ntiles, c->cls_start, c->int_chunk_start,
+                     c->nz, c->ntx, c->nty, c->ntiles, c->cls_start, c->int_chunk_start, c->int_owner,
                      c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick, box, e, c->int_chunk, c->escape,
                      iteration);
 }

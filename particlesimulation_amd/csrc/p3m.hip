@@ -8,8 +8,9 @@
 //    >= cutoff/div (div = 2 by default, tunable; non-periodic, as the chaining mesh; out-of-box particles
 //    clamp to the edge cells, which never loses a pair because clamping is
 //    1-Lipschitz), into a compact float4 (x, y, z, m) copy;
-//  - one work item = one cell's chunk of 64 i-particles; its j-stream is the
-//    contiguous (dz, dy) rows of the (2 div + 1)^3 cell neighbourhood, each
+//  - one work item = 64 consecutive i-particles of one (y, z) row of cells
+//    (spanning x cells cx0..cx1, so sparse cells do not leave lanes idle);
+//    its j-stream is the contiguous (dz, dy) rows of the neighbourhood, each
 //    trimmed to the x cells that can hold a pair within the cutoff (corner
 //    rows drop out for div >= 3).  The four
 //    waves of a workgroup take interleaved 64-particle blocks of that stream
@@ -70,22 +71,15 @@ __global__ void k_sr_scatter(const float4* __restrict__ pm, int64_t n, const int
   sr_idx[d] = (int32_t)i;
 }
 
-__global__ void k_sr_plan(const int32_t* __restrict__ count, int ncell, int32_t* __restrict__ nchunk,
+// Work items per (y, z) row of cells: its particles (x cells in order) in
+// chunks of SR_ICHUNK, so sparse cells share a chunk instead of idling lanes.
+__global__ void k_sr_plan(const int32_t* __restrict__ start, int mx, int nrow, int32_t* __restrict__ nchunk,
                           const int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  nchunk[c] = (count[c] + SR_ICHUNK - 1) / SR_ICHUNK;
-}
-
-__device__ __forceinline__ int find_item(const int32_t* __restrict__ starts, int n, int w) {
-  int lo = 0, hi = n;
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (starts[mid] <= w) lo = mid;
-    else hi = mid;
-  }
-  return lo;
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrow) return;
+  const int count = start[(r + 1) * mx] - start[r * mx];
+  nchunk[r] = (count + SR_ICHUNK - 1) / SR_ICHUNK;
 }
 
 // P3MMethod::referenceForceS1 / S2 (p3mMethod.cpp:194-218).
@@ -108,6 +102,7 @@ __device__ __forceinline__ float reference_force(int cs, float r, float a) {
 
 struct SrParams {
   int mx, my, mz, div;
+  float ihx;         // 1 / cell side in x, as the cell keys use it
   float hx, hy, hz;  // cell sides
   float rc2m;        // rc^2 (1 + 1e-4): row trimming bound with a rounding margin
   float inv_d2, rc2;
@@ -169,7 +164,8 @@ __device__ __forceinline__ bool row_extent(const SrParams& P, int dy, int dz, in
 template <int MODE, bool SOFT>
 __global__ __launch_bounds__(SR_THREADS) void k_sr(
     const float4* __restrict__ sr_pm, const int32_t* __restrict__ sr_start,
-    const int32_t* __restrict__ chunk_start, int ncell, const int32_t* __restrict__ sr_idx,
+    const int32_t* __restrict__ chunk_start, const int32_t* __restrict__ owner, int nrow,
+    const int32_t* __restrict__ sr_idx,
     const float2* __restrict__ table_g, SrParams P, float4* __restrict__ pm,
     float4* __restrict__ vel, float4* __restrict__ acc, int mode, float kick, float3 box,
     int32_t* __restrict__ counters, int32_t* __restrict__ escape, int iteration) {
@@ -182,18 +178,24 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
     for (int i = threadIdx.x; i < TS; i += SR_THREADS) table[i] = table_g[i];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int total = chunk_start[ncell];
+  const int total = chunk_start[nrow];
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) s_w = atomicAdd(&counters[CNT_WORK], 1);
     __syncthreads();
     const int w = s_w;
     if (w >= total) break;
-    const int c = find_item(chunk_start, ncell, w);
-    const int k = w - chunk_start[c];
-    const int cx = c % P.mx, cy = (c / P.mx) % P.my, cz = c / (P.mx * P.my);
-    const int i = sr_start[c] + k * SR_ICHUNK + lane;
-    const bool valid = i < sr_start[c + 1];
+    // a work item: 64 consecutive particles of one (y, z) row of cells; they
+    // may span several x cells [cx0, cx1] (sorted by x cell within the row)
+    const int row = owner[w];
+    const int k = w - chunk_start[row];
+    const int cy = row % P.my, cz = row / P.my;
+    const int rs = sr_start[row * P.mx], re = sr_start[(row + 1) * P.mx];
+    const int i0 = rs + k * SR_ICHUNK;
+    const int i = i0 + lane;
+    const bool valid = i < re;
+    const int cx0 = sr_cell_1d(sr_pm[i0].x, P.ihx, P.mx);
+    const int cx1 = sr_cell_1d(sr_pm[min(i0 + SR_ICHUNK, re) - 1].x, P.ihx, P.mx);
     const float4 pi = valid ? sr_pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     float ax = 0.f, ay = 0.f, az = 0.f;
     int blk = 0;
@@ -205,7 +207,7 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
         if (yc < 0 || yc >= P.my) continue;
         int kx;
         if (!row_extent(P, dy, dz, &kx)) continue;
-        const int xlo = max(cx - kx, 0), xhi = min(cx + kx, P.mx - 1);
+        const int xlo = max(cx0 - kx, 0), xhi = min(cx1 + kx, P.mx - 1);
         const int row = P.mx * (yc + P.my * zc);
         const int js = sr_start[row + xlo], je = sr_start[row + xhi + 1];
         for (int jb = js; jb < je; jb += 64, ++blk) {
@@ -261,28 +263,35 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
   }
 }
 
-// Ordered pairs with 0 < r^2 < rc^2 (accepted) and all (i, j) candidates the
-// pass visits (tested), over the same traversal as k_sr.
+// Ordered pairs with 0 < r^2 < rc^2 (accepted), all (i, j) candidates the
+// pass visits (tested) and the lane-slots it issues for them (issued: 64 per
+// j, idle lanes of partly filled i chunks included), over k_sr's traversal.
 __global__ __launch_bounds__(SR_THREADS) void k_sr_count(
     const float4* __restrict__ sr_pm, const int32_t* __restrict__ sr_start,
-    const int32_t* __restrict__ chunk_start, int ncell, SrParams P,
+    const int32_t* __restrict__ chunk_start, const int32_t* __restrict__ owner, int nrow, SrParams P,
     int32_t* __restrict__ counters, unsigned long long* __restrict__ out) {
   __shared__ int s_w;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int total = chunk_start[ncell];
-  unsigned long long accepted = 0, tested = 0;
+  const int total = chunk_start[nrow];
+  unsigned long long accepted = 0, tested = 0, issued = 0;
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) s_w = atomicAdd(&counters[CNT_WORK], 1);
     __syncthreads();
     const int w = s_w;
     if (w >= total) break;
-    const int c = find_item(chunk_start, ncell, w);
-    const int k = w - chunk_start[c];
-    const int cx = c % P.mx, cy = (c / P.mx) % P.my, cz = c / (P.mx * P.my);
-    const int i = sr_start[c] + k * SR_ICHUNK + lane;
-    const bool valid = i < sr_start[c + 1];
+    // a work item: 64 consecutive particles of one (y, z) row of cells; they
+    // may span several x cells [cx0, cx1] (sorted by x cell within the row)
+    const int row = owner[w];
+    const int k = w - chunk_start[row];
+    const int cy = row % P.my, cz = row / P.my;
+    const int rs = sr_start[row * P.mx], re = sr_start[(row + 1) * P.mx];
+    const int i0 = rs + k * SR_ICHUNK;
+    const int i = i0 + lane;
+    const bool valid = i < re;
+    const int cx0 = sr_cell_1d(sr_pm[i0].x, P.ihx, P.mx);
+    const int cx1 = sr_cell_1d(sr_pm[min(i0 + SR_ICHUNK, re) - 1].x, P.ihx, P.mx);
     const float4 pi = valid ? sr_pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     int blk = 0;
     for (int dz = -P.div; dz <= P.div; ++dz) {
@@ -293,13 +302,14 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr_count(
         if (yc < 0 || yc >= P.my) continue;
         int kx;
         if (!row_extent(P, dy, dz, &kx)) continue;
-        const int xlo = max(cx - kx, 0), xhi = min(cx + kx, P.mx - 1);
+        const int xlo = max(cx0 - kx, 0), xhi = min(cx1 + kx, P.mx - 1);
         const int row = P.mx * (yc + P.my * zc);
         const int js = sr_start[row + xlo], je = sr_start[row + xhi + 1];
         for (int jb = js; jb < je; jb += 64, ++blk) {
           if ((blk & 3) != wave) continue;
           const int jend = min(jb + 64, je);
           if (valid) tested += (unsigned long long)(jend - jb);
+          issued += (unsigned long long)(jend - jb);
           for (int j = jb; j < jend; ++j) {
             const float4 pj = sr_pm[j];
             const float dx = pi.x - pj.x, dy_ = pi.y - pj.y, dz_ = pi.z - pj.z;
@@ -313,10 +323,12 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr_count(
   for (int o = 32; o > 0; o >>= 1) {
     accepted += __shfl_down(accepted, o, 64);
     tested += __shfl_down(tested, o, 64);
+    issued += __shfl_down(issued, o, 64);
   }
   if (lane == 0) {
     atomicAdd(&out[0], accepted);
     atomicAdd(&out[1], tested);
+    atomicAdd(&out[2], issued);
   }
 }
 
@@ -417,6 +429,7 @@ static SrParams sr_params(pmgpu_ctx* c) {
   P.my = c->mcy;
   P.mz = c->mcz;
   P.div = c->sr_div;
+  P.ihx = 1.0f / c->hcx;
   P.hx = c->hcx;
   P.hy = c->hcy;
   P.hz = c->hcz;
@@ -442,10 +455,12 @@ static void sr_sort(pmgpu_ctx* c, int iteration) {
   hipLaunchKernelGGL(k_sr_scatter, dim3(g), dim3(256), 0, c->stream, c->pm[c->cur], c->n, c->sr_key,
                      c->sr_rank, c->sr_start, c->sr_pm, c->sr_idx, c->escape, iteration);
   HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_sr_plan, dim3((c->ncell_sr + 255) / 256), dim3(256), 0, c->stream,
-                     c->sr_count, c->ncell_sr, c->sr_nchunk, c->escape, iteration);
+  const int nrow = c->mcy * c->mcz;
+  hipLaunchKernelGGL(k_sr_plan, dim3((nrow + 255) / 256), dim3(256), 0, c->stream, c->sr_start, c->mcx, nrow,
+                     c->sr_nchunk, c->escape, iteration);
   HIP_CHECK(hipGetLastError());
-  exclusive_scan_i32(c, c->sr_nchunk, c->sr_chunk_start, c->ncell_sr);
+  exclusive_scan_i32(c, c->sr_nchunk, c->sr_chunk_start, nrow);
+  expand_owner(c, c->sr_chunk_start, nrow, c->sr_owner, iteration);
   zero_fill_i32(c, c->counters + CNT_WORK, 1);
   stage_end(c, tok);
 }
@@ -454,7 +469,7 @@ template <int MODE, bool SOFT>
 static void launch_sr(pmgpu_ctx* c, int mode, float kick, int iteration) {
   const int grid = c->num_cu * 8;
   hipLaunchKernelGGL((k_sr<MODE, SOFT>), dim3(grid), dim3(SR_THREADS), 0, c->stream, c->sr_pm,
-                     c->sr_start, c->sr_chunk_start, c->ncell_sr, c->sr_idx, c->sr_table,
+                     c->sr_start, c->sr_chunk_start, c->sr_owner, c->mcy * c->mcz, c->sr_idx, c->sr_table,
                      sr_params(c), c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick,
                      make_float3(c->prm.box[0], c->prm.box[1], c->prm.box[2]), c->counters,
                      c->escape, iteration);
@@ -475,19 +490,20 @@ void p3m_short_range(pmgpu_ctx* c, int mode, float kick, int iteration) {
   stage_end(c, tok);
 }
 
-void p3m_count_pairs(pmgpu_ctx* c, int64_t* accepted, int64_t* tested) {
+void p3m_count_pairs(pmgpu_ctx* c, int64_t* accepted, int64_t* tested, int64_t* issued) {
   const int it = -2147483647 - 1;
   sr_sort(c, it);
-  HIP_CHECK(hipMemsetAsync(c->pair_count, 0, 2 * sizeof(unsigned long long), c->stream));
+  zero_fill_i32(c, (int32_t*)c->pair_count, 6);
   hipLaunchKernelGGL(k_sr_count, dim3(c->num_cu * 8), dim3(SR_THREADS), 0, c->stream, c->sr_pm,
-                     c->sr_start, c->sr_chunk_start, c->ncell_sr, sr_params(c), c->counters,
+                     c->sr_start, c->sr_chunk_start, c->sr_owner, c->mcy * c->mcz, sr_params(c), c->counters,
                      c->pair_count);
   HIP_CHECK(hipGetLastError());
-  unsigned long long h[2] = {0, 0};
+  unsigned long long h[3] = {0, 0, 0};
   HIP_CHECK(hipMemcpyAsync(h, c->pair_count, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
   *accepted = (int64_t)h[0];
   if (tested) *tested = (int64_t)h[1];
+  if (issued) *issued = (int64_t)h[2];
 }
 
 }  // namespace pmgpu

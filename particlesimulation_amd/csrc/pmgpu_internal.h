@@ -33,7 +33,7 @@ constexpr int BLOCKS_PER_CU = 8, BLOCKS_PER_CU_MAX = 32;
 constexpr int INTERP_GATHER_BELOW = 256;
 
 // P3M short range: cell side >= cutoff / div, stencil +-div cells (div tunable)
-constexpr int SR_CELL_DIV = 2, SR_CELL_DIV_MAX = 4;
+constexpr int SR_CELL_DIV = 2, SR_CELL_DIV_MAX = 6;
 constexpr int SR_TABLE = 512;  // padded reference table (the reference uses 500 entries)
 constexpr int SR_TABLE_REF = 500;
 constexpr int SR_TABLE_ACC = 2048;  // accurate (r (s+eps^2)-scaled) table
@@ -93,6 +93,9 @@ struct pmgpu_ctx {
   int32_t* slot_tile = nullptr;         // tile of each used slot
   int32_t* int_nchunk = nullptr;        // interpolation work items per tile (+1)
   int32_t* int_chunk_start = nullptr;   // ntiles + 1
+  int32_t* dep_owner = nullptr;         // tile of each deposit work item
+  int32_t* int_owner = nullptr;         // tile of each interpolation work item
+  int32_t* sr_owner = nullptr;          // short-range cell of each work item
   int4* dep_table = nullptr;            // neighbour (dx,dy,dz,class) ranges of the scheme
   int dep_table_len = 0;
   int num_cu = 256;
@@ -167,6 +170,8 @@ void stage_end(pmgpu_ctx* c, int token);
 // launchers (one per translation unit)
 void zero_fill_i32(pmgpu_ctx* c, int32_t* p, int64_t n);
 void exclusive_scan_i32(pmgpu_ctx* c, const int32_t* in, int32_t* out, int64_t n);
+// owner[w] = t for the work items w in [starts[t], starts[t+1]), t < n (O(1) item lookup)
+void expand_owner(pmgpu_ctx* c, const int32_t* starts, int64_t n, int32_t* owner, int iteration);
 
 void tile_sort(pmgpu_ctx* c);
 size_t scan_scratch_elems(int64_t n);
@@ -184,7 +189,7 @@ void leapfrog_op(pmgpu_ctx* c, int op, float dt, int iteration);
 void p3m_setup(pmgpu_ctx* c);
 void sr_cells_setup(pmgpu_ctx* c);  // short-range cell grid for c->p3m and c->sr_div
 void p3m_short_range(pmgpu_ctx* c, int mode, float kick, int iteration);
-void p3m_count_pairs(pmgpu_ctx* c, int64_t* accepted, int64_t* tested);
+void p3m_count_pairs(pmgpu_ctx* c, int64_t* accepted, int64_t* tested, int64_t* issued);
 // learned correction (mlp.hip)
 struct MlpDevice;
 MlpDevice* mlp_device_create();

@@ -201,6 +201,23 @@ void exclusive_scan_i32(pmgpu_ctx* c, const int32_t* in, int32_t* out, int64_t n
 
 size_t scan_scratch_elems(int64_t n) { return scan_scratch(n); }
 
+namespace {
+// owner[w] = t for every work item w in [starts[t], starts[t+1])
+__global__ void k_expand_owner(const int32_t* __restrict__ starts, int64_t n, int32_t* __restrict__ owner,
+                               const int32_t* __restrict__ escape, int iteration) {
+  if (escape[0] <= iteration) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  for (int32_t w = starts[t]; w < starts[t + 1]; ++w) owner[w] = (int32_t)t;
+}
+}  // namespace
+
+void expand_owner(pmgpu_ctx* c, const int32_t* starts, int64_t n, int32_t* owner, int iteration) {
+  hipLaunchKernelGGL(k_expand_owner, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, starts, n, owner,
+                     c->escape, iteration);
+  HIP_CHECK(hipGetLastError());
+}
+
 void tile_sort(pmgpu_ctx* c) {
   const int64_t nkeys = (int64_t)c->ntiles * NCLASS;
   int tok = stage_begin(c, "tile_sort");
