@@ -250,10 +250,12 @@ int pmgpu_batch_get_tuning(pmgpu_batch* batch, int32_t sim, int32_t* knobs);
    picks the largest whose cell grid has at most 16M cells), knobs[4] the
    interpolation variant: 1 tiles staging the potential in LDS, 2 a gather
    from an E grid, 0 automatic (the gather below 256 particles per tile);
-   both give the same result.  The PM results do
-   not depend on them (the deposit's fixed-point sums are order-independent);
-   the short-range sum order, and so its rounding, follows knobs[3]. */
-#define PMGPU_TUNING_KNOBS 5
+   both give the same result; knobs[5] the short-range work items: 1 chunks
+   of one cell's particles, 2 chunks of a row of cells' particles (fewer idle
+   lanes in sparse cells), 0 automatic (rows).  The PM results do not depend
+   on the knobs (the deposit's fixed-point sums are order-independent); the
+   short-range sum order, and so its rounding, follows knobs[3] and [5]. */
+#define PMGPU_TUNING_KNOBS 6
 int pmgpu_set_tuning(pmgpu_ctx* ctx, const int32_t* knobs);
 int pmgpu_get_tuning(pmgpu_ctx* ctx, int32_t* knobs);
 

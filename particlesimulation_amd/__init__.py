@@ -344,13 +344,14 @@ class PM:
         self._check(self._lib.pmgpu_set_greens_correction(self._ctx, _fptr(w)))
 
     # -- launch tuning ----------------------------------------------------
-    def set_tuning(self, deposit_chunk=4096, interp_chunk=2048, blocks_per_cu=8, sr_div=2, interp_kernel=0):
-        k = (C.c_int32 * 5)(int(deposit_chunk), int(interp_chunk), int(blocks_per_cu), int(sr_div),
-                            int(interp_kernel))
+    def set_tuning(self, deposit_chunk=4096, interp_chunk=2048, blocks_per_cu=8, sr_div=2, interp_kernel=0,
+                   sr_chunking=0):
+        k = (C.c_int32 * 6)(int(deposit_chunk), int(interp_chunk), int(blocks_per_cu), int(sr_div),
+                            int(interp_kernel), int(sr_chunking))
         self._check(self._lib.pmgpu_set_tuning(self._ctx, k))
 
     def get_tuning(self):
-        k = (C.c_int32 * 5)()
+        k = (C.c_int32 * 6)()
         self._check(self._lib.pmgpu_get_tuning(self._ctx, k))
         return tuple(k)
 
@@ -600,7 +601,7 @@ class Batch:
         return pos, vel, acc
 
     def tuning(self, i):
-        k = (C.c_int32 * 5)()
+        k = (C.c_int32 * 6)()
         self._check(self._lib.pmgpu_batch_get_tuning(self._b, int(i), k))
         return tuple(k)
 

@@ -618,7 +618,7 @@ int pmgpu_get_grid(pmgpu_ctx* c, int32_t which, float* out) {
 int pmgpu_set_tuning(pmgpu_ctx* c, const int32_t* knobs) {
   PMGPU_API_BEGIN(c)
   if (!knobs) fail(PMGPU_ERR_INVALID_ARGUMENT, "knobs is NULL");
-  const int dep = knobs[0], itp = knobs[1], bpc = knobs[2], div = knobs[3], ik = knobs[4];
+  const int dep = knobs[0], itp = knobs[1], bpc = knobs[2], div = knobs[3], ik = knobs[4], sc = knobs[5];
   if (dep < DEPOSIT_CHUNK_MIN || dep > DEPOSIT_CHUNK_MAX)
     fail(PMGPU_ERR_INVALID_ARGUMENT, "deposit chunk out of range");
   if (itp < INTERP_CHUNK_MIN || itp > INTERP_CHUNK_MAX)
@@ -627,6 +627,8 @@ int pmgpu_set_tuning(pmgpu_ctx* c, const int32_t* knobs) {
   if (div < SR_CELL_DIV || div > SR_CELL_DIV_MAX) fail(PMGPU_ERR_INVALID_ARGUMENT, "short-range cell division out of range");
   if (ik < 0 || ik > 2) fail(PMGPU_ERR_INVALID_ARGUMENT, "interpolation variant must be 0, 1 or 2");
   c->interp_kernel = ik;
+  if (sc < 0 || sc > 2) fail(PMGPU_ERR_INVALID_ARGUMENT, "short-range chunking must be 0, 1 or 2");
+  c->sr_chunking = sc;
   c->dep_chunk = dep;
   c->int_chunk = itp;
   c->blocks_per_cu = bpc;
@@ -649,6 +651,7 @@ int pmgpu_get_tuning(pmgpu_ctx* c, int32_t* knobs) {
   knobs[2] = c->blocks_per_cu;
   knobs[3] = c->sr_div;
   knobs[4] = c->interp_kernel;
+  knobs[5] = c->sr_chunking;
   PMGPU_API_END(c)
 }
 

@@ -77,11 +77,12 @@ float time_member(pmgpu_ctx* c, bool p3m, int reps) {
   return ms / reps;
 }
 
-void apply_knobs(pmgpu_ctx* c, const int32_t k[5]) {
+void apply_knobs(pmgpu_ctx* c, const int32_t k[6]) {
   c->dep_chunk = k[0];
   c->int_chunk = k[1];
   c->blocks_per_cu = k[2];
   c->interp_kernel = k[4];
+  c->sr_chunking = k[5];
   if (k[3] != c->sr_div) {
     HIP_CHECK(hipStreamSynchronize(c->stream));
     c->sr_div = k[3];
@@ -92,26 +93,28 @@ void apply_knobs(pmgpu_ctx* c, const int32_t k[5]) {
 // Coordinate descent over the knobs (the short-range division only for P3M
 // members), starting from the defaults; the interpolation variant 0 (auto)
 // is replaced by an explicit 1 or 2.
-void autotune_member(pmgpu_ctx* c, bool p3m, int32_t best[5]) {
-  const std::vector<int> cand[5] = {{1024, 2048, 4096, 8192}, {512, 1024, 2048, 4096}, {2, 4, 8, 16}, {2, 3, 4}, {1, 2}};
+void autotune_member(pmgpu_ctx* c, bool p3m, int32_t best[6]) {
+  const std::vector<int> cand[6] = {{1024, 2048, 4096, 8192}, {512, 1024, 2048, 4096}, {2, 4, 8, 16},
+                                    {2, 3, 4, 5}, {1, 2}, {1, 2}};
   best[0] = c->dep_chunk;
   best[1] = c->int_chunk;
   best[2] = c->blocks_per_cu;
   best[3] = c->sr_div;
   best[4] = c->interp_kernel;
+  best[5] = c->sr_chunking;
   float best_ms = time_member(c, p3m, 3);
   std::vector<int> order = {4, 2, 0, 1};
-  if (p3m) order.insert(order.begin(), 3);
+  if (p3m) order.insert(order.begin(), {3, 5});
   for (int k : order) {
     for (int v : cand[k]) {
       if (v == best[k]) continue;
-      int32_t trial[5] = {best[0], best[1], best[2], best[3], best[4]};
+      int32_t trial[6] = {best[0], best[1], best[2], best[3], best[4], best[5]};
       trial[k] = v;
       apply_knobs(c, trial);
       const float ms = time_member(c, p3m, 3);
       if (ms < 0.97f * best_ms) {
         best_ms = ms;
-        std::copy(trial, trial + 5, best);
+        std::copy(trial, trial + 6, best);
       }
     }
   }
@@ -228,13 +231,13 @@ int pmgpu_batch_prepare(pmgpu_batch* b, int32_t autotune) {
       auto it = rep.find(key);
       if (it == rep.end() || b->sims[it->second]->n < c->n) rep[key] = i;
     }
-    std::map<std::tuple<int, int, int, int, int, int>, std::array<int32_t, 5>> best;
+    std::map<std::tuple<int, int, int, int, int, int>, std::array<int32_t, 6>> best;
     for (auto& kv : rep) {
       pmgpu_ctx* c = b->sims[kv.second];
       c->iter = INT_MIN;
-      int32_t k[5];
+      int32_t k[6];
       autotune_member(c, b->use_p3m[kv.second] != 0, k);
-      best[kv.first] = {k[0], k[1], k[2], k[3], k[4]};
+      best[kv.first] = {k[0], k[1], k[2], k[3], k[4], k[5]};
     }
     for (int i = 0; i < n; ++i) {
       pmgpu_ctx* c = b->sims[i];

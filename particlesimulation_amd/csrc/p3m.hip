@@ -8,8 +8,9 @@
 //    >= cutoff/div (div = 2 by default, tunable; non-periodic, as the chaining mesh; out-of-box particles
 //    clamp to the edge cells, which never loses a pair because clamping is
 //    1-Lipschitz), into a compact float4 (x, y, z, m) copy;
-//  - one work item = 64 consecutive i-particles of one (y, z) row of cells
-//    (spanning x cells cx0..cx1, so sparse cells do not leave lanes idle);
+//  - one work item = 64 consecutive i-particles of one cell, or of one
+//    (y, z) row of cells (spanning x cells cx0..cx1, so sparse cells do not
+//    leave lanes idle; tunable);
 //    its j-stream is the contiguous (dz, dy) rows of the neighbourhood, each
 //    trimmed to the x cells that can hold a pair within the cutoff (corner
 //    rows drop out for div >= 3).  The four
@@ -71,14 +72,15 @@ __global__ void k_sr_scatter(const float4* __restrict__ pm, int64_t n, const int
   sr_idx[d] = (int32_t)i;
 }
 
-// Work items per (y, z) row of cells: its particles (x cells in order) in
-// chunks of SR_ICHUNK, so sparse cells share a chunk instead of idling lanes.
-__global__ void k_sr_plan(const int32_t* __restrict__ start, int mx, int nrow, int32_t* __restrict__ nchunk,
+// Work items per unit in chunks of SR_ICHUNK particles; a unit is a cell
+// (stride 1) or a (y, z) row of cells (stride mx: its particles, x cells in
+// order, so sparse cells share a chunk instead of idling lanes).
+__global__ void k_sr_plan(const int32_t* __restrict__ start, int stride, int nunit, int32_t* __restrict__ nchunk,
                           const int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
   int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrow) return;
-  const int count = start[(r + 1) * mx] - start[r * mx];
+  if (r >= nunit) return;
+  const int count = start[(r + 1) * stride] - start[r * stride];
   nchunk[r] = (count + SR_ICHUNK - 1) / SR_ICHUNK;
 }
 
@@ -101,7 +103,7 @@ __device__ __forceinline__ float reference_force(int cs, float r, float a) {
 }
 
 struct SrParams {
-  int mx, my, mz, div;
+  int mx, my, mz, div, rowspan;
   float ihx;         // 1 / cell side in x, as the cell keys use it
   float hx, hy, hz;  // cell sides
   float rc2m;        // rc^2 (1 + 1e-4): row trimming bound with a rounding margin
@@ -164,7 +166,7 @@ __device__ __forceinline__ bool row_extent(const SrParams& P, int dy, int dz, in
 template <int MODE, bool SOFT>
 __global__ __launch_bounds__(SR_THREADS) void k_sr(
     const float4* __restrict__ sr_pm, const int32_t* __restrict__ sr_start,
-    const int32_t* __restrict__ chunk_start, const int32_t* __restrict__ owner, int nrow,
+    const int32_t* __restrict__ chunk_start, const int32_t* __restrict__ owner, int nunit,
     const int32_t* __restrict__ sr_idx,
     const float2* __restrict__ table_g, SrParams P, float4* __restrict__ pm,
     float4* __restrict__ vel, float4* __restrict__ acc, int mode, float kick, float3 box,
@@ -178,24 +180,38 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
     for (int i = threadIdx.x; i < TS; i += SR_THREADS) table[i] = table_g[i];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int total = chunk_start[nrow];
+  const int total = chunk_start[nunit];
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) s_w = atomicAdd(&counters[CNT_WORK], 1);
     __syncthreads();
     const int w = s_w;
     if (w >= total) break;
-    // a work item: 64 consecutive particles of one (y, z) row of cells; they
-    // may span several x cells [cx0, cx1] (sorted by x cell within the row)
-    const int row = owner[w];
-    const int k = w - chunk_start[row];
-    const int cy = row % P.my, cz = row / P.my;
-    const int rs = sr_start[row * P.mx], re = sr_start[(row + 1) * P.mx];
+    // a work item: 64 consecutive particles of one unit -- a cell, or a
+    // (y, z) row of cells whose particles may span x cells [cx0, cx1]
+    // (sorted by x cell within the row)
+    const int u = owner[w];
+    const int k = w - chunk_start[u];
+    int cy, cz, rs, re, cx0, cx1;
+    if (P.rowspan) {
+      cy = u % P.my;
+      cz = u / P.my;
+      rs = sr_start[u * P.mx];
+      re = sr_start[(u + 1) * P.mx];
+    } else {
+      cx0 = cx1 = u % P.mx;
+      cy = (u / P.mx) % P.my;
+      cz = u / (P.mx * P.my);
+      rs = sr_start[u];
+      re = sr_start[u + 1];
+    }
     const int i0 = rs + k * SR_ICHUNK;
     const int i = i0 + lane;
     const bool valid = i < re;
-    const int cx0 = sr_cell_1d(sr_pm[i0].x, P.ihx, P.mx);
-    const int cx1 = sr_cell_1d(sr_pm[min(i0 + SR_ICHUNK, re) - 1].x, P.ihx, P.mx);
+    if (P.rowspan) {
+      cx0 = sr_cell_1d(sr_pm[i0].x, P.ihx, P.mx);
+      cx1 = sr_cell_1d(sr_pm[min(i0 + SR_ICHUNK, re) - 1].x, P.ihx, P.mx);
+    }
     const float4 pi = valid ? sr_pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     float ax = 0.f, ay = 0.f, az = 0.f;
     int blk = 0;
@@ -268,12 +284,12 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
 // j, idle lanes of partly filled i chunks included), over k_sr's traversal.
 __global__ __launch_bounds__(SR_THREADS) void k_sr_count(
     const float4* __restrict__ sr_pm, const int32_t* __restrict__ sr_start,
-    const int32_t* __restrict__ chunk_start, const int32_t* __restrict__ owner, int nrow, SrParams P,
+    const int32_t* __restrict__ chunk_start, const int32_t* __restrict__ owner, int nunit, SrParams P,
     int32_t* __restrict__ counters, unsigned long long* __restrict__ out) {
   __shared__ int s_w;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int total = chunk_start[nrow];
+  const int total = chunk_start[nunit];
   unsigned long long accepted = 0, tested = 0, issued = 0;
   for (;;) {
     __syncthreads();
@@ -281,17 +297,31 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr_count(
     __syncthreads();
     const int w = s_w;
     if (w >= total) break;
-    // a work item: 64 consecutive particles of one (y, z) row of cells; they
-    // may span several x cells [cx0, cx1] (sorted by x cell within the row)
-    const int row = owner[w];
-    const int k = w - chunk_start[row];
-    const int cy = row % P.my, cz = row / P.my;
-    const int rs = sr_start[row * P.mx], re = sr_start[(row + 1) * P.mx];
+    // a work item: 64 consecutive particles of one unit -- a cell, or a
+    // (y, z) row of cells whose particles may span x cells [cx0, cx1]
+    // (sorted by x cell within the row)
+    const int u = owner[w];
+    const int k = w - chunk_start[u];
+    int cy, cz, rs, re, cx0, cx1;
+    if (P.rowspan) {
+      cy = u % P.my;
+      cz = u / P.my;
+      rs = sr_start[u * P.mx];
+      re = sr_start[(u + 1) * P.mx];
+    } else {
+      cx0 = cx1 = u % P.mx;
+      cy = (u / P.mx) % P.my;
+      cz = u / (P.mx * P.my);
+      rs = sr_start[u];
+      re = sr_start[u + 1];
+    }
     const int i0 = rs + k * SR_ICHUNK;
     const int i = i0 + lane;
     const bool valid = i < re;
-    const int cx0 = sr_cell_1d(sr_pm[i0].x, P.ihx, P.mx);
-    const int cx1 = sr_cell_1d(sr_pm[min(i0 + SR_ICHUNK, re) - 1].x, P.ihx, P.mx);
+    if (P.rowspan) {
+      cx0 = sr_cell_1d(sr_pm[i0].x, P.ihx, P.mx);
+      cx1 = sr_cell_1d(sr_pm[min(i0 + SR_ICHUNK, re) - 1].x, P.ihx, P.mx);
+    }
     const float4 pi = valid ? sr_pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     int blk = 0;
     for (int dz = -P.div; dz <= P.div; ++dz) {
@@ -423,12 +453,17 @@ void p3m_setup(pmgpu_ctx* c) {
   c->sr_inv_d2 = (float)(1.0 / (double)d2);
 }
 
+// Row-span work items (knobs[5] = 2, or automatic) or one cell per unit (1).
+static bool sr_rowspan(const pmgpu_ctx* c) { return c->sr_chunking != 1; }
+static int sr_units(const pmgpu_ctx* c) { return sr_rowspan(c) ? c->mcy * c->mcz : c->ncell_sr; }
+
 static SrParams sr_params(pmgpu_ctx* c) {
   SrParams P;
   P.mx = c->mcx;
   P.my = c->mcy;
   P.mz = c->mcz;
   P.div = c->sr_div;
+  P.rowspan = sr_rowspan(c);
   P.ihx = 1.0f / c->hcx;
   P.hx = c->hcx;
   P.hy = c->hcy;
@@ -455,12 +490,12 @@ static void sr_sort(pmgpu_ctx* c, int iteration) {
   hipLaunchKernelGGL(k_sr_scatter, dim3(g), dim3(256), 0, c->stream, c->pm[c->cur], c->n, c->sr_key,
                      c->sr_rank, c->sr_start, c->sr_pm, c->sr_idx, c->escape, iteration);
   HIP_CHECK(hipGetLastError());
-  const int nrow = c->mcy * c->mcz;
-  hipLaunchKernelGGL(k_sr_plan, dim3((nrow + 255) / 256), dim3(256), 0, c->stream, c->sr_start, c->mcx, nrow,
+  const int nunit = sr_units(c), stride = sr_rowspan(c) ? c->mcx : 1;
+  hipLaunchKernelGGL(k_sr_plan, dim3((nunit + 255) / 256), dim3(256), 0, c->stream, c->sr_start, stride, nunit,
                      c->sr_nchunk, c->escape, iteration);
   HIP_CHECK(hipGetLastError());
-  exclusive_scan_i32(c, c->sr_nchunk, c->sr_chunk_start, nrow);
-  expand_owner(c, c->sr_chunk_start, nrow, c->sr_owner, iteration);
+  exclusive_scan_i32(c, c->sr_nchunk, c->sr_chunk_start, nunit);
+  expand_owner(c, c->sr_chunk_start, nunit, c->sr_owner, iteration);
   zero_fill_i32(c, c->counters + CNT_WORK, 1);
   stage_end(c, tok);
 }
@@ -469,7 +504,7 @@ template <int MODE, bool SOFT>
 static void launch_sr(pmgpu_ctx* c, int mode, float kick, int iteration) {
   const int grid = c->num_cu * 8;
   hipLaunchKernelGGL((k_sr<MODE, SOFT>), dim3(grid), dim3(SR_THREADS), 0, c->stream, c->sr_pm,
-                     c->sr_start, c->sr_chunk_start, c->sr_owner, c->mcy * c->mcz, c->sr_idx, c->sr_table,
+                     c->sr_start, c->sr_chunk_start, c->sr_owner, sr_units(c), c->sr_idx, c->sr_table,
                      sr_params(c), c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick,
                      make_float3(c->prm.box[0], c->prm.box[1], c->prm.box[2]), c->counters,
                      c->escape, iteration);
@@ -495,7 +530,7 @@ void p3m_count_pairs(pmgpu_ctx* c, int64_t* accepted, int64_t* tested, int64_t* 
   sr_sort(c, it);
   zero_fill_i32(c, (int32_t*)c->pair_count, 6);
   hipLaunchKernelGGL(k_sr_count, dim3(c->num_cu * 8), dim3(SR_THREADS), 0, c->stream, c->sr_pm,
-                     c->sr_start, c->sr_chunk_start, c->sr_owner, c->mcy * c->mcz, sr_params(c), c->counters,
+                     c->sr_start, c->sr_chunk_start, c->sr_owner, sr_units(c), sr_params(c), c->counters,
                      c->pair_count);
   HIP_CHECK(hipGetLastError());
   unsigned long long h[3] = {0, 0, 0};

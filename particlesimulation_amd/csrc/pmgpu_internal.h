@@ -105,6 +105,7 @@ struct pmgpu_ctx {
   int sr_div = pmgpu::SR_CELL_DIV;
   bool sr_div_set = false;              // set explicitly (else chosen at p3m_configure)
   int interp_kernel = 0;                // 0 auto, 1 tiled (LDS halo), 2 gather from an E grid
+  int sr_chunking = 0;                  // 0 auto (rows), 1 cells, 2 rows of cells
   float4* efield = nullptr;             // M, the gather variant's E grid
   int32_t* counters = nullptr;          // small device counters
   long long* fx_accum = nullptr;        // multi-chunk tiles x 512 fixed-point sums

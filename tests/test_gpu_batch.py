@@ -61,7 +61,7 @@ def test_batch_members_match_single_runs(gpu_lib, lanes, autotune):
             assert np.array_equal(x, rx), i
             assert np.array_equal(v, rv), i
         knobs = b.tuning(i)
-        assert len(knobs) == 5 and all(k > 0 for k in knobs[:4]) and knobs[4] in (0, 1, 2)
+        assert len(knobs) == 6 and all(k > 0 for k in knobs[:4]) and knobs[4] in (0, 1, 2) and knobs[5] in (0, 1, 2)
     ms = b.time(2)
     lat = b.member_latency(2)
     assert ms > 0 and np.all(lat > 0)
@@ -76,8 +76,8 @@ def test_tuning_does_not_change_results(gpu_lib):
     pos[:5000] = rng.uniform(0, g, (5000, 3))  # sparse tiles too
     m = np.full(50000, 1.0 / 50000, np.float32)
     ref = None
-    for knobs in ((4096, 2048, 8, 2, 1), (1024, 512, 2, 2, 1), (16384, 16384, 32, 3, 1), (2048, 256, 1, 4, 1),
-                  (4096, 2048, 8, 2, 2), (1024, 512, 4, 2, 0)):
+    for knobs in ((4096, 2048, 8, 2, 1, 0), (1024, 512, 2, 2, 1, 1), (16384, 16384, 32, 3, 1, 2),
+                  (2048, 256, 1, 4, 1, 0), (4096, 2048, 8, 2, 2, 0), (1024, 512, 4, 2, 0, 0)):
         pm = P.PM((g, g, g), interpolation=P.TSC, finite_diff=P.FOUR_POINT, greens_function=P.DISCRETE_LAPLACIAN,
                   box=(0, 0, 0))
         pm.set_tuning(*knobs)
@@ -97,7 +97,7 @@ def test_tuning_does_not_change_results(gpu_lib):
             # the gather variant forms E in another kernel: equal up to FMA contraction
             assert np.max(np.abs(out[1] - ref[1])) <= 1e-6 * np.abs(ref[1]).max(), knobs
     with pytest.raises(P.InvalidArgument):
-        pm.set_tuning(512, 2048, 8, 2, 0)
+        pm.set_tuning(512, 2048, 8, 2, 0, 0)
 
 
 @pytest.mark.parametrize("scheme,fds", [(0, 0), (1, 1), (2, 0), (2, 1)])

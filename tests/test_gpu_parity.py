@@ -186,8 +186,8 @@ def test_p3m_matches_oracle(gpu_lib, oracle, cs, use_table, eps):
     assert abs(pm.p3m_count_pairs() - pairs) <= 2 + 1e-5 * pairs
 
 
-@pytest.mark.parametrize("div", [3, 4])
-def test_p3m_cell_division_keeps_the_pair_set(gpu_lib, oracle, div):
+@pytest.mark.parametrize("div,chunking", [(3, 2), (4, 1), (5, 2), (6, 1)])
+def test_p3m_cell_division_keeps_the_pair_set(gpu_lib, oracle, div, chunking):
     # finer short-range cells with trimmed rows visit fewer candidates but must
     # accept exactly the same pairs and give the same force
     grid = (64, 64, 64)
@@ -205,6 +205,7 @@ def test_p3m_cell_division_keeps_the_pair_set(gpu_lib, oracle, div):
     a2 = pm.get_particles()[2]
     k = list(pm.get_tuning())
     k[3] = div
+    k[5] = chunking
     pm.set_tuning(*k)
     pairs_d, tested_d = pm.p3m_count_pairs(tested=True)
     assert pairs_d == pairs2
