@@ -259,7 +259,7 @@ int pmgpu_create(const pmgpu_pm_params* params, pmgpu_ctx** out) {
     dalloc(&c->tile_slot, c->ntiles + 1);
     dalloc(&c->int_nchunk, c->ntiles + 1);
     dalloc(&c->int_chunk_start, c->ntiles + 1);
-    dalloc(&c->counters, 16);
+    dalloc(&c->counters, 32);  // [16, 24): per-XCD work counters of the short-range pass
     dalloc(&c->escape, 1);
     ensure_scan_scratch(c, nkeys);
     std::vector<int4> tab = deposit_range_table(c->prm.interpolation);

@@ -33,7 +33,7 @@ namespace {
 
 constexpr int SR_THREADS = 256;
 constexpr int SR_ICHUNK = 64;
-constexpr int CNT_WORK = 1;  // ctx->counters slot of the work counter
+constexpr int CNT_WORK = 16;  // ctx->counters slots of the per-XCD work counters (NXCD)
 constexpr float kGc = 0.079577471545947668f;  // 1/(4 pi), the code-unit coupling
 
 __device__ __forceinline__ int clampi(int a, int lo, int hi) { return a < lo ? lo : (a > hi ? hi : a); }
@@ -183,10 +183,10 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
   const int total = chunk_start[nunit];
   for (;;) {
     __syncthreads();
-    if (threadIdx.x == 0) s_w = atomicAdd(&counters[CNT_WORK], 1);
+    if (threadIdx.x == 0) s_w = next_item(counters + CNT_WORK, total, blockIdx.x & (NXCD - 1));
     __syncthreads();
     const int w = s_w;
-    if (w >= total) break;
+    if (w < 0) break;
     // a work item: 64 consecutive particles of one unit -- a cell, or a
     // (y, z) row of cells whose particles may span x cells [cx0, cx1]
     // (sorted by x cell within the row)
@@ -293,10 +293,10 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr_count(
   unsigned long long accepted = 0, tested = 0, issued = 0;
   for (;;) {
     __syncthreads();
-    if (threadIdx.x == 0) s_w = atomicAdd(&counters[CNT_WORK], 1);
+    if (threadIdx.x == 0) s_w = next_item(counters + CNT_WORK, total, blockIdx.x & (NXCD - 1));
     __syncthreads();
     const int w = s_w;
-    if (w >= total) break;
+    if (w < 0) break;
     // a work item: 64 consecutive particles of one unit -- a cell, or a
     // (y, z) row of cells whose particles may span x cells [cx0, cx1]
     // (sorted by x cell within the row)
@@ -496,7 +496,7 @@ static void sr_sort(pmgpu_ctx* c, int iteration) {
   HIP_CHECK(hipGetLastError());
   exclusive_scan_i32(c, c->sr_nchunk, c->sr_chunk_start, nunit);
   expand_owner(c, c->sr_chunk_start, nunit, c->sr_owner, iteration);
-  zero_fill_i32(c, c->counters + CNT_WORK, 1);
+  zero_fill_i32(c, c->counters + CNT_WORK, NXCD);
   stage_end(c, tok);
 }
 

@@ -206,6 +206,24 @@ void mlp_apply_to_greens(pmgpu_ctx* c);
 void escape_check(pmgpu_ctx* c, int32_t* flag_dev);
 void sr_table_host(float rc, float a, float eps, int cs, std::vector<float>& table, float& delta2);
 
+// XCD-aware dynamic work distribution.  The hardware places workgroup b on
+// XCD b % 8, each XCD with its own L2.  Work items [0, total) are split into
+// NXCD contiguous segments with one counter each (cnt[0..NXCD), zeroed before
+// the launch); a workgroup takes items from its XCD's segment, so an XCD's
+// L2 holds the data of neighbouring items, and then from the other segments.
+// Returns -1 when every segment is exhausted.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int next_item(int32_t* cnt, int total, int xcd) {
+  for (int k = 0; k < NXCD; ++k) {
+    const int x = (xcd + k) & (NXCD - 1);
+    const int b = (int)((int64_t)total * x / NXCD), e = (int)((int64_t)total * (x + 1) / NXCD);
+    if (b >= e) continue;
+    const int w = b + atomicAdd(&cnt[x], 1);
+    if (w < e) return w;
+  }
+  return -1;
+}
+
 // Rank of this lane's key among all lanes of the grid that share it: one
 // integer atomic per distinct key per wavefront (lanes with equal keys are
 // ranked by lane order), so densely populated keys do not serialise 64
