@@ -242,9 +242,9 @@ int pmgpu_batch_get_tuning(pmgpu_batch* batch, int32_t sim, int32_t* knobs);
 
 /* ---- launch tuning ------------------------------------------------------ */
 /* No counterpart in the reference.  knobs[0] particles per deposit work item
-   (1024..16384, default 4096), knobs[1] particles per interpolation work item
+   (1024..16384, default 1024), knobs[1] particles per interpolation work item
    (256..16384, default 2048), knobs[2] workgroups per CU of the persistent
-   deposit and interpolation kernels (1..32, default 8), knobs[3] the
+   deposit and interpolation kernels (1..32, default 16), knobs[3] the
    short-range cell division: cells of side >= cutoff / knobs[3] and a
    (2 knobs[3] + 1)^3 neighbourhood (2..6; unless set here, p3m_configure
    picks the largest whose cell grid has at most 16M cells), knobs[4] the

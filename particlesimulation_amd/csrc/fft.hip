@@ -211,15 +211,23 @@ __global__ __launch_bounds__(256) void k_fft_x_c2r(const float2* __restrict__ sp
   for (int e = threadIdx.x; e < ROWS * L; e += 256) dst[e] = buf[(e / L) * P + (e % L)];
 }
 
-// Column tiles for the y and z passes: TW columns of kx per workgroup, the
-// last tile also takes the Nyquist column kx = Nx/2.
+// Column tiles for the y and z passes: TW columns of kx per workgroup (rows
+// of TW * 8 bytes, 128 B for TW = 16), the last tile also takes the Nyquist
+// column kx = Nx/2, transformed after the others by the first L/8 threads.
 template <int L>
 struct ColTile {
   static constexpr int T = L / 8;
-  static constexpr int TW = L <= 256 ? 16 : (L == 512 ? 8 : 4);
+  static constexpr int TW = L <= 512 ? 16 : 8;
   static constexpr int NC = TW + 1;
-  static constexpr int THREADS = NC * T;
+  static constexpr int THREADS = TW * T;
 };
+
+template <int L, bool INV>
+__device__ __forceinline__ void col_fft(float2* buf, int ncols, const float2* tw) {
+  using CT = ColTile<L>;
+  fft_lds<L, INV>(buf, 1, CT::NC, threadIdx.x % CT::TW, threadIdx.x / CT::TW, true, tw);
+  if (ncols > CT::TW) fft_lds<L, INV>(buf, 1, CT::NC, CT::TW, threadIdx.x, threadIdx.x < CT::T, tw);
+}
 
 // y pass (forward or inverse), in place on the spectrum.
 template <int L, bool INV>
@@ -240,8 +248,7 @@ __global__ __launch_bounds__(ColTile<L>::THREADS) void k_fft_y(float2* __restric
     buf[n * CT::NC + col] = base[(int64_t)n * nxh + col];
   }
   __syncthreads();
-  const int col = threadIdx.x % CT::NC, t = threadIdx.x / CT::NC;
-  fft_lds<L, INV>(buf, 1, CT::NC, col, t, col < ncols, tw);
+  col_fft<L, INV>(buf, ncols, tw);
   for (int e = threadIdx.x; e < L * ncols; e += CT::THREADS) {
     const int n = e / ncols, col2 = e % ncols;
     base[(int64_t)n * nxh + col2] = buf[n * CT::NC + col2];
@@ -273,8 +280,7 @@ __global__ __launch_bounds__(ColTile<L>::THREADS) void k_fft_z(
     buf[n * CT::NC + col] = v;
   }
   __syncthreads();
-  const int col = threadIdx.x % CT::NC, t = threadIdx.x / CT::NC;
-  if constexpr (MODE != Z_MUL_INV) fft_lds<L, false>(buf, 1, CT::NC, col, t, col < ncols, tw);
+  if constexpr (MODE != Z_MUL_INV) col_fft<L, false>(buf, ncols, tw);
   if constexpr (MODE == Z_GREEN) {
     for (int e = threadIdx.x; e < L * ncols; e += CT::THREADS) {
       const int n = e / ncols, col2 = e % ncols;
@@ -282,7 +288,7 @@ __global__ __launch_bounds__(ColTile<L>::THREADS) void k_fft_z(
     }
     __syncthreads();
   }
-  if constexpr (MODE != Z_FWD) fft_lds<L, true>(buf, 1, CT::NC, col, t, col < ncols, tw);
+  if constexpr (MODE != Z_FWD) col_fft<L, true>(buf, ncols, tw);
   for (int e = threadIdx.x; e < L * ncols; e += CT::THREADS) {
     const int n = e / ncols, col2 = e % ncols;
     dst[off + n * zstride + col2] = buf[n * CT::NC + col2];

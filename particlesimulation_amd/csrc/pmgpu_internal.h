@@ -26,9 +26,9 @@ constexpr int NCLASS = 27;
 // Work-item size of the deposit (virtual particles per work item).
 // work-item sizes (particles) of the deposit and interpolation passes: defaults
 // and limits of the tunable values (pmgpu_set_tuning)
-constexpr int DEPOSIT_CHUNK = 4096, DEPOSIT_CHUNK_MIN = 1024, DEPOSIT_CHUNK_MAX = 16384;
+constexpr int DEPOSIT_CHUNK = 1024, DEPOSIT_CHUNK_MIN = 1024, DEPOSIT_CHUNK_MAX = 16384;
 constexpr int INTERP_CHUNK = 2048, INTERP_CHUNK_MIN = 256, INTERP_CHUNK_MAX = 16384;
-constexpr int BLOCKS_PER_CU = 8, BLOCKS_PER_CU_MAX = 32;
+constexpr int BLOCKS_PER_CU = 16, BLOCKS_PER_CU_MAX = 32;
 // automatic interpolation variant: the gather variant below this many particles per tile
 constexpr int INTERP_GATHER_BELOW = 256;
 

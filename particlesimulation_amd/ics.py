@@ -164,6 +164,8 @@ C2_P3M_DISK = Workload("p3m_disk_1M_128", 1048576, (128, 128, 128), 32.0, 2, 0, 
                        a_cells=3.0, rc_cells=3.0, kind="disk")
 C3_PM_CLUMPS = Workload("pm_clumps_16M_512", 16777216, (512, 512, 512), 1.0, 2, 1, 0, 1e-3, 20, kind="clumps",
                         periodic=True)
-C5_PM_CLUMPS_256 = Workload("pm_clumps_4M_256", 4194304, (256, 256, 256), 1.0, 2, 1, 2, 1e-3, 20,
-                            a_cells=3.0, kind="clumps", periodic=True)
+# the learned-correction workload: plain PM uses the discrete-Laplacian G, the
+# learned PM multiplies it by the network's factor
+C5_PM_CLUMPS_256 = Workload("pm_clumps_4M_256", 4194304, (256, 256, 256), 1.0, 2, 1, 0, 1e-3, 20,
+                            kind="clumps", periodic=True)
 WORKLOADS = {w.name: w for w in (C1_PM_PLUMMER, C2_P3M_DISK, C3_PM_CLUMPS, C5_PM_CLUMPS_256)}
