@@ -96,10 +96,11 @@ typedef struct pmgpu_pm_params {
  * no counterpart here.
  * use_table: 0 = the direct formula (shortRangeForce, p3mMethod.cpp:220-238),
  *            1 = the reference's 500-entry table (useSRForceTable = true),
- *            2 = an accurate 2048-entry table of T r (r^2 + eps^2), which
- *                follows the direct formula to ~1e-5 (an extension: the
- *                reference's table interpolates T ~ r^-3 linearly in r^2 and
- *                is coarse for close pairs).
+ *            2 = an accurate table of the pair factor in 128 log-spaced
+ *                bins per binade of r^2 / rc^2 (cut from its float bits) over
+ *                [2^-28, 1), within 2e-5 of the direct formula (an
+ *                extension: the reference's table interpolates T ~ r^-3
+ *                linearly in r^2 and is coarse for close pairs).
  */
 typedef struct pmgpu_p3m_params {
   float cutoff;

@@ -616,27 +616,75 @@ static double ref_force_d(int cs, double r, double a) {
 
 /*
  * Accurate short-range table (an extension; use_table == 2): the direct
- * formula's pair factor T(s) = (-G/(s+eps^2) + R(sqrt s))/sqrt s of
- * shortRangeForce (p3mMethod.cpp:220-238) times sqrt(s) (s + eps^2),
- * tabulated on the reference's grid s_i = i*rc^2/(cnt-1).  The product
- * g = -G + R (s + eps^2) is smooth at s = 0, where T itself diverges (like
- * s^(-3/2) for eps = 0) and the reference's table (initSRForceTable)
- * interpolates poorly; the force is
- * F = m_i m_j lerp(g, s) / (sqrt(s) (s + eps^2)) r_ij.
+ * formula's pair factor F(s) = (-G/(r^2+eps^2) + R(r))/r of shortRangeForce
+ * (p3mMethod.cpp:220-238), r^2 = s rc^2, tabulated over s = r^2/rc^2 in bins
+ * cut from the float bits of s: bin k holds the floats whose bits >> 16 are
+ * k; bins run from 2^-ORC_LOG_BINADES to 1 (128 per binade, log-spaced, so
+ * the r^-3 singularity at small r keeps a fixed relative accuracy) and one
+ * zero bin at s = 1 takes every s >= 1.  Within a bin of binade 2^E,
+ * s = s0 + (y - 1) 2^E with y the low 16 bits of s set into the mantissa of
+ * 1.0f, and F is taken linear in y: F = T + S y, the chord through the bin's
+ * ends lowered by half the sum of its largest deviations above and below F.
+ * Bits below 2^-ORC_LOG_BINADES clamp to the first bin (pairs closer than
+ * rc 2^-14 get the force at that distance).  Built in FP64.
+ * table receives 2 * orc_sr_table_log_entries() floats (T, S).
  */
-int orc_sr_table_accurate(float rc, float a, float eps, int cs, int cnt, float* table, float* delta2) {
-  float d2 = rc * rc / (float)(cnt - 1);
-  const double G = 1.0 / (4.0 * kPi);
-  for (int i = 0; i < cnt; ++i) {
-    double s = (double)i * (double)d2;
-    double e2 = (double)eps * eps;
-    table[i] = (float)(-G + ref_force_d(cs, sqrt(s), a) * (s + e2));
+#define ORC_LOG_BINADES 28
+#define ORC_LOG_LO ((uint32_t)(127 - ORC_LOG_BINADES) << 23)
+#define ORC_LOG_HI 0x3F800000u
+
+int orc_sr_table_log_entries(void) { return (int)((ORC_LOG_HI >> 16) - (ORC_LOG_LO >> 16)) + 1; }
+
+static double orc_bits_to_double(uint32_t b) {
+  float f;
+  memcpy(&f, &b, sizeof f);
+  return (double)f;
+}
+
+static double orc_pair_factor_d(int cs, double s, double a, double e2) {
+  const double G = 1.0 / (4.0 * kPi), r = sqrt(s);
+  return (-G / (s + e2) + ref_force_d(cs, r, a)) / r;
+}
+
+int orc_sr_table_log(float rc, float a, float eps, int cs, float* table, float* inv_rc2) {
+  const float c = 1.0f / (rc * rc);
+  const double e2 = (double)eps * eps, cd = (double)c;
+  const uint32_t k0 = ORC_LOG_LO >> 16, k1 = ORC_LOG_HI >> 16;
+  for (uint32_t k = k0; k < k1; ++k) {
+    double s0 = orc_bits_to_double(k << 16), s1 = orc_bits_to_double((k + 1) << 16);
+    int E;
+    frexp(s0, &E);
+    double unit = ldexp(1.0, E - 1);
+    double f0 = orc_pair_factor_d(cs, s0 / cd, a, e2), f1 = orc_pair_factor_d(cs, s1 / cd, a, e2);
+    double lo = 0.0, hi = 0.0;
+    for (int t = 1; t < 32; ++t) {
+      double u = t / 32.0;
+      double e = f0 + u * (f1 - f0) - orc_pair_factor_d(cs, (s0 + u * (s1 - s0)) / cd, a, e2);
+      if (e < lo) lo = e;
+      if (e > hi) hi = e;
+    }
+    double S = (f1 - f0) / (s1 - s0) * unit;
+    table[2 * (k - k0)] = (float)(f0 - S - 0.5 * (lo + hi));
+    table[2 * (k - k0) + 1] = (float)S;
   }
-  *delta2 = d2;
+  table[2 * (k1 - k0)] = 0.0f;
+  table[2 * (k1 - k0) + 1] = 0.0f;
+  *inv_rc2 = c;
   return 0;
 }
 
-#define ORC_ACCURATE_TABLE 2048
+static inline float orc_sr_table_log_eval(const float* table, float r2, float inv_rc2) {
+  float s = r2 * inv_rc2;
+  uint32_t b;
+  memcpy(&b, &s, sizeof b);
+  if (b < ORC_LOG_LO) b = ORC_LOG_LO;
+  if (b > ORC_LOG_HI) b = ORC_LOG_HI;
+  uint32_t k = (b >> 16) - (ORC_LOG_LO >> 16);
+  uint32_t yb = (b & 0xFFFFu) | 0x3F800000u;
+  float y;
+  memcpy(&y, &yb, sizeof y);
+  return table[2 * k] + table[2 * k + 1] * y;
+}
 
 /* P3MMethod::shortRangeForce, p3mMethod.cpp:220-238 (direct formula). */
 static inline void sr_force_direct(const float rij[3], float mi, float mj, int cs, float a,
@@ -699,7 +747,7 @@ static int cmp_yid(const void* pa, const void* pb) {
  * Every (particle, slot) accumulator has a single writer cell, so the cell
  * loop runs on OpenMP threads exactly as the reference's std::thread pool.
  * use_table: 0 direct formula, 1 the reference's table, 2 the accurate table
- * of orc_sr_table_accurate (an extension, same chaining and accumulation).
+ * of orc_sr_table_log (an extension, same chaining and accumulation).
  * acc_out (n x 3) receives (sum of slots + own) / m.  cell_mask (one byte per
  * chaining cell, NULL = all) restricts the pass to the marked cells; a
  * particle's result is then complete only if all 27 cells around it are
@@ -772,9 +820,10 @@ int orc_p3m_sr(int64_t n, const float* pos, const float* mass, const float box_o
   }
   if (cell_mask) memcpy(active, cell_mask, (size_t)ncell);
   else memset(active, 1, (size_t)ncell);
-  float table[ORC_ACCURATE_TABLE], d2 = 0.0f;
-  const int tmax = (use_table == 2 ? ORC_ACCURATE_TABLE : 500) - 2;
-  if (use_table == 2) orc_sr_table_accurate(rc, a, eps, cs, ORC_ACCURATE_TABLE, table, &d2);
+  float* table = (float*)malloc(sizeof(float) * 2 * (size_t)orc_sr_table_log_entries());
+  float d2 = 0.0f, inv_rc2 = 0.0f;
+  if (!table) return -1;
+  if (use_table == 2) orc_sr_table_log(rc, a, eps, cs, table, &inv_rc2);
   else orc_sr_table(rc, a, eps, cs, 500, table, &d2);
   const float rc2 = rc * rc;
   int64_t pairs = 0;
@@ -803,12 +852,7 @@ int orc_p3m_sr(int64_t n, const float* pos, const float* mass, const float box_o
           if (!use_table) {
             sr_force_direct(rij, mi, mj, cs, a, eps, f);
           } else if (use_table == 2) {
-            float ksi = r2 / d2;
-            int t = (int)ksi;
-            if (t > tmax) t = tmax;
-            float g = table[t] + (ksi - t) * (table[t + 1] - table[t]);
-            float q = 1.0f / (sqrtf(r2) * (r2 + eps * eps));
-            float F = mi * mj * g * q;
+            float F = mi * mj * orc_sr_table_log_eval(table, r2, inv_rc2);
             f[0] = F * rij[0];
             f[1] = F * rij[1];
             f[2] = F * rij[2];
@@ -855,6 +899,7 @@ int orc_p3m_sr(int64_t n, const float* pos, const float* mass, const float box_o
   }
   if (pairs_out) *pairs_out = pairs;
   free(cell_of); free(count); free(start); free(list); free(own); free(slots); free(active);
+  free(table);
   return 0;
 }
 

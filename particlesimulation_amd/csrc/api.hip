@@ -464,7 +464,7 @@ int pmgpu_p3m_configure(pmgpu_ctx* c, const pmgpu_p3m_params* p) {
     }
   }
   sr_cells_setup(c);
-  dalloc(&c->sr_table, SR_TABLE_ACC);  // the larger of the two tables
+  dalloc(&c->sr_table, SR_TABLE_LOG);  // the larger of the two tables
   dalloc(&c->pair_count, 3);
   c->p3m_ready = true;
   if (c->n > 0) {
@@ -625,7 +625,7 @@ int pmgpu_set_tuning(pmgpu_ctx* c, const int32_t* knobs) {
     fail(PMGPU_ERR_INVALID_ARGUMENT, "interpolation chunk out of range");
   if (bpc < 1 || bpc > BLOCKS_PER_CU_MAX) fail(PMGPU_ERR_INVALID_ARGUMENT, "blocks per CU out of range");
   if (div < SR_CELL_DIV || div > SR_CELL_DIV_MAX) fail(PMGPU_ERR_INVALID_ARGUMENT, "short-range cell division out of range");
-  if (ik < 0 || ik > 2) fail(PMGPU_ERR_INVALID_ARGUMENT, "interpolation variant must be 0, 1 or 2");
+  if (ik < 0 || ik > 3) fail(PMGPU_ERR_INVALID_ARGUMENT, "interpolation variant must be 0 .. 3");
   c->interp_kernel = ik;
   if (sc < 0 || sc > 2) fail(PMGPU_ERR_INVALID_ARGUMENT, "short-range chunking must be 0, 1 or 2");
   c->sr_chunking = sc;

@@ -288,6 +288,103 @@ __global__ __launch_bounds__(256) void k_interp_gather(
   }
 }
 
+// Interpolation straight from the potential, one particle per thread, with
+// no field grid: the field interpolated with the assignment weights w and
+// the difference stencil s (E = -sum_o s_o phi(x + o)) is, per component,
+// a separable sum over the potential with the derivative axis weighted by
+// the convolution D = w * s, e.g. a_x = -sum D_x(X) w_y(Y) w_z(Z) phi(X, Y, Z).
+// The same quantity as k_interp up to rounding (the order of the sums
+// differs).
+template <int SCHEME, int FDS>
+__global__ __launch_bounds__(256) void k_interp_phi(
+    const float* __restrict__ phi, int nx, int ny, int nz, int64_t n, float4* __restrict__ pm,
+    float4* __restrict__ vel, float4* __restrict__ acc, int mode, float kick, float3 box, ExtField ext,
+    int32_t* __restrict__ escape, int iteration) {
+  if (escape[0] <= iteration) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  constexpr int SLO = SCHEME == PMGPU_TSC ? -1 : 0;
+  constexpr int CNT = SCHEME == PMGPU_TSC ? 3 : (SCHEME == PMGPU_CIC ? 2 : 1);
+  constexpr int F = FDS == PMGPU_TWO_POINT ? 1 : 2;
+  constexpr int EXT = CNT + 2 * F;
+  // difference stencil s_o, o = -F .. F (E = -sum_o s_o phi(x + o))
+  constexpr float S2[3] = {-0.5f, 0.0f, 0.5f};
+  constexpr float S4[5] = {1.0f / 12, -8.0f / 12, 0.0f, 8.0f / 12, -1.0f / 12};
+  float4 p = pm[i];
+  const int ax = anchor_of(p.x, SCHEME), ay = anchor_of(p.y, SCHEME), az = anchor_of(p.z, SCHEME);
+  float wx[3], wy[3], wz[3];
+  interp_weights<SCHEME>(p.x - (float)ax, wx);
+  interp_weights<SCHEME>(p.y - (float)ay, wy);
+  interp_weights<SCHEME>(p.z - (float)az, wz);
+  float dx[EXT], dy[EXT], dz[EXT];
+#pragma unroll
+  for (int e = 0; e < EXT; ++e) dx[e] = dy[e] = dz[e] = 0.0f;
+#pragma unroll
+  for (int a = 0; a < CNT; ++a)
+#pragma unroll
+    for (int o = -F; o <= F; ++o) {
+      const float so = F == 1 ? S2[o + 1] : S4[o + 2];
+      if (so == 0.0f) continue;
+      dx[a + F + o] = fmaf(wx[a], so, dx[a + F + o]);
+      dy[a + F + o] = fmaf(wy[a], so, dy[a + F + o]);
+      dz[a + F + o] = fmaf(wz[a], so, dz[a + F + o]);
+    }
+  int gx[EXT], gy[EXT], gz[EXT];
+#pragma unroll
+  for (int e = 0; e < EXT; ++e) {
+    gx[e] = wrapc(ax + SLO - F + e, nx);
+    gy[e] = wrapc(ay + SLO - F + e, ny) * nx;
+    gz[e] = wrapc(az + SLO - F + e, nz) * ny;
+  }
+  auto P = [&](int a, int b, int c) { return phi[(int64_t)gx[a] + (int64_t)gy[b] + (int64_t)nx * gz[c]]; };
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+#pragma unroll
+  for (int b = 0; b < CNT; ++b)
+#pragma unroll
+    for (int c = 0; c < CNT; ++c) {
+      const float wyz = wy[b] * wz[c];
+      float line = 0.0f;
+#pragma unroll
+      for (int e = 0; e < EXT; ++e) line = fmaf(dx[e], P(e, F + b, F + c), line);
+      sx = fmaf(wyz, line, sx);
+    }
+#pragma unroll
+  for (int a = 0; a < CNT; ++a)
+#pragma unroll
+    for (int c = 0; c < CNT; ++c) {
+      const float wxz = wx[a] * wz[c];
+      float line = 0.0f;
+#pragma unroll
+      for (int e = 0; e < EXT; ++e) line = fmaf(dy[e], P(F + a, e, F + c), line);
+      sy = fmaf(wxz, line, sy);
+    }
+#pragma unroll
+  for (int a = 0; a < CNT; ++a)
+#pragma unroll
+    for (int b = 0; b < CNT; ++b) {
+      const float wxy = wx[a] * wy[b];
+      float line = 0.0f;
+#pragma unroll
+      for (int e = 0; e < EXT; ++e) line = fmaf(dz[e], P(F + a, F + b, e), line);
+      sz = fmaf(wxy, line, sz);
+    }
+  const float k = SCHEME == PMGPU_TSC ? -0.125f : -1.0f;
+  sx *= k;
+  sy *= k;
+  sz *= k;
+  const float3 g = ext_field(ext, p.x, p.y, p.z);
+  sx += g.x;
+  sy += g.y;
+  sz += g.z;
+  if (mode & MODE_ACC) acc[i] = make_float4(sx, sy, sz, 0.f);
+  if (mode & (MODE_KICK | MODE_DRIFT)) {
+    float4 v = vel[i];
+    leapfrog_tail(p, v, sx, sy, sz, mode, kick, box, escape, iteration);
+    vel[i] = v;
+    pm[i] = p;
+  }
+}
+
 // Leapfrog pieces on the sorted state (source/leapfrog.cpp:5-24).
 __global__ void k_leapfrog(float4* __restrict__ pm, float4* __restrict__ vel,
                            const float4* __restrict__ acc, float4* __restrict__ vint, int64_t n,
@@ -381,8 +478,32 @@ static void interpolate_gather(pmgpu_ctx* c, int mode, float kick, int iteration
   HIP_CHECK(hipGetLastError());
 }
 
+// The potential-gather variant: no field grid.
+static void interpolate_phi(pmgpu_ctx* c, int mode, float kick, int iteration) {
+  const unsigned gp = (unsigned)((c->n + 255) / 256);
+  const ExtField e = ext_of(c);
+  const float3 box = box_of(c);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(gp), dim3(256), 0, c->stream, (const float*)c->phi, c->nx, c->ny, c->nz, c->n,
+                       c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick, box, e, c->escape, iteration);
+  };
+  const int s = c->prm.interpolation, f = c->prm.finite_diff;
+  if (s == PMGPU_NGP && f == PMGPU_TWO_POINT) go(k_interp_phi<PMGPU_NGP, PMGPU_TWO_POINT>);
+  else if (s == PMGPU_NGP) go(k_interp_phi<PMGPU_NGP, PMGPU_FOUR_POINT>);
+  else if (s == PMGPU_CIC && f == PMGPU_TWO_POINT) go(k_interp_phi<PMGPU_CIC, PMGPU_TWO_POINT>);
+  else if (s == PMGPU_CIC) go(k_interp_phi<PMGPU_CIC, PMGPU_FOUR_POINT>);
+  else if (f == PMGPU_TWO_POINT) go(k_interp_phi<PMGPU_TSC, PMGPU_TWO_POINT>);
+  else go(k_interp_phi<PMGPU_TSC, PMGPU_FOUR_POINT>);
+  HIP_CHECK(hipGetLastError());
+}
+
 void interpolate(pmgpu_ctx* c, int mode, float kick, int iteration) {
   int tok = stage_begin(c, "interp");
+  if (c->interp_kernel == 3) {
+    interpolate_phi(c, mode, kick, iteration);
+    stage_end(c, tok);
+    return;
+  }
   // auto: the gather variant when tiles are sparse (the tiled variant's
   // per-tile staging of the potential and field then dominates)
   const bool gather = c->interp_kernel == 2 || (c->interp_kernel == 0 && c->n < (int64_t)INTERP_GATHER_BELOW * c->ntiles);

@@ -26,6 +26,8 @@
 #include "pmgpu_internal.h"
 
 #include <cmath>
+#include <cstring>
+#include <algorithm>
 
 namespace pmgpu {
 
@@ -110,7 +112,8 @@ struct SrParams {
   float inv_d2, rc2;
   float a, eps2;
   int cs;
-  float tmax_f;  // last table index; entries past the cutoff are zero
+  float tmax_f;   // last index of the reference table; entries past the cutoff are zero
+  float inv_rc2;  // 1 / rc^2: the accurate table's argument is s = r^2 / rc^2
 };
 
 // Pair factor F(r^2) such that a_i += m_j F (x_i - x_j).
@@ -118,8 +121,16 @@ struct SrParams {
 //   MODE 1: shortRangeForceFromTable (p3mMethod.cpp:240-245) on the
 //           reference's 500-entry table, zero-padded past the cutoff so the
 //           r < rc test is the table range itself;
-//   MODE 2: the accurate table g = T r (s + eps^2), F = g / (r (s + eps^2)).
-template <int MODE, bool SOFT>
+//   MODE 2: the accurate table of F itself over s = r^2 / rc^2, binned by the
+//           float bits of s: bits >> 16 selects one of 128 bins per binade
+//           (log-spaced, so the r^-3 and r^-1 singularities at small r are
+//           resolved to a fixed relative accuracy), and the low 16 bits, set
+//           into the mantissa of 1.0f, give y in [1, 1 + 2^-7) on which F is
+//           linear within the bin: F = T + S y.  The bits are clamped to
+//           [2^-28, 1]: pairs closer than rc 2^-14 take the value at that
+//           distance, and s >= 1 (and a NaN) lands on the zero entry.
+//           The LDS copy holds bin k at k mod SR_LOG_WRAP.
+template <int MODE>
 __device__ __forceinline__ float pair_factor(float r2, float mj, const float2* table,
                                              const SrParams& P) {
   if constexpr (MODE == 0) {
@@ -127,20 +138,19 @@ __device__ __forceinline__ float pair_factor(float r2, float mj, const float2* t
     const float R = reference_force(P.cs, r, P.a);
     const float F = (-kGc / (r2 + P.eps2) + R) / r;
     return (r2 < P.rc2 && r2 > 0.0f) ? F * mj : 0.0f;
-  } else {
+  } else if constexpr (MODE == 1) {
     // t = min(floor(ksi), tmax); a NaN r^2 also lands on the zero entry
     // tmax.  Past the cutoff every entry is (0, 0), so the fraction of an
     // out-of-range ksi is irrelevant.
     const float ksi = r2 * P.inv_d2;
     const int t = (int)fminf(ksi, P.tmax_f);
     const float2 tv = table[t];
-    float F = fmaf(__builtin_amdgcn_fractf(ksi), tv.y, tv.x) * mj;
-    if constexpr (MODE == 2) {
-      const float q = __frsqrt_rn(fmaxf(r2, 1.0e-12f));  // self pair: finite F times d = 0
-      if constexpr (SOFT) F *= q * __frcp_rn(r2 + P.eps2);
-      else F *= q * q * q;
-    }
-    return F;
+    return fmaf(__builtin_amdgcn_fractf(ksi), tv.y, tv.x) * mj;
+  } else {
+    const uint32_t b = min(max(__float_as_uint(r2 * P.inv_rc2), SR_LOG_LO), SR_LOG_HI);
+    const float2 tv = table[(b >> SR_LOG_SHIFT) & (SR_LOG_WRAP - 1)];
+    const float y = __uint_as_float((b & ((1u << SR_LOG_SHIFT) - 1u)) | 0x3F800000u);
+    return fmaf(tv.y, y, tv.x) * mj;
   }
 }
 
@@ -163,7 +173,7 @@ __device__ __forceinline__ bool row_extent(const SrParams& P, int dy, int dz, in
   return true;
 }
 
-template <int MODE, bool SOFT>
+template <int MODE>
 __global__ __launch_bounds__(SR_THREADS) void k_sr(
     const float4* __restrict__ sr_pm, const int32_t* __restrict__ sr_start,
     const int32_t* __restrict__ chunk_start, const int32_t* __restrict__ owner, int nunit,
@@ -172,12 +182,24 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
     float4* __restrict__ vel, float4* __restrict__ acc, int mode, float kick, float3 box,
     int32_t* __restrict__ counters, int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
-  constexpr int TS = MODE == 2 ? SR_TABLE_ACC : SR_TABLE;
-  __shared__ float2 table[MODE == 0 ? 1 : TS];
-  __shared__ float3 part[SR_THREADS];
-  __shared__ int s_w;
-  if constexpr (MODE != 0)
-    for (int i = threadIdx.x; i < TS; i += SR_THREADS) table[i] = table_g[i];
+  // One LDS array (so the table starts at address 0): MODE 2 keeps its bins
+  // at k mod SR_LOG_WRAP and the partial sums and work-item slot in the
+  // entries below and above them; MODE 1 has the table, then the rest.
+  constexpr uint32_t KLO = (SR_LOG_LO >> SR_LOG_SHIFT) % SR_LOG_WRAP;
+  constexpr uint32_t KHI = (SR_LOG_HI >> SR_LOG_SHIFT) % SR_LOG_WRAP;
+  static_assert(KLO < KHI && KLO >= SR_THREADS * 3 / 2 && KHI + 1 < SR_LOG_WRAP,
+                "the wrapped table must leave room for the partial sums");
+  constexpr int PART = MODE == 2 ? 0 : (MODE == 1 ? SR_TABLE : 0);
+  constexpr int SLOT = MODE == 2 ? SR_LOG_WRAP - 1 : PART + SR_THREADS * 3 / 2;
+  constexpr int LDS = MODE == 2 ? SR_LOG_WRAP : SLOT + 1;
+  __shared__ float2 lds[LDS];
+  float2* table = lds;
+  float3* part = reinterpret_cast<float3*>(lds + PART);
+  int& s_w = *reinterpret_cast<int*>(lds + SLOT);
+  if constexpr (MODE == 1)
+    for (int i = threadIdx.x; i < SR_TABLE; i += SR_THREADS) table[i] = table_g[i];
+  if constexpr (MODE == 2)
+    for (int i = threadIdx.x; i < SR_TABLE_LOG; i += SR_THREADS) table[(KLO + i) % SR_LOG_WRAP] = table_g[i];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int total = chunk_start[nunit];
@@ -234,7 +256,7 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
             const float4 pj = sr_pm[j];  // uniform address: scalar load
             const float dx = pi.x - pj.x, dy_ = pi.y - pj.y, dz_ = pi.z - pj.z;
             const float r2 = fmaf(dz_, dz_, fmaf(dy_, dy_, dx * dx));
-            const float F = pair_factor<MODE, SOFT>(r2, pj.w, table, P);
+            const float F = pair_factor<MODE>(r2, pj.w, table, P);
             ax = fmaf(F, dx, ax);
             ay = fmaf(F, dy_, ay);
             az = fmaf(F, dz_, az);
@@ -418,22 +440,42 @@ void sr_table_host(float rc, float a, float eps, int cs, std::vector<float>& tab
   }
 }
 
-// The accurate table: g(s) = -G + R(sqrt s) (s + eps^2), the direct formula's
-// pair factor times sqrt(s) (s + eps^2), on SR_TABLE_ACC points in s.
-static void sr_table_accurate_host(float rc, float a, float eps, int cs, std::vector<float>& table,
-                                   float& d2) {
-  const int cnt = SR_TABLE_ACC;
-  d2 = rc * rc / (float)(cnt - 1);
-  const double G = 1.0 / (4.0 * M_PI);
-  std::vector<float> g(cnt);
-  for (int i = 0; i < cnt; ++i) {
-    double s = (double)i * (double)d2;
-    g[i] = (float)(-G + ref_force_d(cs, std::sqrt(s), a) * (s + (double)eps * eps));
-  }
-  table.assign(2 * SR_TABLE_ACC, 0.0f);
-  for (int i = 0; i + 1 < cnt; ++i) {
-    table[2 * i] = g[i];
-    table[2 * i + 1] = g[i + 1] - g[i];
+// The accurate table of the direct formula's pair factor
+// F(s) = (-G/(r^2 + eps^2) + R(r)) / r, r^2 = s rc^2 (shortRangeForce,
+// p3mMethod.cpp:220-238), in the bit-binned layout pair_factor<2> reads:
+// entry k covers the floats s whose bits >> 16 equal k + (2^-24 bits >> 16);
+// within it s = s0 + (y - 1) 2^E (E the binade of s0), and the entry is the
+// chord of F through the bin's ends, written as T + S y and lowered by half
+// the sum of its largest deviations above and below F (so the error is
+// centred); the last entry (s >= 1) is zero.  Built in FP64.
+static void sr_table_log_host(float rc, float a, float eps, int cs, std::vector<float>& table,
+                              float& inv_rc2) {
+  inv_rc2 = 1.0f / (rc * rc);
+  const double c = (double)inv_rc2, G = 1.0 / (4.0 * M_PI), e2 = (double)eps * eps;
+  auto F = [&](double sp) {
+    const double s = sp / c, r = std::sqrt(s);
+    return (-G / (s + e2) + ref_force_d(cs, r, a)) / r;
+  };
+  auto from_bits = [](uint32_t b) {
+    float f;
+    std::memcpy(&f, &b, sizeof f);
+    return (double)f;
+  };
+  table.assign(2 * SR_TABLE_LOG, 0.0f);
+  const uint32_t k0 = SR_LOG_LO >> SR_LOG_SHIFT, k1 = SR_LOG_HI >> SR_LOG_SHIFT;
+  for (uint32_t k = k0; k < k1; ++k) {
+    const double s0 = from_bits(k << SR_LOG_SHIFT), s1 = from_bits((k + 1) << SR_LOG_SHIFT);
+    const double unit = std::ldexp(1.0, std::ilogb(s0));
+    const double f0 = F(s0), f1 = F(s1);
+    double lo = 0.0, hi = 0.0;
+    for (int t = 1; t < 32; ++t) {
+      const double u = t / 32.0, e = f0 + u * (f1 - f0) - F(s0 + u * (s1 - s0));
+      lo = std::min(lo, e);
+      hi = std::max(hi, e);
+    }
+    const double S = (f1 - f0) / (s1 - s0) * unit;
+    table[2 * (k - k0)] = (float)(f0 - S - 0.5 * (lo + hi));
+    table[2 * (k - k0) + 1] = (float)S;
   }
 }
 
@@ -441,9 +483,10 @@ void p3m_setup(pmgpu_ctx* c) {
   std::vector<float> tab;
   float d2 = 0.0f;
   if (c->p3m.use_table == 2) {
-    sr_table_accurate_host(c->p3m.cutoff, c->p3m.particle_diameter, c->p3m.softening,
-                           c->p3m.cloud_shape, tab, d2);
-    c->sr_tmax = SR_TABLE_ACC - 1;
+    sr_table_log_host(c->p3m.cutoff, c->p3m.particle_diameter, c->p3m.softening,
+                      c->p3m.cloud_shape, tab, c->sr_inv_rc2);
+    d2 = 1.0f;
+    c->sr_tmax = SR_TABLE_LOG - 1;
   } else {
     sr_table_host(c->p3m.cutoff, c->p3m.particle_diameter, c->p3m.softening, c->p3m.cloud_shape,
                   tab, d2);
@@ -475,6 +518,7 @@ static SrParams sr_params(pmgpu_ctx* c) {
   P.eps2 = c->p3m.softening * c->p3m.softening;
   P.cs = c->p3m.cloud_shape;
   P.tmax_f = (float)c->sr_tmax;
+  P.inv_rc2 = c->sr_inv_rc2;
   return P;
 }
 
@@ -500,10 +544,10 @@ static void sr_sort(pmgpu_ctx* c, int iteration) {
   stage_end(c, tok);
 }
 
-template <int MODE, bool SOFT>
+template <int MODE>
 static void launch_sr(pmgpu_ctx* c, int mode, float kick, int iteration) {
   const int grid = c->num_cu * 8;
-  hipLaunchKernelGGL((k_sr<MODE, SOFT>), dim3(grid), dim3(SR_THREADS), 0, c->stream, c->sr_pm,
+  hipLaunchKernelGGL((k_sr<MODE>), dim3(grid), dim3(SR_THREADS), 0, c->stream, c->sr_pm,
                      c->sr_start, c->sr_chunk_start, c->sr_owner, sr_units(c), c->sr_idx, c->sr_table,
                      sr_params(c), c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick,
                      make_float3(c->prm.box[0], c->prm.box[1], c->prm.box[2]), c->counters,
@@ -514,13 +558,10 @@ static void launch_sr(pmgpu_ctx* c, int mode, float kick, int iteration) {
 void p3m_short_range(pmgpu_ctx* c, int mode, float kick, int iteration) {
   sr_sort(c, iteration);
   int tok = stage_begin(c, "sr_pairs");
-  const bool soft = c->p3m.softening != 0.0f;
   switch (c->p3m.use_table) {
-    case 0: launch_sr<0, false>(c, mode, kick, iteration); break;
-    case 1: launch_sr<1, false>(c, mode, kick, iteration); break;
-    default:
-      if (soft) launch_sr<2, true>(c, mode, kick, iteration);
-      else launch_sr<2, false>(c, mode, kick, iteration);
+    case 0: launch_sr<0>(c, mode, kick, iteration); break;
+    case 1: launch_sr<1>(c, mode, kick, iteration); break;
+    default: launch_sr<2>(c, mode, kick, iteration);
   }
   stage_end(c, tok);
 }

@@ -36,7 +36,16 @@ constexpr int INTERP_GATHER_BELOW = 256;
 constexpr int SR_CELL_DIV = 2, SR_CELL_DIV_MAX = 6;
 constexpr int SR_TABLE = 512;  // padded reference table (the reference uses 500 entries)
 constexpr int SR_TABLE_REF = 500;
-constexpr int SR_TABLE_ACC = 2048;  // accurate (r (s+eps^2)-scaled) table
+// The accurate pair table: bins of s = r^2 / rc^2 taken from the float bits
+// of s (2^(23 - SR_LOG_SHIFT) bins per binade) over SR_LOG_BINADES binades
+// below s = 1, plus a zero entry for s >= 1.  In LDS, bin k sits at
+// k mod SR_LOG_WRAP, so its byte address is a shift and a mask of the bits.
+constexpr int SR_LOG_SHIFT = 16;
+constexpr int SR_LOG_BINADES = 28;
+constexpr int SR_LOG_WRAP = 4096;
+constexpr uint32_t SR_LOG_LO = (uint32_t)(127 - SR_LOG_BINADES) << 23;  // bits of 2^-28
+constexpr uint32_t SR_LOG_HI = 0x3F800000u;                              // bits of 1.0f
+constexpr int SR_TABLE_LOG = (int)((SR_LOG_HI >> SR_LOG_SHIFT) - (SR_LOG_LO >> SR_LOG_SHIFT)) + 1;
 
 struct Stage {
   const char* name;
@@ -131,8 +140,9 @@ struct pmgpu_ctx {
   pmgpu_p3m_params p3m{};
   int mcx = 0, mcy = 0, mcz = 0, ncell_sr = 0;
   float hcx = 0, hcy = 0, hcz = 0;
-  float2* sr_table = nullptr;           // SR_TABLE_ACC entries (T[t], T[t+1]-T[t])
+  float2* sr_table = nullptr;           // SR_TABLE_LOG entries (the larger of the two tables)
   float sr_inv_d2 = 0.0f;
+  float sr_inv_rc2 = 0.0f;               // 1 / rc^2, the accurate table's argument scale
   int sr_tmax = 0;
   int32_t* sr_count = nullptr;          // ncell_sr + 1
   int32_t* sr_start = nullptr;          // ncell_sr + 1

@@ -77,7 +77,8 @@ def test_tuning_does_not_change_results(gpu_lib):
     m = np.full(50000, 1.0 / 50000, np.float32)
     ref = None
     for knobs in ((4096, 2048, 8, 2, 1, 0), (1024, 512, 2, 2, 1, 1), (16384, 16384, 32, 3, 1, 2),
-                  (2048, 256, 1, 4, 1, 0), (4096, 2048, 8, 2, 2, 0), (1024, 512, 4, 2, 0, 0)):
+                  (2048, 256, 1, 4, 1, 0), (4096, 2048, 8, 2, 2, 0), (1024, 512, 4, 2, 0, 0),
+                  (1024, 512, 4, 2, 3, 0)):
         pm = P.PM((g, g, g), interpolation=P.TSC, finite_diff=P.FOUR_POINT, greens_function=P.DISCRETE_LAPLACIAN,
                   box=(0, 0, 0))
         pm.set_tuning(*knobs)
@@ -94,14 +95,17 @@ def test_tuning_does_not_change_results(gpu_lib):
         if knobs[4] == 1:
             assert np.array_equal(out[1], ref[1]), knobs
         else:
-            # the gather variant forms E in another kernel: equal up to FMA contraction
-            assert np.max(np.abs(out[1] - ref[1])) <= 1e-6 * np.abs(ref[1]).max(), knobs
+            # the gather variants form E differently: equal up to rounding; the potential
+            # gather sums differences of phi per particle, so its rounding scales with |phi|
+            tol = 1e-6 if knobs[4] == 2 else 1e-5
+            assert np.max(np.abs(out[1] - ref[1])) <= tol * np.abs(ref[1]).max(), knobs
     with pytest.raises(P.InvalidArgument):
         pm.set_tuning(512, 2048, 8, 2, 0, 0)
 
 
-@pytest.mark.parametrize("scheme,fds", [(0, 0), (1, 1), (2, 0), (2, 1)])
-def test_gather_interpolation_matches_oracle(gpu_lib, oracle, scheme, fds):
+@pytest.mark.parametrize("variant", [2, 3])
+@pytest.mark.parametrize("scheme,fds", [(0, 0), (1, 1), (2, 0), (2, 1), (1, 0), (0, 1)])
+def test_gather_interpolation_matches_oracle(gpu_lib, oracle, scheme, fds, variant):
     P = gpu_lib
     grid = (64, 32, 64)
     rng = np.random.default_rng(21 + scheme)
@@ -109,7 +113,7 @@ def test_gather_interpolation_matches_oracle(gpu_lib, oracle, scheme, fds):
     pos = pos.astype(np.float32)
     m = rng.uniform(0.5, 1.5, 20000).astype(np.float32) / 20000
     pm = P.PM(grid, interpolation=scheme, finite_diff=fds, greens_function=P.DISCRETE_LAPLACIAN, box=(0, 0, 0))
-    pm.set_tuning(interp_kernel=2)
+    pm.set_tuning(interp_kernel=variant)
     pm.set_particles(pos, None, m)
     pm.init_greens()
     pm.pm_step()

@@ -438,3 +438,30 @@ def test_direct_sum_two_bodies(oracle):
     assert np.allclose(acc1[0, 0], GC * 5 * 2 / 5 ** 1.5)
     acc2 = oracle.direct_sum(pos, m, GC, eps=1.0, kernel=2, targets=[1])
     assert np.allclose(acc2[0, 0], -GC * 3 / 5)
+
+
+@pytest.mark.parametrize("cs,eps", [(0, 0.0), (1, 0.0), (1, 0.4)])
+def test_sr_log_table_follows_the_pair_factor(oracle, cs, eps):
+    # the bit-binned table against the closed-form pair factor in FP64: relative
+    # error bounded where F is not near a zero, exactly zero from the cutoff on,
+    # and the value at 2^-28 rc^2 for closer pairs
+    rc, a = 3.0, 2.5
+    tab, c = oracle.sr_table_log(rc, a, eps, cs)
+    assert tab.shape == (28 * 128 + 1, 2) and c == np.float32(1 / 9)
+    def exact(r):
+        return (-GC / (r * r + eps * eps) + np.vectorize(lambda x: _ref_force64(cs, x, a))(r)) / r
+
+    def lookup(r2):
+        s = (np.float32(r2) * np.float32(c)).astype(np.float32)
+        b = np.clip(s.view(np.uint32), (127 - 28) << 23, 0x3F800000)
+        k = (b >> 16) - (((127 - 28) << 23) >> 16)
+        y = ((b & 0xFFFF) | 0x3F800000).astype(np.uint32).view(np.float32)
+        return tab[k, 0] + tab[k, 1] * y
+
+    r = np.exp(_rng(15).uniform(np.log(1e-2), np.log(0.6 * a), 20000))
+    r2 = (r * r).astype(np.float32)
+    ref = exact(np.sqrt(r2.astype(np.float64)))
+    assert np.max(np.abs(lookup(r2) - ref) / np.abs(ref)) < 3e-5
+    assert np.all(lookup(np.float32([9.0, 9.5, 40.0, np.inf, np.nan])) == 0.0)
+    tiny = np.float32([1e-12, 1e-20, 0.0])
+    assert np.all(lookup(tiny) == lookup(np.float32([9.0 * 2.0 ** -28]))[0])
