@@ -13,11 +13,11 @@
 //    leave lanes idle; tunable);
 //    its j-stream is the contiguous (dz, dy) rows of the neighbourhood, each
 //    trimmed to the x cells that can hold a pair within the cutoff (corner
-//    rows drop out for div >= 3).  The four
+//    rows drop out for div >= 3).  The SR_WAVES
 //    waves of a workgroup take interleaved 64-particle blocks of that stream
 //    and read each j through the scalar unit (uniform address), so the inner
 //    loop is pure FP32 VALU with scalar operands;
-//  - the four partial sums are added in LDS in fixed wave order; the pair
+//  - the per-wave partial sums are added in LDS in fixed wave order; the pair
 //    force comes from a table in LDS or from the direct formula.
 // Newton's third law is not used: every ordered pair is evaluated once on
 // the side of its i, which removes all write conflicts.  The mass m_i that
@@ -33,7 +33,8 @@ namespace pmgpu {
 
 namespace {
 
-constexpr int SR_THREADS = 256;
+constexpr int SR_THREADS = 512;
+constexpr int SR_WAVES = SR_THREADS / 64;  // waves sharing a work item's j-stream
 constexpr int SR_ICHUNK = 64;
 constexpr int CNT_WORK = 16;  // ctx->counters slots of the per-XCD work counters (NXCD)
 constexpr float kGc = 0.079577471545947668f;  // 1/(4 pi), the code-unit coupling
@@ -183,14 +184,18 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
     int32_t* __restrict__ counters, int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
   // One LDS array (so the table starts at address 0): MODE 2 keeps its bins
-  // at k mod SR_LOG_WRAP and the partial sums and work-item slot in the
-  // entries below and above them; MODE 1 has the table, then the rest.
+  // at k mod SR_LOG_WRAP and the partial sums (half of the waves' at a
+  // time) and work-item slot in the entries below and above them; MODE 1 has
+  // the table, then the rest.
+  constexpr int HALF = SR_WAVES / 2;
+  constexpr int PART_F2 = HALF * 64 * 3 / 2;  // float2 entries of HALF waves' float3 sums
   constexpr uint32_t KLO = (SR_LOG_LO >> SR_LOG_SHIFT) % SR_LOG_WRAP;
   constexpr uint32_t KHI = (SR_LOG_HI >> SR_LOG_SHIFT) % SR_LOG_WRAP;
-  static_assert(KLO < KHI && KLO >= SR_THREADS * 3 / 2 && KHI + 1 < SR_LOG_WRAP,
+  static_assert(KLO < KHI && KLO >= PART_F2 && KHI + 1 < SR_LOG_WRAP,
                 "the wrapped table must leave room for the partial sums");
+  static_assert(SR_WAVES % 2 == 0, "the partial sums are added in halves");
   constexpr int PART = MODE == 2 ? 0 : (MODE == 1 ? SR_TABLE : 0);
-  constexpr int SLOT = MODE == 2 ? SR_LOG_WRAP - 1 : PART + SR_THREADS * 3 / 2;
+  constexpr int SLOT = MODE == 2 ? SR_LOG_WRAP - 1 : PART + PART_F2;
   constexpr int LDS = MODE == 2 ? SR_LOG_WRAP : SLOT + 1;
   __shared__ float2 lds[LDS];
   float2* table = lds;
@@ -249,9 +254,9 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
         const int row = P.mx * (yc + P.my * zc);
         const int js = sr_start[row + xlo], je = sr_start[row + xhi + 1];
         for (int jb = js; jb < je; jb += 64, ++blk) {
-          if ((blk & 3) != wave) continue;
+          if (blk % SR_WAVES != wave) continue;
           const int jend = min(jb + 64, je);
-#pragma unroll 4
+#pragma unroll 8
           for (int j = jb; j < jend; ++j) {
             const float4 pj = sr_pm[j];  // uniform address: scalar load
             const float dx = pi.x - pj.x, dy_ = pi.y - pj.y, dz_ = pi.z - pj.z;
@@ -264,12 +269,23 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
         }
       }
     }
-    part[threadIdx.x] = make_float3(ax, ay, az);
+    // waves HALF.. hand their sums to waves 0..HALF-1, which hand theirs to
+    // wave 0: a fixed order, whatever the timing
+    if (wave >= HALF) part[threadIdx.x - HALF * 64] = make_float3(ax, ay, az);
+    __syncthreads();
+    if (wave < HALF) {
+      const float3 o = part[threadIdx.x];
+      ax += o.x;
+      ay += o.y;
+      az += o.z;
+    }
+    __syncthreads();
+    if (wave > 0 && wave < HALF) part[threadIdx.x - 64] = make_float3(ax, ay, az);
     __syncthreads();
     if (wave == 0 && valid) {
-      float3 s = part[lane];
-      for (int q = 1; q < 4; ++q) {
-        const float3 o = part[lane + 64 * q];
+      float3 s = make_float3(ax, ay, az);
+      for (int q = 1; q < HALF; ++q) {
+        const float3 o = part[lane + 64 * (q - 1)];
         s.x += o.x;
         s.y += o.y;
         s.z += o.z;
@@ -358,7 +374,7 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr_count(
         const int row = P.mx * (yc + P.my * zc);
         const int js = sr_start[row + xlo], je = sr_start[row + xhi + 1];
         for (int jb = js; jb < je; jb += 64, ++blk) {
-          if ((blk & 3) != wave) continue;
+          if (blk % SR_WAVES != wave) continue;
           const int jend = min(jb + 64, je);
           if (valid) tested += (unsigned long long)(jend - jb);
           issued += (unsigned long long)(jend - jb);
