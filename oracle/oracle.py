@@ -69,7 +69,8 @@ def lib():
         L.orc_reference_force.restype = C.c_float
         L.orc_sr_table.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, _f32p,
                                    C.POINTER(C.c_float)]
-        L.orc_sr_table_log.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, _f32p, C.POINTER(C.c_float)]
+        L.orc_sr_table_log.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, _f32p, C.POINTER(C.c_float),
+                                       C.POINTER(C.c_uint32)]
         L.orc_p3m_sr.argtypes = [C.c_int64, _f32p, _f32p, _f32p, C.c_float, C.c_float, C.c_float, C.c_float,
                                  C.c_float, C.c_int, C.c_int, C.c_int, C.c_void_p, _f32p,
                                  C.POINTER(C.c_int64)]
@@ -218,11 +219,12 @@ def sr_table(rc, a, eps, cs, cnt=500):
 
 def sr_table_log(rc, a, eps, cs):
     """The accurate (use_table=2) table: (entries, 2) float32 (T, S) rows over
-    bit-binned s = r^2 / rc^2, and the scale 1 / rc^2."""
+    bit-binned s = r^2 * scale, the scale and the float bits of the first bin."""
     t = np.zeros((lib().orc_sr_table_log_entries(), 2), np.float32)
     c = C.c_float(0.0)
-    lib().orc_sr_table_log(float(rc), float(a), float(eps), cs, t, C.byref(c))
-    return t, c.value
+    lo = C.c_uint32(0)
+    lib().orc_sr_table_log(float(rc), float(a), float(eps), cs, t, C.byref(c), C.byref(lo))
+    return t, c.value, lo.value
 
 
 def p3m_mesh_dims(box_orig, rc_orig):

@@ -616,24 +616,23 @@ static double ref_force_d(int cs, double r, double a) {
 
 /*
  * Accurate short-range table (an extension; use_table == 2): the direct
- * formula's pair factor F(s) = (-G/(r^2+eps^2) + R(r))/r of shortRangeForce
- * (p3mMethod.cpp:220-238), r^2 = s rc^2, tabulated over s = r^2/rc^2 in bins
- * cut from the float bits of s: bin k holds the floats whose bits >> 16 are
- * k; bins run from 2^-ORC_LOG_BINADES to 1 (128 per binade, log-spaced, so
- * the r^-3 singularity at small r keeps a fixed relative accuracy) and one
- * zero bin at s = 1 takes every s >= 1.  Within a bin of binade 2^E,
+ * formula's pair factor F = (-G/(r^2+eps^2) + R(r))/r of shortRangeForce
+ * (p3mMethod.cpp:220-238), tabulated over s = r^2 scale in ORC_LOG_ENTRIES
+ * bins cut from the float bits of s: bin i holds the floats whose bits >> 16
+ * are (lo >> 16) + i, 128 bins per binade (log-spaced, so the r^-3
+ * singularity at small r keeps a fixed relative accuracy), and the last bin
+ * (bits hi) is zero and takes every larger s.  Within a bin of binade 2^E,
  * s = s0 + (y - 1) 2^E with y the low 16 bits of s set into the mantissa of
  * 1.0f, and F is taken linear in y: F = T + S y, the chord through the bin's
  * ends lowered by half the sum of its largest deviations above and below F.
- * Bits below 2^-ORC_LOG_BINADES clamp to the first bin (pairs closer than
- * rc 2^-14 get the force at that distance).  Built in FP64.
- * table receives 2 * orc_sr_table_log_entries() floats (T, S).
+ * Bits below lo clamp to the first bin.  With no softening and a <= rc
+ * (F = 0 for r >= a) the scale is 1 and hi the bin boundary at or above the
+ * bits of a^2; otherwise the scale is 1/rc^2 and hi the bits of 1.0 (the
+ * cutoff).  Built in FP64.  table receives 2 * ORC_LOG_ENTRIES floats (T, S).
  */
-#define ORC_LOG_BINADES 28
-#define ORC_LOG_LO ((uint32_t)(127 - ORC_LOG_BINADES) << 23)
-#define ORC_LOG_HI 0x3F800000u
+#define ORC_LOG_ENTRIES 4096
 
-int orc_sr_table_log_entries(void) { return (int)((ORC_LOG_HI >> 16) - (ORC_LOG_LO >> 16)) + 1; }
+int orc_sr_table_log_entries(void) { return ORC_LOG_ENTRIES; }
 
 static double orc_bits_to_double(uint32_t b) {
   float f;
@@ -641,45 +640,62 @@ static double orc_bits_to_double(uint32_t b) {
   return (double)f;
 }
 
+static uint32_t orc_float_bits(float f) {
+  uint32_t b;
+  memcpy(&b, &f, sizeof b);
+  return b;
+}
+
 static double orc_pair_factor_d(int cs, double s, double a, double e2) {
   const double G = 1.0 / (4.0 * kPi), r = sqrt(s);
   return (-G / (s + e2) + ref_force_d(cs, r, a)) / r;
 }
 
-int orc_sr_table_log(float rc, float a, float eps, int cs, float* table, float* inv_rc2) {
-  const float c = 1.0f / (rc * rc);
+int orc_sr_table_log(float rc, float a, float eps, int cs, float* table, float* scale, uint32_t* lo_bits) {
+  const uint32_t bin = 1u << 16;
+  float c;
+  uint32_t hi;
+  if (eps == 0.0f && a <= rc && a >= 1.0e-6f) {
+    c = 1.0f;
+    hi = (orc_float_bits(a * a) + bin - 1) / bin * bin;
+  } else {
+    c = 1.0f / (rc * rc);
+    hi = orc_float_bits(1.0f);
+  }
+  const uint32_t lo = hi - (uint32_t)(ORC_LOG_ENTRIES - 1) * bin;
   const double e2 = (double)eps * eps, cd = (double)c;
-  const uint32_t k0 = ORC_LOG_LO >> 16, k1 = ORC_LOG_HI >> 16;
-  for (uint32_t k = k0; k < k1; ++k) {
-    double s0 = orc_bits_to_double(k << 16), s1 = orc_bits_to_double((k + 1) << 16);
+  for (int i = 0; i + 1 < ORC_LOG_ENTRIES; ++i) {
+    uint32_t b0 = lo + (uint32_t)i * bin;
+    double s0 = orc_bits_to_double(b0), s1 = orc_bits_to_double(b0 + bin);
     int E;
     frexp(s0, &E);
     double unit = ldexp(1.0, E - 1);
     double f0 = orc_pair_factor_d(cs, s0 / cd, a, e2), f1 = orc_pair_factor_d(cs, s1 / cd, a, e2);
-    double lo = 0.0, hi = 0.0;
+    double dlo = 0.0, dhi = 0.0;
     for (int t = 1; t < 32; ++t) {
       double u = t / 32.0;
       double e = f0 + u * (f1 - f0) - orc_pair_factor_d(cs, (s0 + u * (s1 - s0)) / cd, a, e2);
-      if (e < lo) lo = e;
-      if (e > hi) hi = e;
+      if (e < dlo) dlo = e;
+      if (e > dhi) dhi = e;
     }
     double S = (f1 - f0) / (s1 - s0) * unit;
-    table[2 * (k - k0)] = (float)(f0 - S - 0.5 * (lo + hi));
-    table[2 * (k - k0) + 1] = (float)S;
+    table[2 * i] = (float)(f0 - S - 0.5 * (dlo + dhi));
+    table[2 * i + 1] = (float)S;
   }
-  table[2 * (k1 - k0)] = 0.0f;
-  table[2 * (k1 - k0) + 1] = 0.0f;
-  *inv_rc2 = c;
+  table[2 * (ORC_LOG_ENTRIES - 1)] = 0.0f;
+  table[2 * (ORC_LOG_ENTRIES - 1) + 1] = 0.0f;
+  *scale = c;
+  *lo_bits = lo;
   return 0;
 }
 
-static inline float orc_sr_table_log_eval(const float* table, float r2, float inv_rc2) {
-  float s = r2 * inv_rc2;
-  uint32_t b;
-  memcpy(&b, &s, sizeof b);
-  if (b < ORC_LOG_LO) b = ORC_LOG_LO;
-  if (b > ORC_LOG_HI) b = ORC_LOG_HI;
-  uint32_t k = (b >> 16) - (ORC_LOG_LO >> 16);
+static inline float orc_sr_table_log_eval(const float* table, float r2, float scale, uint32_t lo) {
+  const uint32_t hi = lo + (uint32_t)(ORC_LOG_ENTRIES - 1) * (1u << 16);
+  float s = r2 * scale;
+  uint32_t b = orc_float_bits(s);
+  if (b < lo) b = lo;
+  if (b > hi) b = hi;
+  uint32_t k = (b >> 16) - (lo >> 16);
   uint32_t yb = (b & 0xFFFFu) | 0x3F800000u;
   float y;
   memcpy(&y, &yb, sizeof y);
@@ -821,9 +837,10 @@ int orc_p3m_sr(int64_t n, const float* pos, const float* mass, const float box_o
   if (cell_mask) memcpy(active, cell_mask, (size_t)ncell);
   else memset(active, 1, (size_t)ncell);
   float* table = (float*)malloc(sizeof(float) * 2 * (size_t)orc_sr_table_log_entries());
-  float d2 = 0.0f, inv_rc2 = 0.0f;
+  float d2 = 0.0f, log_scale = 1.0f;
+  uint32_t log_lo = 0;
   if (!table) return -1;
-  if (use_table == 2) orc_sr_table_log(rc, a, eps, cs, table, &inv_rc2);
+  if (use_table == 2) orc_sr_table_log(rc, a, eps, cs, table, &log_scale, &log_lo);
   else orc_sr_table(rc, a, eps, cs, 500, table, &d2);
   const float rc2 = rc * rc;
   int64_t pairs = 0;
@@ -852,7 +869,7 @@ int orc_p3m_sr(int64_t n, const float* pos, const float* mass, const float box_o
           if (!use_table) {
             sr_force_direct(rij, mi, mj, cs, a, eps, f);
           } else if (use_table == 2) {
-            float F = mi * mj * orc_sr_table_log_eval(table, r2, inv_rc2);
+            float F = mi * mj * orc_sr_table_log_eval(table, r2, log_scale, log_lo);
             f[0] = F * rij[0];
             f[1] = F * rij[1];
             f[2] = F * rij[2];

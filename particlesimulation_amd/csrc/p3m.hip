@@ -113,8 +113,9 @@ struct SrParams {
   float inv_d2, rc2;
   float a, eps2;
   int cs;
-  float tmax_f;   // last index of the reference table; entries past the cutoff are zero
-  float inv_rc2;  // 1 / rc^2: the accurate table's argument is s = r^2 / rc^2
+  float tmax_f;         // last index of the reference table; entries past the cutoff are zero
+  float log_scale;      // the accurate table's argument is s = r^2 log_scale (MODE 3)
+  uint32_t log_lo, log_hi;  // float bits of its first bin and of its zero bin
 };
 
 // Pair factor F(r^2) such that a_i += m_j F (x_i - x_j).
@@ -122,18 +123,22 @@ struct SrParams {
 //   MODE 1: shortRangeForceFromTable (p3mMethod.cpp:240-245) on the
 //           reference's 500-entry table, zero-padded past the cutoff so the
 //           r < rc test is the table range itself;
-//   MODE 2: the accurate table of F itself over s = r^2 / rc^2, binned by the
-//           float bits of s: bits >> 16 selects one of 128 bins per binade
-//           (log-spaced, so the r^-3 and r^-1 singularities at small r are
-//           resolved to a fixed relative accuracy), and the low 16 bits, set
-//           into the mantissa of 1.0f, give y in [1, 1 + 2^-7) on which F is
-//           linear within the bin: F = T + S y.  The bits are clamped to
-//           [2^-28, 1]: pairs closer than rc 2^-14 take the value at that
-//           distance, and s >= 1 (and a NaN) lands on the zero entry.
-//           The LDS copy holds bin k at k mod SR_LOG_WRAP.
+//   MODE 2, 3: the accurate table of F itself over s = r^2 (MODE 2) or
+//           s = r^2 / rc^2 (MODE 3), binned by the float bits of s:
+//           bits >> 16 selects one of 128 bins per binade (log-spaced, so
+//           the r^-3 and r^-1 singularities at small r are resolved to a
+//           fixed relative accuracy), and the low 16 bits, set into the
+//           mantissa of 1.0f, give y in [1, 1 + 2^-7) on which F is linear
+//           within the bin: F = T + S y.  The bits are clamped to
+//           [log_lo, log_hi], 32 binades: closer pairs take the value of the
+//           first bin, and the last bin (and a NaN) is zero.  MODE 2 serves
+//           the unsoftened force with a <= rc, which vanishes for r >= a, so
+//           the zero bin can start at the bin boundary at or above a^2 with
+//           no scale; MODE 3 puts the cutoff s = 1 on a bin boundary with
+//           one multiply.  The LDS copy holds bin k at k mod SR_LOG_WRAP.
 template <int MODE>
 __device__ __forceinline__ float pair_factor(float r2, float mj, const float2* table,
-                                             const SrParams& P) {
+                                             const SrParams& P, uint32_t log_hi_v) {
   if constexpr (MODE == 0) {
     const float r = sqrtf(r2);
     const float R = reference_force(P.cs, r, P.a);
@@ -148,7 +153,9 @@ __device__ __forceinline__ float pair_factor(float r2, float mj, const float2* t
     const float2 tv = table[t];
     return fmaf(__builtin_amdgcn_fractf(ksi), tv.y, tv.x) * mj;
   } else {
-    const uint32_t b = min(max(__float_as_uint(r2 * P.inv_rc2), SR_LOG_LO), SR_LOG_HI);
+    const float s = MODE == 3 ? r2 * P.log_scale : r2;
+    // clamp in the float domain (s >= 0, where float order is bit order)
+    const uint32_t b = __float_as_uint(__builtin_amdgcn_fmed3f(s, __uint_as_float(P.log_lo), __uint_as_float(log_hi_v)));
     const float2 tv = table[(b >> SR_LOG_SHIFT) & (SR_LOG_WRAP - 1)];
     const float y = __uint_as_float((b & ((1u << SR_LOG_SHIFT) - 1u)) | 0x3F800000u);
     return fmaf(tv.y, y, tv.x) * mj;
@@ -183,31 +190,26 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
     float4* __restrict__ vel, float4* __restrict__ acc, int mode, float kick, float3 box,
     int32_t* __restrict__ counters, int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
-  // One LDS array (so the table starts at address 0): MODE 2 keeps its bins
-  // at k mod SR_LOG_WRAP and the partial sums (half of the waves' at a
-  // time) and work-item slot in the entries below and above them; MODE 1 has
-  // the table, then the rest.
+  // The partial sums of half of the waves at a time.
   constexpr int HALF = SR_WAVES / 2;
-  constexpr int PART_F2 = HALF * 64 * 3 / 2;  // float2 entries of HALF waves' float3 sums
-  constexpr uint32_t KLO = (SR_LOG_LO >> SR_LOG_SHIFT) % SR_LOG_WRAP;
-  constexpr uint32_t KHI = (SR_LOG_HI >> SR_LOG_SHIFT) % SR_LOG_WRAP;
-  static_assert(KLO < KHI && KLO >= PART_F2 && KHI + 1 < SR_LOG_WRAP,
-                "the wrapped table must leave room for the partial sums");
   static_assert(SR_WAVES % 2 == 0, "the partial sums are added in halves");
-  constexpr int PART = MODE == 2 ? 0 : (MODE == 1 ? SR_TABLE : 0);
-  constexpr int SLOT = MODE == 2 ? SR_LOG_WRAP - 1 : PART + PART_F2;
-  constexpr int LDS = MODE == 2 ? SR_LOG_WRAP : SLOT + 1;
-  __shared__ float2 lds[LDS];
-  float2* table = lds;
-  float3* part = reinterpret_cast<float3*>(lds + PART);
-  int& s_w = *reinterpret_cast<int*>(lds + SLOT);
+  constexpr int TS = MODE >= 2 ? SR_LOG_WRAP : (MODE == 1 ? SR_TABLE : 1);
+  __shared__ float2 table[TS];
+  __shared__ float3 part[HALF * 64];
+  __shared__ int s_w;
   if constexpr (MODE == 1)
     for (int i = threadIdx.x; i < SR_TABLE; i += SR_THREADS) table[i] = table_g[i];
-  if constexpr (MODE == 2)
-    for (int i = threadIdx.x; i < SR_TABLE_LOG; i += SR_THREADS) table[(KLO + i) % SR_LOG_WRAP] = table_g[i];
+  if constexpr (MODE >= 2) {
+    const uint32_t k0 = P.log_lo >> SR_LOG_SHIFT;
+    for (int i = threadIdx.x; i < SR_TABLE_LOG; i += SR_THREADS) table[(k0 + i) % SR_LOG_WRAP] = table_g[i];
+  }
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int total = chunk_start[nunit];
+  // the upper clamp bound in a VGPR, so the clamp is one v_med3_u32 (GFX9
+  // VALU instructions read at most one scalar operand)
+  uint32_t log_hi_v;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(log_hi_v) : "s"(P.log_hi));
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) s_w = next_item(counters + CNT_WORK, total, blockIdx.x & (NXCD - 1));
@@ -261,7 +263,7 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
             const float4 pj = sr_pm[j];  // uniform address: scalar load
             const float dx = pi.x - pj.x, dy_ = pi.y - pj.y, dz_ = pi.z - pj.z;
             const float r2 = fmaf(dz_, dz_, fmaf(dy_, dy_, dx * dx));
-            const float F = pair_factor<MODE>(r2, pj.w, table, P);
+            const float F = pair_factor<MODE>(r2, pj.w, table, P, log_hi_v);
             ax = fmaf(F, dx, ax);
             ay = fmaf(F, dy_, ay);
             az = fmaf(F, dz_, az);
@@ -457,41 +459,59 @@ void sr_table_host(float rc, float a, float eps, int cs, std::vector<float>& tab
 }
 
 // The accurate table of the direct formula's pair factor
-// F(s) = (-G/(r^2 + eps^2) + R(r)) / r, r^2 = s rc^2 (shortRangeForce,
-// p3mMethod.cpp:220-238), in the bit-binned layout pair_factor<2> reads:
-// entry k covers the floats s whose bits >> 16 equal k + (2^-24 bits >> 16);
-// within it s = s0 + (y - 1) 2^E (E the binade of s0), and the entry is the
-// chord of F through the bin's ends, written as T + S y and lowered by half
-// the sum of its largest deviations above and below F (so the error is
-// centred); the last entry (s >= 1) is zero.  Built in FP64.
+// F(s) = (-G/(r^2 + eps^2) + R(r)) / r (shortRangeForce,
+// p3mMethod.cpp:220-238) in the bit-binned layout pair_factor<2, 3> reads,
+// s = r^2 scale: entry i covers the floats s whose bits >> 16 equal
+// (lo >> 16) + i; within it s = s0 + (y - 1) 2^E (E the binade of s0), and
+// the entry is the chord of F through the bin's ends, written as T + S y and
+// lowered by half the sum of its largest deviations above and below F (so
+// the error is centred); the last entry is zero.  With no softening and
+// a <= rc (F = 0 for r >= a) the table runs over r^2 itself (scale 1, the
+// zero bin at the bin boundary at or above a^2); otherwise over r^2 / rc^2
+// (the zero bin at s = 1, the cutoff).  Built in FP64.
 static void sr_table_log_host(float rc, float a, float eps, int cs, std::vector<float>& table,
-                              float& inv_rc2) {
-  inv_rc2 = 1.0f / (rc * rc);
-  const double c = (double)inv_rc2, G = 1.0 / (4.0 * M_PI), e2 = (double)eps * eps;
-  auto F = [&](double sp) {
-    const double s = sp / c, r = std::sqrt(s);
-    return (-G / (s + e2) + ref_force_d(cs, r, a)) / r;
+                              float& scale, uint32_t& lo, uint32_t& hi, int& mode) {
+  auto to_bits = [](float f) {
+    uint32_t b;
+    std::memcpy(&b, &f, sizeof b);
+    return b;
   };
   auto from_bits = [](uint32_t b) {
     float f;
     std::memcpy(&f, &b, sizeof f);
     return (double)f;
   };
+  const uint32_t bin = 1u << SR_LOG_SHIFT;
+  if (eps == 0.0f && a <= rc && a >= 1.0e-6f) {
+    mode = 2;
+    scale = 1.0f;
+    hi = (to_bits(a * a) + bin - 1) / bin * bin;
+  } else {
+    mode = 3;
+    scale = 1.0f / (rc * rc);
+    hi = to_bits(1.0f);
+  }
+  lo = hi - (uint32_t)(SR_TABLE_LOG - 1) * bin;
+  const double c = (double)scale, G = 1.0 / (4.0 * M_PI), e2 = (double)eps * eps;
+  auto F = [&](double sp) {
+    const double s = sp / c, r = std::sqrt(s);
+    return (-G / (s + e2) + ref_force_d(cs, r, a)) / r;
+  };
   table.assign(2 * SR_TABLE_LOG, 0.0f);
-  const uint32_t k0 = SR_LOG_LO >> SR_LOG_SHIFT, k1 = SR_LOG_HI >> SR_LOG_SHIFT;
-  for (uint32_t k = k0; k < k1; ++k) {
-    const double s0 = from_bits(k << SR_LOG_SHIFT), s1 = from_bits((k + 1) << SR_LOG_SHIFT);
+  for (int i = 0; i + 1 < SR_TABLE_LOG; ++i) {
+    const uint32_t b0 = lo + (uint32_t)i * bin;
+    const double s0 = from_bits(b0), s1 = from_bits(b0 + bin);
     const double unit = std::ldexp(1.0, std::ilogb(s0));
     const double f0 = F(s0), f1 = F(s1);
-    double lo = 0.0, hi = 0.0;
+    double dlo = 0.0, dhi = 0.0;
     for (int t = 1; t < 32; ++t) {
       const double u = t / 32.0, e = f0 + u * (f1 - f0) - F(s0 + u * (s1 - s0));
-      lo = std::min(lo, e);
-      hi = std::max(hi, e);
+      dlo = std::min(dlo, e);
+      dhi = std::max(dhi, e);
     }
     const double S = (f1 - f0) / (s1 - s0) * unit;
-    table[2 * (k - k0)] = (float)(f0 - S - 0.5 * (lo + hi));
-    table[2 * (k - k0) + 1] = (float)S;
+    table[2 * i] = (float)(f0 - S - 0.5 * (dlo + dhi));
+    table[2 * i + 1] = (float)S;
   }
 }
 
@@ -500,7 +520,8 @@ void p3m_setup(pmgpu_ctx* c) {
   float d2 = 0.0f;
   if (c->p3m.use_table == 2) {
     sr_table_log_host(c->p3m.cutoff, c->p3m.particle_diameter, c->p3m.softening,
-                      c->p3m.cloud_shape, tab, c->sr_inv_rc2);
+                      c->p3m.cloud_shape, tab, c->sr_log_scale, c->sr_log_lo, c->sr_log_hi,
+                      c->sr_log_mode);
     d2 = 1.0f;
     c->sr_tmax = SR_TABLE_LOG - 1;
   } else {
@@ -534,7 +555,9 @@ static SrParams sr_params(pmgpu_ctx* c) {
   P.eps2 = c->p3m.softening * c->p3m.softening;
   P.cs = c->p3m.cloud_shape;
   P.tmax_f = (float)c->sr_tmax;
-  P.inv_rc2 = c->sr_inv_rc2;
+  P.log_scale = c->sr_log_scale;
+  P.log_lo = c->sr_log_lo;
+  P.log_hi = c->sr_log_hi;
   return P;
 }
 
@@ -577,7 +600,9 @@ void p3m_short_range(pmgpu_ctx* c, int mode, float kick, int iteration) {
   switch (c->p3m.use_table) {
     case 0: launch_sr<0>(c, mode, kick, iteration); break;
     case 1: launch_sr<1>(c, mode, kick, iteration); break;
-    default: launch_sr<2>(c, mode, kick, iteration);
+    default:
+      if (c->sr_log_mode == 2) launch_sr<2>(c, mode, kick, iteration);
+      else launch_sr<3>(c, mode, kick, iteration);
   }
   stage_end(c, tok);
 }

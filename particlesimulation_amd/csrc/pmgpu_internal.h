@@ -36,16 +36,13 @@ constexpr int INTERP_GATHER_BELOW = 256;
 constexpr int SR_CELL_DIV = 2, SR_CELL_DIV_MAX = 6;
 constexpr int SR_TABLE = 512;  // padded reference table (the reference uses 500 entries)
 constexpr int SR_TABLE_REF = 500;
-// The accurate pair table: bins of s = r^2 / rc^2 taken from the float bits
-// of s (2^(23 - SR_LOG_SHIFT) bins per binade) over SR_LOG_BINADES binades
-// below s = 1, plus a zero entry for s >= 1.  In LDS, bin k sits at
-// k mod SR_LOG_WRAP, so its byte address is a shift and a mask of the bits.
+// The accurate pair table: SR_LOG_WRAP bins of s = r^2 (times a scale)
+// taken from the float bits of s, 2^(23 - SR_LOG_SHIFT) bins per binade, the
+// last of them zero.  In LDS, bin k sits at k mod SR_LOG_WRAP, so its byte
+// address is a shift and a mask of the bits.
 constexpr int SR_LOG_SHIFT = 16;
-constexpr int SR_LOG_BINADES = 28;
 constexpr int SR_LOG_WRAP = 4096;
-constexpr uint32_t SR_LOG_LO = (uint32_t)(127 - SR_LOG_BINADES) << 23;  // bits of 2^-28
-constexpr uint32_t SR_LOG_HI = 0x3F800000u;                              // bits of 1.0f
-constexpr int SR_TABLE_LOG = (int)((SR_LOG_HI >> SR_LOG_SHIFT) - (SR_LOG_LO >> SR_LOG_SHIFT)) + 1;
+constexpr int SR_TABLE_LOG = SR_LOG_WRAP;
 
 struct Stage {
   const char* name;
@@ -142,7 +139,9 @@ struct pmgpu_ctx {
   float hcx = 0, hcy = 0, hcz = 0;
   float2* sr_table = nullptr;           // SR_TABLE_LOG entries (the larger of the two tables)
   float sr_inv_d2 = 0.0f;
-  float sr_inv_rc2 = 0.0f;               // 1 / rc^2, the accurate table's argument scale
+  float sr_log_scale = 1.0f;             // the accurate table's argument s = r^2 * scale
+  uint32_t sr_log_lo = 0, sr_log_hi = 0;  // float bits of its first and last (zero) bin
+  int sr_log_mode = 3;                    // 2: over r^2 (unsoftened, a <= rc), 3: over r^2 / rc^2
   int sr_tmax = 0;
   int32_t* sr_count = nullptr;          // ncell_sr + 1
   int32_t* sr_start = nullptr;          // ncell_sr + 1

@@ -440,28 +440,47 @@ def test_direct_sum_two_bodies(oracle):
     assert np.allclose(acc2[0, 0], -GC * 3 / 5)
 
 
-@pytest.mark.parametrize("cs,eps", [(0, 0.0), (1, 0.0), (1, 0.4)])
-def test_sr_log_table_follows_the_pair_factor(oracle, cs, eps):
+@pytest.mark.parametrize("cs,a,eps", [(0, 2.5, 0.0), (1, 2.5, 0.0), (1, 3.0, 0.0), (1, 2.5, 0.4), (0, 3.5, 0.0)])
+def test_sr_log_table_follows_the_pair_factor(oracle, cs, a, eps):
     # the bit-binned table against the closed-form pair factor in FP64: relative
-    # error bounded where F is not near a zero, exactly zero from the cutoff on,
-    # and the value at 2^-28 rc^2 for closer pairs
-    rc, a = 3.0, 2.5
-    tab, c = oracle.sr_table_log(rc, a, eps, cs)
-    assert tab.shape == (28 * 128 + 1, 2) and c == np.float32(1 / 9)
+    # error bounded where F is not near a zero, zero from the cutoff (or, with
+    # no softening and a <= rc, from r = a) on, and the value of the first bin
+    # for closer pairs
+    rc = 3.0
+    tab, c, lo = oracle.sr_table_log(rc, a, eps, cs)
+    assert tab.shape == (4096, 2)
+    unscaled = eps == 0.0 and a <= rc
+    assert c == (1.0 if unscaled else np.float32(1 / 9))
+    hi = lo + 4095 * 65536
+    if unscaled:  # the zero bin starts at the bin boundary at or above a^2
+        assert hi - 65536 < np.float32(a * a).view(np.uint32) <= hi
+    else:  # ... or at the cutoff s = 1
+        assert hi == np.float32(1.0).view(np.uint32)
+
     def exact(r):
         return (-GC / (r * r + eps * eps) + np.vectorize(lambda x: _ref_force64(cs, x, a))(r)) / r
 
     def lookup(r2):
         s = (np.float32(r2) * np.float32(c)).astype(np.float32)
-        b = np.clip(s.view(np.uint32), (127 - 28) << 23, 0x3F800000)
-        k = (b >> 16) - (((127 - 28) << 23) >> 16)
+        b = np.clip(s.view(np.uint32), lo, hi)
+        k = (b >> 16) - (lo >> 16)
         y = ((b & 0xFFFF) | 0x3F800000).astype(np.uint32).view(np.float32)
         return tab[k, 0] + tab[k, 1] * y
 
-    r = np.exp(_rng(15).uniform(np.log(1e-2), np.log(0.6 * a), 20000))
+    r = np.exp(_rng(15).uniform(np.log(1e-2), np.log(0.6 * min(a, rc)), 20000))
     r2 = (r * r).astype(np.float32)
     ref = exact(np.sqrt(r2.astype(np.float64)))
-    assert np.max(np.abs(lookup(r2) - ref) / np.abs(ref)) < 3e-5
-    assert np.all(lookup(np.float32([9.0, 9.5, 40.0, np.inf, np.nan])) == 0.0)
-    tiny = np.float32([1e-12, 1e-20, 0.0])
-    assert np.all(lookup(tiny) == lookup(np.float32([9.0 * 2.0 ** -28]))[0])
+    rr = np.sqrt(r2.astype(np.float64))
+    err = np.abs(lookup(r2) - ref)
+    # relative to F where the mesh part is small, and to the bare softened
+    # Newtonian factor where F itself tends to zero (r -> a)
+    assert np.max((err / np.abs(ref))[rr < 0.4 * min(a, rc)]) < 3e-5
+    assert np.max(err / (GC / (rr * (rr * rr + eps * eps)))) < 2e-5
+    far = np.float32([9.0, 9.5, 40.0, np.inf])
+    assert np.all(lookup(far) == 0.0)
+    if unscaled:  # F vanishes from r = a on
+        assert np.all(lookup(np.float32([a * a * 1.02, 0.5 * (a * a + 9.0)])) == 0.0)
+    elif a > rc:  # the force is cut at rc, not before
+        assert np.all(lookup(np.float32([8.5, 8.9])) != 0.0)
+    first = np.uint32(lo).view(np.float32) / np.float32(c)
+    assert np.all(lookup(np.float32([1e-30, 0.0])) == lookup(np.float32([first]))[0])
