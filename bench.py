@@ -30,7 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "particle-updates/s, P3M 1M on 128^3 at fixed force err; learned vs plain"
-PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector (with packed FP32)
+PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 vector peak
 PEAK_HBM_GBS = 8000.0
 SR_FLOP_PER_PAIR = 20      # per ordered pair within the cutoff
 GC = 1.0 / (4.0 * np.pi)
@@ -90,6 +90,17 @@ def force_error(P, w, pm, pos, m, n_targets=512, seed=7):
     e_rms = float(np.sqrt((diff ** 2).sum() / (ref ** 2).sum()))
     mean_rel = float(np.mean(np.linalg.norm(diff, axis=1) / (np.linalg.norm(ref, axis=1) + 1e-30)))
     return {"e_rms": e_rms, "mean_rel": mean_rel, "targets": int(len(t)), "reference": kind}
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch of the short-range kernel on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE measurement (profiles/), or None."""
+    path = os.path.join(ROOT, "profiles", f"k_sr_{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        d = json.load(f)
+    return d["traffic_bytes_per_launch"], f"profiles/k_sr_{workload}_traffic.json ({d['kernel']})"
 
 
 def cpu_baseline(w, pos, vel, m, pairs_total, budget_s=15.0):
@@ -207,14 +218,17 @@ def main():
                     "traffic": None, "avg_launch_ms": avg_s * 1e3, "ordered_pairs": pairs,
                     "tested_pairs": tested, "flop_per_pair": SR_FLOP_PER_PAIR,
                     "sr_table_mode": w.sr_table, "sr_cell_division": pm.get_tuning()[3]}
+        roofline["traffic"], roofline["traffic_source"] = measured_traffic(w.name)
     else:
         roofline = None
-    # algorithmic-byte rates of the bandwidth-bound stages (per launch)
+    # algorithmic-byte rates of the bandwidth-bound stages (per launch); the
+    # interpolation reads x (16 B) and writes a (16 B) per particle, and in a
+    # PM step also reads and writes v and writes x (48 B more)
     nx, ny, nz = w.grid
     M, K, n = nx * ny * nz, (nx // 2 + 1) * ny * nz, w.n
     alg = {"deposit": 16 * n + 4 * M, "fft_x_r2c": 4 * M + 8 * K, "fft_y_fwd": 16 * K,
            "fft_z_green": 20 * K, "fft_y_inv": 16 * K, "fft_x_c2r": 8 * K + 4 * M,
-           "interp": 4 * M + (16 + 16 + 16 + 16 + 16) * n}
+           "interp": 4 * M + (32 if w.p3m else 80) * n}
     hbm = {}
     for k, b in alg.items():
         if k in stages and stages[k][1]:
