@@ -197,6 +197,12 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
   __shared__ float2 table[TS];
   __shared__ float3 part[HALF * 64];
   __shared__ int s_w;
+  const int total = chunk_start[nunit];
+  // the first work item before the table: workgroups that find none (small
+  // particle sets leave most of the grid idle) exit without loading it
+  if (threadIdx.x == 0) s_w = next_item(counters + CNT_WORK, total, blockIdx.x & (NXCD - 1));
+  __syncthreads();
+  if (s_w < 0) return;
   if constexpr (MODE == 1)
     for (int i = threadIdx.x; i < SR_TABLE; i += SR_THREADS) table[i] = table_g[i];
   if constexpr (MODE >= 2) {
@@ -205,14 +211,13 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
   }
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int total = chunk_start[nunit];
-  // the upper clamp bound in a VGPR, so the clamp is one v_med3_u32 (GFX9
+  // the upper clamp bound in a VGPR, so the clamp is one v_med3_f32 (GFX9
   // VALU instructions read at most one scalar operand)
   uint32_t log_hi_v;
   asm volatile("v_mov_b32 %0, %1" : "=v"(log_hi_v) : "s"(P.log_hi));
-  for (;;) {
+  for (bool fetched = true;; fetched = false) {
     __syncthreads();
-    if (threadIdx.x == 0) s_w = next_item(counters + CNT_WORK, total, blockIdx.x & (NXCD - 1));
+    if (!fetched && threadIdx.x == 0) s_w = next_item(counters + CNT_WORK, total, blockIdx.x & (NXCD - 1));
     __syncthreads();
     const int w = s_w;
     if (w < 0) break;
