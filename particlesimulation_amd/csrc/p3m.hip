@@ -197,6 +197,10 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
   __shared__ float2 table[TS];
   __shared__ float3 part[HALF * 64];
   __shared__ int s_w;
+  // the j-ranges of the work item's (dz, dy) rows, filled in parallel
+  constexpr int MAXROWS = (2 * SR_CELL_DIV_MAX + 1) * (2 * SR_CELL_DIV_MAX + 1);
+  static_assert(MAXROWS <= SR_THREADS, "one thread per neighbourhood row");
+  __shared__ int2 rows[MAXROWS];
   const int total = chunk_start[nunit];
   // the first work item before the table: workgroups that find none (small
   // particle sets leave most of the grid idle) exit without loading it
@@ -247,19 +251,28 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
       cx1 = sr_cell_1d(sr_pm[min(i0 + SR_ICHUNK, re) - 1].x, P.ihx, P.mx);
     }
     const float4 pi = valid ? sr_pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    float ax = 0.f, ay = 0.f, az = 0.f;
-    int blk = 0;
-    for (int dz = -P.div; dz <= P.div; ++dz) {
-      const int zc = cz + dz;
-      if (zc < 0 || zc >= P.mz) continue;
-      for (int dy = -P.div; dy <= P.div; ++dy) {
-        const int yc = cy + dy;
-        if (yc < 0 || yc >= P.my) continue;
-        int kx;
-        if (!row_extent(P, dy, dz, &kx)) continue;
+    // row t = (dz + div) (2 div + 1) + dy + div: its j-range, trimmed to the
+    // x cells that can hold a pair (empty outside the cell grid)
+    const int side = 2 * P.div + 1;
+    if (threadIdx.x < side * side) {
+      const int dz = (int)threadIdx.x / side - P.div, dy = (int)threadIdx.x % side - P.div;
+      const int zc = cz + dz, yc = cy + dy;
+      int2 r = make_int2(0, 0);
+      int kx;
+      if (zc >= 0 && zc < P.mz && yc >= 0 && yc < P.my && row_extent(P, dy, dz, &kx)) {
         const int xlo = max(cx0 - kx, 0), xhi = min(cx1 + kx, P.mx - 1);
         const int row = P.mx * (yc + P.my * zc);
-        const int js = sr_start[row + xlo], je = sr_start[row + xhi + 1];
+        r = make_int2(sr_start[row + xlo], sr_start[row + xhi + 1]);
+      }
+      rows[threadIdx.x] = r;
+    }
+    __syncthreads();
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    int blk = 0;
+    for (int t = 0; t < side * side; ++t) {
+      {
+        const int2 r = rows[t];
+        const int js = __builtin_amdgcn_readfirstlane(r.x), je = __builtin_amdgcn_readfirstlane(r.y);
         for (int jb = js; jb < je; jb += 64, ++blk) {
           if (blk % SR_WAVES != wave) continue;
           const int jend = min(jb + 64, je);
