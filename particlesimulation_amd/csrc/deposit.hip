@@ -7,7 +7,7 @@
 // list is cut into work items of dep_chunk particles (tunable).  A work item sums
 // its contributions in LDS as 64-bit fixed point (integer adds are
 // associative, so the sum does not depend on thread or atomic order), then
-//  - a tile with one work item writes its 512 cells straight to the density
+//  - a tile with one work item writes its TILE_CELLS cells straight to the density
 //    grid (every cell is written exactly once: no clearing pass, no float
 //    atomics),
 //  - a tile with several work items adds its partial sums into a per-tile

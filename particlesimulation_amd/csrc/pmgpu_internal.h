@@ -19,9 +19,9 @@ namespace pmgpu {
 // particles of its 26 neighbours that reach into it as a few contiguous
 // class ranges.
 constexpr int TILE_X = 16;
-constexpr int TILE_Y = 8;
-constexpr int TILE_Z = 4;
-constexpr int TILE_CELLS = TILE_X * TILE_Y * TILE_Z;  // 512
+constexpr int TILE_Y = 16;
+constexpr int TILE_Z = 8;
+constexpr int TILE_CELLS = TILE_X * TILE_Y * TILE_Z;  // 2048
 constexpr int NCLASS = 27;
 // Work-item size of the deposit (virtual particles per work item).
 // work-item sizes (particles) of the deposit and interpolation passes: defaults
@@ -29,8 +29,9 @@ constexpr int NCLASS = 27;
 constexpr int DEPOSIT_CHUNK = 1024, DEPOSIT_CHUNK_MIN = 1024, DEPOSIT_CHUNK_MAX = 16384;
 constexpr int INTERP_CHUNK = 2048, INTERP_CHUNK_MIN = 256, INTERP_CHUNK_MAX = 16384;
 constexpr int BLOCKS_PER_CU = 16, BLOCKS_PER_CU_MAX = 32;
-// automatic interpolation variant: the gather variant below this many particles per tile
-constexpr int INTERP_GATHER_BELOW = 256;
+// automatic interpolation variant: the gather variant below this many
+// particles per tile (half a particle per cell)
+constexpr int INTERP_GATHER_BELOW = TILE_CELLS / 2;
 
 // P3M short range: cell side >= cutoff / div, stencil +-div cells (div tunable)
 constexpr int SR_CELL_DIV = 2, SR_CELL_DIV_MAX = 6;
