@@ -147,12 +147,20 @@ static void alloc_particle_arrays(pmgpu_ctx* c, int64_t n) {
   }
 }
 
-// The short-range cell grid: side >= cutoff / sr_div over the particle box.
+// The short-range cell grid over the particle box: y and z sides >= cutoff /
+// sr_div, x sides >= cutoff / (sr_div sr_xfine).  Finer x cells trim each
+// neighbour row closer to the cutoff sphere (a row is contiguous in memory,
+// so its length costs nothing but candidates).  sr_xfine is the largest of
+// 4, 2, 1 that keeps the grid within SR_CELLS_MAX cells.
 void sr_cells_setup(pmgpu_ctx* c) {
   const double side = (double)c->p3m.cutoff / c->sr_div * (1.0 + 1e-5);
   int m[3];
-  for (int i = 0; i < 3; ++i) m[i] = std::max(1, (int)std::floor((double)c->p3m.box[i] / side));
-  if ((int64_t)m[0] * m[1] * m[2] > 64ll * 1024 * 1024)
+  for (int i = 1; i < 3; ++i) m[i] = std::max(1, (int)std::floor((double)c->p3m.box[i] / side));
+  for (c->sr_xfine = 4;; c->sr_xfine /= 2) {
+    m[0] = std::max(1, (int)std::floor((double)c->p3m.box[0] / (side / c->sr_xfine)));
+    if ((int64_t)m[0] * m[1] * m[2] <= SR_CELLS_MAX || c->sr_xfine == 1) break;
+  }
+  if ((int64_t)m[0] * m[1] * m[2] > SR_CELLS_MAX)
     fail(PMGPU_ERR_UNSUPPORTED, "cutoff too small for the box (too many short-range cells)");
   c->mcx = m[0];
   c->mcy = m[1];

@@ -107,6 +107,7 @@ __device__ __forceinline__ float reference_force(int cs, float r, float a) {
 
 struct SrParams {
   int mx, my, mz, div, rowspan;
+  int divx;          // x reach in x cells: div times the x refinement
   float ihx;         // 1 / cell side in x, as the cell keys use it
   float hx, hy, hz;  // cell sides
   float rc2m;        // rc^2 (1 + 1e-4): row trimming bound with a rounding margin
@@ -171,7 +172,7 @@ __device__ __forceinline__ bool row_extent(const SrParams& P, int dy, int dz, in
   const float gy = (float)max(abs(dy) - 1, 0) * P.hy, gz = (float)max(abs(dz) - 1, 0) * P.hz;
   const float rem = P.rc2m - gy * gy - gz * gz;
   if (!(rem > 0.0f)) return false;
-  int k = P.div;
+  int k = P.divx;
   while (k > 1) {
     const float gx = (float)(k - 1) * P.hx;
     if (gx * gx < rem) break;
@@ -561,6 +562,7 @@ static SrParams sr_params(pmgpu_ctx* c) {
   P.my = c->mcy;
   P.mz = c->mcz;
   P.div = c->sr_div;
+  P.divx = c->sr_div * c->sr_xfine;
   P.rowspan = sr_rowspan(c);
   P.ihx = 1.0f / c->hcx;
   P.hx = c->hcx;

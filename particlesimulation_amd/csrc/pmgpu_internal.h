@@ -35,6 +35,7 @@ constexpr int INTERP_GATHER_BELOW = TILE_CELLS / 2;
 
 // P3M short range: cell side >= cutoff / div, stencil +-div cells (div tunable)
 constexpr int SR_CELL_DIV = 2, SR_CELL_DIV_MAX = 6;
+constexpr int64_t SR_CELLS_MAX = 128ll * 1024 * 1024;  // short-range cells, x refinement included
 constexpr int SR_TABLE = 512;  // padded reference table (the reference uses 500 entries)
 constexpr int SR_TABLE_REF = 500;
 // The accurate pair table: SR_LOG_WRAP bins of s = r^2 (times a scale)
@@ -111,6 +112,7 @@ struct pmgpu_ctx {
   int blocks_per_cu = pmgpu::BLOCKS_PER_CU;
   int sr_div = pmgpu::SR_CELL_DIV;
   bool sr_div_set = false;              // set explicitly (else chosen at p3m_configure)
+  int sr_xfine = 1;                     // x cells are this many times finer than y and z cells
   int interp_kernel = 0;                // 0 auto, 1 tiled (LDS halo), 2 gather from an E grid
   int sr_chunking = 0;                  // 0 auto (rows), 1 cells, 2 rows of cells
   float4* efield = nullptr;             // M, the gather variant's E grid
