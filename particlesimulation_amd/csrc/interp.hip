@@ -257,6 +257,12 @@ __global__ __launch_bounds__(256) void k_interp_gather(
     gy[a] = wrapc(ay + SLO + a, ny);
     gz[a] = wrapc(az + SLO + a, nz);
   }
+  // 32-bit cell offsets (the grid has at most 2^30 cells)
+  uint32_t rowoff[3][3];
+#pragma unroll
+  for (int b = 0; b < CNT; ++b)
+#pragma unroll
+    for (int c = 0; c < CNT; ++c) rowoff[b][c] = (uint32_t)nx * ((uint32_t)gy[b] + (uint32_t)ny * (uint32_t)gz[c]);
   float sx = 0.f, sy = 0.f, sz = 0.f;
 #pragma unroll
   for (int a = 0; a < CNT; ++a) {
@@ -268,7 +274,7 @@ __global__ __launch_bounds__(256) void k_interp_gather(
       for (int c = 0; c < CNT; ++c) {
         float t3 = t2 * wz[c];
         if constexpr (SCHEME == PMGPU_TSC) t3 *= 0.125f;
-        const float4 e = E[(int64_t)gx[a] + (int64_t)nx * (gy[b] + (int64_t)ny * gz[c])];
+        const float4 e = E[(uint32_t)gx[a] + rowoff[b][c]];
         sx += t3 * e.x;
         sy += t3 * e.y;
         sz += t3 * e.z;
