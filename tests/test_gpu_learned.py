@@ -214,7 +214,7 @@ def test_training_reduces_heldout_error(gpu_lib, oracle):
     assert abs(hist[0] - loss0) <= 1e-6 * loss0
     print("train loss", hist[::25])
     assert hist[-1] < 0.97 * hist[0]
-    # held-out snapshots, judged by the FP64 periodic (Ewald) direct sum
+    # held-out snapshots, measured against the FP64 periodic (Ewald) direct sum
     num0 = num1 = den = 0.0
     for s in (500, 501, 502, 503):
         pos, m, tgt = learned.snapshot(grid, n, s)
