@@ -14,7 +14,8 @@ The line also carries
   cpu_baseline  the CPU restatement of the reference (oracle/, OpenMP) timed
                 on a bounded sample of the same workload on this host,
   force_error   RMS and mean relative error of the GPU force against an FP64
-                direct sum on a seeded subsample.
+                direct sum on a seeded subsample,
+  learned_vs_plain  the recorded learned-vs-plain PM comparison (profiles/).
 """
 from __future__ import annotations
 
@@ -101,6 +102,22 @@ def measured_traffic(workload):
     with open(path) as f:
         d = json.load(f)
     return d["traffic_bytes_per_launch"], f"profiles/k_sr_{workload}_traffic.json ({d['kernel']})"
+
+
+def measured_learned():
+    """The learned-vs-plain comparison (error against the periodic direct sum
+    and throughput of plain PM, learned PM and P3M on the 4M-particle clumps
+    workload), from the committed measurement in profiles/, or None.  Training
+    takes tens of seconds, so the bench reports the recorded run."""
+    path = os.path.join(ROOT, "profiles", "learned_pm_clumps_4M_256.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    large = d["large"]
+    return {"source": f"profiles/learned_pm_clumps_4M_256.json ({d['command']})", "workload": large["workload"],
+            "targets": large["targets"], "reference": large["reference"],
+            **{k: large[k] for k in ("plain_pm", "learned_pm", "plain_p3m") if k in large}}
 
 
 def cpu_baseline(w, pos, vel, m, pairs_total, budget_s=15.0):
@@ -256,6 +273,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "force_error": err,
+        "learned_vs_plain": measured_learned(),
         "stage_ms_per_step": per_step,
         "hbm_stages": hbm,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
