@@ -160,13 +160,16 @@ def test_external_field_matches_oracle(gpu_lib, oracle):
 # P3M short range
 # --------------------------------------------------------------------------
 
-@pytest.mark.parametrize("cs,use_table,eps", [(1, 1, 0.0), (0, 1, 0.0), (1, 0, 0.0), (1, 1, 0.3), (1, 2, 0.0),
-                                               (0, 2, 0.3)])
-def test_p3m_matches_oracle(gpu_lib, oracle, cs, use_table, eps):
+@pytest.mark.parametrize("cs,use_table,eps,a", [(1, 1, 0.0, 3.0), (0, 1, 0.0, 3.0), (1, 0, 0.0, 3.0), (1, 1, 0.3, 3.0),
+                                                 (1, 2, 0.0, 3.0), (0, 2, 0.3, 3.0), (1, 2, 0.0, 2.2),
+                                                 (0, 2, 0.0, 3.6)])
+def test_p3m_matches_oracle(gpu_lib, oracle, cs, use_table, eps, a):
+    # use_table 2 bins r^2 itself when the force vanishes inside the cutoff
+    # (no softening, a <= rc) and r^2 / rc^2 otherwise (softened, or a > rc)
     grid = (64, 64, 64)
     H = 0.25
     pos, vel, m = _particles(6000, grid, 30 + cs, clustered=True)
-    a = rc = 3.0
+    rc = 3.0
     gf = 1 + cs
     pm = _make(gpu_lib, grid, 2, 0, gf, a=a)
     pm.set_particles(pos, vel, m)
