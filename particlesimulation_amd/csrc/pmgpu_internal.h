@@ -242,19 +242,23 @@ __device__ __forceinline__ int next_item(int32_t* cnt, int total, int xcd) {
 // atomics on one address.
 __device__ __forceinline__ int32_t wave_rank(int32_t* count, int32_t key, bool active) {
   const int lane = threadIdx.x & 63;
+  // group the lanes by key (one ballot per distinct key), then one atomic
+  // instruction for all groups at once: the leaders' atomics overlap
+  // instead of waiting for each other's return
   unsigned long long pending = __ballot(active);
-  int32_t rank = 0;
+  unsigned long long mine = 0;  // the lanes that share this lane's key
   while (pending) {
     const int leader = __ffsll((unsigned long long)pending) - 1;
-    const int32_t lk = __shfl(key, leader);
+    const int32_t lk = __builtin_amdgcn_readlane(key, leader);
     const unsigned long long same = __ballot(active && key == lk) & pending;
-    int32_t base = 0;
-    if (lane == leader) base = atomicAdd(&count[lk], (int32_t)__popcll(same));
-    base = __shfl(base, leader);
-    if (active && key == lk) rank = base + (int32_t)__popcll(same & ((1ull << lane) - 1ull));
+    if (active && key == lk) mine = same;
     pending &= ~same;
   }
-  return rank;
+  const int leader = active ? __ffsll((unsigned long long)mine) - 1 : lane;
+  int32_t base = 0;
+  if (active && lane == leader) base = atomicAdd(&count[key], (int32_t)__popcll(mine));
+  base = __shfl(base, leader);
+  return active ? base + (int32_t)__popcll(mine & ((1ull << lane) - 1ull)) : 0;
 }
 
 // escapedComputationalBox (source/pmMethod.cpp:146-156): inclusive [0, box];
