@@ -253,7 +253,8 @@ int pmgpu_batch_get_tuning(pmgpu_batch* batch, int32_t sim, int32_t* knobs);
    at the same side has at most 16M cells), knobs[4] the
    interpolation variant: 1 tiles staging the potential in LDS, 2 a gather
    from an E grid, 3 a gather straight from the potential (no E grid), 0
-   automatic (the gather below half a particle per cell, else the tiles);
+   automatic (the gather for grids of fewer than 4 tiles per CU, else the
+   tiles);
    all give the same result up to rounding; knobs[5] the short-range work items: 1 chunks
    of one cell's particles, 2 chunks of a row of cells' particles (fewer idle
    lanes in sparse cells), 0 automatic (rows).  The PM results do not depend

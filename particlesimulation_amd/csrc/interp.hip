@@ -108,10 +108,19 @@ __global__ __launch_bounds__(256) void k_interp(
     const int ox = (t % ntx) * TILE_X, oy = ((t / ntx) % nty) * TILE_Y,
               oz = (t / (ntx * nty)) * TILE_Z;
     const int px0 = ox + Gm::SLO - Gm::F, py0 = oy + Gm::SLO - Gm::F, pz0 = oz + Gm::SLO - Gm::F;
-    for (int e = threadIdx.x; e < Gm::PX * Gm::PY * Gm::PZ; e += 256) {
-      const int ix = e % Gm::PX, iy = (e / Gm::PX) % Gm::PY, iz = e / (Gm::PX * Gm::PY);
-      const int gx = wrapc(px0 + ix, nx), gy = wrapc(py0 + iy, ny), gz = wrapc(pz0 + iz, nz);
-      sphi[e] = phi[(int64_t)gx + (int64_t)nx * (gy + (int64_t)ny * gz)];
+    constexpr int NP = Gm::PX * Gm::PY * Gm::PZ, UNR = 16;
+    for (int e0 = threadIdx.x; e0 < NP; e0 += 256 * UNR) {
+      float v[UNR];
+#pragma unroll
+      for (int q = 0; q < UNR; ++q) {
+        const int e = e0 + 256 * q;
+        const int ix = e % Gm::PX, iy = (e / Gm::PX) % Gm::PY, iz = e / (Gm::PX * Gm::PY);
+        const uint32_t gx = wrapc(px0 + ix, nx), gy = wrapc(py0 + iy, ny), gz = wrapc(pz0 + iz, nz);
+        v[q] = e < NP ? phi[gx + (uint32_t)nx * (gy + (uint32_t)ny * gz)] : 0.0f;
+      }
+#pragma unroll
+      for (int q = 0; q < UNR; ++q)
+        if (e0 + 256 * q < NP) sphi[e0 + 256 * q] = v[q];
     }
     __syncthreads();
     for (int e = threadIdx.x; e < Gm::EX * Gm::EY * Gm::EZ; e += 256) {
@@ -512,7 +521,8 @@ void interpolate(pmgpu_ctx* c, int mode, float kick, int iteration) {
   }
   // auto: the gather variant when tiles are sparse (the tiled variant's
   // per-tile staging of the potential and field then dominates)
-  const bool gather = c->interp_kernel == 2 || (c->interp_kernel == 0 && c->n < (int64_t)INTERP_GATHER_BELOW * c->ntiles);
+  const bool gather = c->interp_kernel == 2 ||
+                      (c->interp_kernel == 0 && c->ntiles < INTERP_GATHER_TILES_PER_CU * c->num_cu);
   if (gather) {
     interpolate_gather(c, mode, kick, iteration);
     stage_end(c, tok);

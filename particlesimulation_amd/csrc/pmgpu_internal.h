@@ -29,9 +29,10 @@ constexpr int NCLASS = 27;
 constexpr int DEPOSIT_CHUNK = 1024, DEPOSIT_CHUNK_MIN = 1024, DEPOSIT_CHUNK_MAX = 16384;
 constexpr int INTERP_CHUNK = 2048, INTERP_CHUNK_MIN = 256, INTERP_CHUNK_MAX = 16384;
 constexpr int BLOCKS_PER_CU = 16, BLOCKS_PER_CU_MAX = 32;
-// automatic interpolation variant: the gather variant below this many
-// particles per tile (half a particle per cell)
-constexpr int INTERP_GATHER_BELOW = TILE_CELLS / 2;
+// automatic interpolation variant: the gather variant for grids of fewer
+// tiles than this many per CU (too few work items to fill the device), the
+// tiled variant otherwise
+constexpr int INTERP_GATHER_TILES_PER_CU = 4;
 
 // P3M short range: cell side >= cutoff / div, stencil +-div cells (div tunable)
 constexpr int SR_CELL_DIV = 2, SR_CELL_DIV_MAX = 6;
