@@ -189,9 +189,22 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit(
         else b = mid;
       }
     }
+    // the next particle's load is issued before this one's contributions
+    float4 pnext = make_float4(0.f, 0.f, 0.f, 0.f);
+    {
+      const int v = vbeg + (int)threadIdx.x;
+      if (v < vend) {
+        while (rpre[a + 1] <= v) ++a;
+        pnext = pm[rstart[a] + (v - rpre[a])];
+      }
+    }
     for (int v = vbeg + (int)threadIdx.x; v < vend; v += DEP_THREADS) {
-      while (rpre[a + 1] <= v) ++a;
-      const float4 p = pm[rstart[a] + (v - rpre[a])];
+      const float4 p = pnext;
+      const int vn = v + DEP_THREADS;
+      if (vn < vend) {
+        while (rpre[a + 1] <= vn) ++a;
+        pnext = pm[rstart[a] + (vn - rpre[a])];
+      }
       const int ax = anchor_of(p.x, SCHEME), ay = anchor_of(p.y, SCHEME), az = anchor_of(p.z, SCHEME);
       float wx[3], wy[3], wz[3];
       axis_weights(SCHEME, p.x - (float)ax, wx);
