@@ -243,9 +243,21 @@ __global__ __launch_bounds__(ColTile<L>::THREADS) void k_fft_y(float2* __restric
   const int c0 = tile * CT::TW;
   const int ncols = tile == ntile - 1 ? CT::TW + 1 : CT::TW;
   float2* base = spec + (int64_t)z * L * nxh + c0;
-  for (int e = threadIdx.x; e < L * ncols; e += CT::THREADS) {
-    const int n = e / ncols, col = e % ncols;
-    buf[n * CT::NC + col] = base[(int64_t)n * nxh + col];
+  // all of a thread's loads are issued before the first LDS store
+  constexpr int PER = (L * CT::NC + CT::THREADS - 1) / CT::THREADS;
+  float2 v[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = threadIdx.x + q * CT::THREADS;
+    if (e < L * ncols) {
+      const int n = e / ncols, col = e % ncols;
+      v[q] = base[(int64_t)n * nxh + col];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = threadIdx.x + q * CT::THREADS;
+    if (e < L * ncols) buf[(e / ncols) * CT::NC + e % ncols] = v[q];
   }
   __syncthreads();
   col_fft<L, INV>(buf, ncols, tw);
@@ -273,18 +285,39 @@ __global__ __launch_bounds__(ColTile<L>::THREADS) void k_fft_z(
   const int ncols = tile == ntile - 1 ? CT::TW + 1 : CT::TW;
   const int64_t zstride = (int64_t)ny * nxh;
   const int64_t off = (int64_t)y * nxh + c0;
-  for (int e = threadIdx.x; e < L * ncols; e += CT::THREADS) {
-    const int n = e / ncols, col = e % ncols;
-    float2 v = src[off + n * zstride + col];
-    if constexpr (MODE == Z_MUL_INV) v = scale2(v, greens[off + n * zstride + col]);
-    buf[n * CT::NC + col] = v;
+  // all of a thread's loads (and its G values) are issued before the first
+  // LDS store
+  constexpr int PER = (L * CT::NC + CT::THREADS - 1) / CT::THREADS;
+  float2 v[PER];
+  float g[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = threadIdx.x + q * CT::THREADS;
+    if (e < L * ncols) {
+      const int n = e / ncols, col = e % ncols;
+      v[q] = src[off + n * zstride + col];
+      if constexpr (MODE != Z_FWD) g[q] = greens[off + n * zstride + col];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = threadIdx.x + q * CT::THREADS;
+    if (e < L * ncols) {
+      float2 w = v[q];
+      if constexpr (MODE == Z_MUL_INV) w = scale2(w, g[q]);
+      buf[(e / ncols) * CT::NC + e % ncols] = w;
+    }
   }
   __syncthreads();
   if constexpr (MODE != Z_MUL_INV) col_fft<L, false>(buf, ncols, tw);
   if constexpr (MODE == Z_GREEN) {
-    for (int e = threadIdx.x; e < L * ncols; e += CT::THREADS) {
-      const int n = e / ncols, col2 = e % ncols;
-      buf[n * CT::NC + col2] = scale2(buf[n * CT::NC + col2], greens[off + n * zstride + col2]);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = threadIdx.x + q * CT::THREADS;
+      if (e < L * ncols) {
+        const int i = (e / ncols) * CT::NC + e % ncols;
+        buf[i] = scale2(buf[i], g[q]);
+      }
     }
     __syncthreads();
   }
