@@ -189,8 +189,18 @@ __global__ __launch_bounds__(256) void k_fft_x_c2r(const float2* __restrict__ sp
   __shared__ float2 buf[ROWS * P];
   const int row0 = blockIdx.x * ROWS;
   const float2* src = spec + (int64_t)row0 * (L + 1);
-  for (int e = threadIdx.x; e < ROWS * (L + 1); e += 256)
-    buf[(e / (L + 1)) * P + (e % (L + 1))] = src[e];
+  constexpr int PER = (ROWS * (L + 1) + 255) / 256;
+  float2 v[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = threadIdx.x + q * 256;
+    if (e < ROWS * (L + 1)) v[q] = src[e];
+  }
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = threadIdx.x + q * 256;
+    if (e < ROWS * (L + 1)) buf[(e / (L + 1)) * P + (e % (L + 1))] = v[q];
+  }
   __syncthreads();
   const int col = threadIdx.x / T, t = threadIdx.x % T;
   float2 z[8];
