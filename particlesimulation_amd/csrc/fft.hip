@@ -222,32 +222,35 @@ __global__ __launch_bounds__(256) void k_fft_x_c2r(const float2* __restrict__ sp
 }
 
 // Column tiles for the y and z passes: TW columns of kx per workgroup (rows
-// of TW * 8 bytes, 128 B for TW = 16), the last tile also takes the Nyquist
-// column kx = Nx/2, transformed after the others by the first L/8 threads.
-template <int L>
+// of TW * 8 bytes: 256 B for TW = 32, 128 B for 16), the last tile also
+// takes the Nyquist column kx = Nx/2, transformed after the others by the
+// first L/8 threads.  TW = 32 up to L = 256 when Nx/2 >= 32, 16 up to 512,
+// 8 for 1024: the LDS tile stays within 68 KB.
+template <int L, int TW_>
 struct ColTile {
   static constexpr int T = L / 8;
-  static constexpr int TW = L <= 512 ? 16 : 8;
+  static constexpr int TW = TW_;
   static constexpr int NC = TW + 1;
   static constexpr int THREADS = TW * T;
+  static_assert(THREADS <= 1024, "one workgroup per column tile");
 };
 
-template <int L, bool INV>
+template <int L, int TW, bool INV>
 __device__ __forceinline__ void col_fft(float2* buf, int ncols, const float2* tw) {
-  using CT = ColTile<L>;
+  using CT = ColTile<L, TW>;
   fft_lds<L, INV>(buf, 1, CT::NC, threadIdx.x % CT::TW, threadIdx.x / CT::TW, true, tw);
   if (ncols > CT::TW) fft_lds<L, INV>(buf, 1, CT::NC, CT::TW, threadIdx.x, threadIdx.x < CT::T, tw);
 }
 
 // y pass (forward or inverse), in place on the spectrum.
-template <int L, bool INV>
-__global__ __launch_bounds__(ColTile<L>::THREADS) void k_fft_y(float2* __restrict__ spec, int nxh,
+template <int L, int TW, bool INV>
+__global__ __launch_bounds__(TW * (L / 8)) void k_fft_y(float2* __restrict__ spec, int nxh,
                                                                int ntile,
                                                                const float2* __restrict__ tw,
                                                                const int32_t* __restrict__ escape,
                                                                int iteration) {
   if (escape[0] <= iteration) return;
-  using CT = ColTile<L>;
+  using CT = ColTile<L, TW>;
   __shared__ float2 buf[L * CT::NC];
   const int tile = blockIdx.x, z = blockIdx.y;
   const int c0 = tile * CT::TW;
@@ -270,7 +273,7 @@ __global__ __launch_bounds__(ColTile<L>::THREADS) void k_fft_y(float2* __restric
     if (e < L * ncols) buf[(e / ncols) * CT::NC + e % ncols] = v[q];
   }
   __syncthreads();
-  col_fft<L, INV>(buf, ncols, tw);
+  col_fft<L, TW, INV>(buf, ncols, tw);
   for (int e = threadIdx.x; e < L * ncols; e += CT::THREADS) {
     const int n = e / ncols, col2 = e % ncols;
     base[(int64_t)n * nxh + col2] = buf[n * CT::NC + col2];
@@ -283,12 +286,12 @@ __global__ __launch_bounds__(ColTile<L>::THREADS) void k_fft_y(float2* __restric
 // so a cached density spectrum can be reused with several G).
 constexpr int Z_GREEN = 0, Z_FWD = 1, Z_MUL_INV = 2;
 
-template <int L, int MODE>
-__global__ __launch_bounds__(ColTile<L>::THREADS) void k_fft_z(
+template <int L, int TW, int MODE>
+__global__ __launch_bounds__(TW * (L / 8)) void k_fft_z(
     const float2* __restrict__ src, float2* __restrict__ dst, const float* __restrict__ greens, int nxh,
     int ny, int ntile, const float2* __restrict__ tw, const int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
-  using CT = ColTile<L>;
+  using CT = ColTile<L, TW>;
   __shared__ float2 buf[L * CT::NC];
   const int tile = blockIdx.x, y = blockIdx.y;
   const int c0 = tile * CT::TW;
@@ -319,7 +322,7 @@ __global__ __launch_bounds__(ColTile<L>::THREADS) void k_fft_z(
     }
   }
   __syncthreads();
-  if constexpr (MODE != Z_MUL_INV) col_fft<L, false>(buf, ncols, tw);
+  if constexpr (MODE != Z_MUL_INV) col_fft<L, TW, false>(buf, ncols, tw);
   if constexpr (MODE == Z_GREEN) {
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -331,7 +334,7 @@ __global__ __launch_bounds__(ColTile<L>::THREADS) void k_fft_z(
     }
     __syncthreads();
   }
-  if constexpr (MODE != Z_FWD) col_fft<L, true>(buf, ncols, tw);
+  if constexpr (MODE != Z_FWD) col_fft<L, TW, true>(buf, ncols, tw);
   for (int e = threadIdx.x; e < L * ncols; e += CT::THREADS) {
     const int n = e / ncols, col2 = e % ncols;
     dst[off + n * zstride + col2] = buf[n * CT::NC + col2];
@@ -353,20 +356,39 @@ int rows_per_block(int L) {
   return 256 / (L / 8);
 }
 
+// the tile width for length L with Nx/2 columns (Nx >= 32, so 16 divides Nx/2)
+constexpr int tile_width(int L) { return L <= 512 ? 16 : 8; }
+
+template <int L, int TW, bool INV>
+void launch_y_tw(pmgpu_ctx* c, float2* spec) {
+  using CT = ColTile<L, TW>;
+  const int ntile = (c->nx / 2) / CT::TW;
+  hipLaunchKernelGGL((k_fft_y<L, TW, INV>), dim3(ntile, c->nz), dim3(CT::THREADS), 0, c->stream, spec,
+                     c->nxh, ntile, c->tw_y, c->escape, c->iter);
+}
+
 template <int L, bool INV>
 void launch_y(pmgpu_ctx* c, float2* spec) {
-  using CT = ColTile<L>;
+  if constexpr (L <= 256) {
+    if (c->nx / 2 >= 32) return launch_y_tw<L, 32, INV>(c, spec);
+  }
+  launch_y_tw<L, tile_width(L), INV>(c, spec);
+}
+
+template <int L, int TW, int MODE>
+void launch_z_tw(pmgpu_ctx* c, const float2* src, float2* dst, const float* g) {
+  using CT = ColTile<L, TW>;
   const int ntile = (c->nx / 2) / CT::TW;
-  hipLaunchKernelGGL((k_fft_y<L, INV>), dim3(ntile, c->nz), dim3(CT::THREADS), 0, c->stream, spec,
-                     c->nxh, ntile, c->tw_y, c->escape, c->iter);
+  hipLaunchKernelGGL((k_fft_z<L, TW, MODE>), dim3(ntile, c->ny), dim3(CT::THREADS), 0, c->stream, src, dst,
+                     g, c->nxh, c->ny, ntile, c->tw_z, c->escape, c->iter);
 }
 
 template <int L, int MODE>
 void launch_z(pmgpu_ctx* c, const float2* src, float2* dst, const float* g) {
-  using CT = ColTile<L>;
-  const int ntile = (c->nx / 2) / CT::TW;
-  hipLaunchKernelGGL((k_fft_z<L, MODE>), dim3(ntile, c->ny), dim3(CT::THREADS), 0, c->stream, src, dst,
-                     g, c->nxh, c->ny, ntile, c->tw_z, c->escape, c->iter);
+  if constexpr (L <= 256) {
+    if (c->nx / 2 >= 32) return launch_z_tw<L, 32, MODE>(c, src, dst, g);
+  }
+  launch_z_tw<L, tile_width(L), MODE>(c, src, dst, g);
 }
 
 template <bool INV>
