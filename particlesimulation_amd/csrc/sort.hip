@@ -50,13 +50,17 @@ __device__ int32_t block_exclusive_scan(int32_t v, int32_t* total) {
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int32_t* __restrict__ in,
                                                               int64_t n,
                                                               int32_t* __restrict__ sums) {
-  int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
-  int32_t s = 0;
+  // striped (coalesced) reads: the order of a sum does not matter
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x;
+  int32_t v[SCAN_ITEMS];
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    int64_t i = base + k;
-    if (i < n) s += in[i];
+    const int64_t i = base + (int64_t)k * SCAN_THREADS;
+    v[k] = i < n ? in[i] : 0;
   }
+  int32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) s += v[k];
   int32_t tot;
   block_exclusive_scan(s, &tot);
   if (threadIdx.x == 0) sums[blockIdx.x] = tot;
@@ -68,24 +72,45 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const int32_t* __re
                                                              int32_t* __restrict__ out,
                                                              int64_t n,
                                                              const int32_t* __restrict__ offsets) {
-  int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  // striped (coalesced) loads into LDS, one padding int per 16 so that a
+  // thread's 16 consecutive elements spread over the banks; each thread scans
+  // its 16, the block scans the thread totals, and the results leave striped
+  __shared__ int32_t tile[SCAN_TILE + SCAN_TILE / SCAN_ITEMS];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
   int32_t v[SCAN_ITEMS];
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int64_t i = base + (int64_t)k * SCAN_THREADS + threadIdx.x;
+    v[k] = i < n ? in[i] : 0;
+  }
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int e = k * SCAN_THREADS + threadIdx.x;
+    tile[e + e / SCAN_ITEMS] = v[k];
+  }
+  __syncthreads();
+  const int own = threadIdx.x * (SCAN_ITEMS + 1);
   int32_t s = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    int64_t i = base + k;
-    v[k] = i < n ? in[i] : 0;
+    v[k] = tile[own + k];
     s += v[k];
   }
   int32_t tot;
-  int32_t pre = block_exclusive_scan(s, &tot);
-  int32_t off = offsets ? offsets[blockIdx.x] : 0;
+  int32_t pre = block_exclusive_scan(s, &tot);  // synchronises the block
+  const int32_t off = offsets ? offsets[blockIdx.x] : 0;
   pre += off;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    int64_t i = base + k;
-    if (i < n) out[i] = pre;
+    tile[own + k] = pre;
     pre += v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int e = k * SCAN_THREADS + threadIdx.x;
+    const int64_t i = base + e;
+    if (i < n) out[i] = tile[e + e / SCAN_ITEMS];
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = off + tot;
 }
