@@ -91,16 +91,20 @@ __global__ void k_deposit_plan(int ntiles, int ntx, int nty, int ntz, int nrange
                                int32_t* __restrict__ counters, int64_t max_slots, int dep_chunk,
                                int int_chunk, const int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  // one wave per tile, its lanes over the ranges
+  const int t = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
   if (t >= ntiles) return;
-  int64_t total = 0;
+  long long total = 0;
   int32_t own = 0;
-  for (int r = 0; r < nranges; ++r) {
+  for (int r = lane; r < nranges; r += 64) {
     int32_t s, l;
     tile_range(r, t, ntx, nty, ntz, table, cls_start, &s, &l);
     if (r == 0) own = l;
     total += l;
   }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) total += __shfl_down(total, o, 64);
+  if (lane != 0) return;
   int nch = (int)((total + dep_chunk - 1) / dep_chunk);
   if (nch < 1) nch = 1;
   dep_nchunk[t] = nch;
@@ -315,7 +319,7 @@ void deposit(pmgpu_ctx* c) {
   const int nranges = 1 + c->dep_table_len;
   int tok = stage_begin(c, "deposit_plan");
   zero_fill_i32(c, c->counters, 4);
-  hipLaunchKernelGGL(k_deposit_plan, dim3((c->ntiles + 255) / 256), dim3(256), 0, c->stream,
+  hipLaunchKernelGGL(k_deposit_plan, dim3((unsigned)(((int64_t)c->ntiles * 64 + 255) / 256)), dim3(256), 0, c->stream,
                      c->ntiles, c->ntx, c->nty, c->ntz, nranges, c->dep_table, c->cls_start,
                      c->tile_nchunk, c->int_nchunk, c->tile_slot, c->slot_tile, c->counters,
                      c->fx_slots, c->dep_chunk, c->int_chunk, c->escape, c->iter);
