@@ -248,9 +248,10 @@ int pmgpu_batch_get_tuning(pmgpu_batch* batch, int32_t sim, int32_t* knobs);
    deposit and interpolation kernels (1..32, default 16), knobs[3] the
    short-range cell division: cells of y and z side >= cutoff / knobs[3]
    (x cells up to 4 times finer, as the cell count allows) and a
-   (2 knobs[3] + 1)^2 neighbourhood of rows (2..6; unless set here,
-   p3m_configure picks the largest whose y-z cell grid times the x cells
-   at the same side has at most 16M cells), knobs[4] the
+   (2 knobs[3] + 1)^2 neighbourhood of rows (1..6; unless set here,
+   p3m_configure picks the largest from 3 whose y-z cell grid times the x
+   cells at the same side has at most 16M cells, else 2, or 1 when even 2
+   would give more than 128M cells), knobs[4] the
    interpolation variant: 1 tiles staging the potential in LDS, 2 a gather
    from an E grid, 3 a gather straight from the potential (no E grid), 0
    automatic (the gather for grids of fewer than 4 tiles per CU, else the

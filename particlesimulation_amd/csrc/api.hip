@@ -461,11 +461,16 @@ int pmgpu_p3m_configure(pmgpu_ctx* c, const pmgpu_p3m_params* p) {
   if (!c->sr_div_set) {
     // the finest division whose cell grid stays modest (finer cells visit
     // fewer candidate pairs; the pair set is the same for every division)
-    c->sr_div = SR_CELL_DIV;
-    for (int d = SR_CELL_DIV_MAX; d > SR_CELL_DIV; --d) {
+    auto cells_at = [&](int d) {
       double cells = 1.0;
       for (int i = 0; i < 3; ++i) cells *= std::floor((double)p->box[i] / ((double)p->cutoff / d * (1.0 + 1e-5)));
-      if (cells <= 16.0 * 1024 * 1024) {
+      return cells;
+    };
+    // cells of side >= cutoff (the reference's chaining mesh) only when even
+    // half that side gives too many cells
+    c->sr_div = cells_at(2) <= (double)SR_CELLS_MAX ? 2 : SR_CELL_DIV;
+    for (int d = SR_CELL_DIV_MAX; d > 2; --d) {
+      if (cells_at(d) <= 16.0 * 1024 * 1024) {
         c->sr_div = d;
         break;
       }

@@ -35,7 +35,7 @@ constexpr int BLOCKS_PER_CU = 16, BLOCKS_PER_CU_MAX = 32;
 constexpr int INTERP_GATHER_TILES_PER_CU = 4;
 
 // P3M short range: cell side >= cutoff / div, stencil +-div cells (div tunable)
-constexpr int SR_CELL_DIV = 2, SR_CELL_DIV_MAX = 6;
+constexpr int SR_CELL_DIV = 1, SR_CELL_DIV_MAX = 6;
 constexpr int64_t SR_CELLS_MAX = 128ll * 1024 * 1024;  // short-range cells, x refinement included
 constexpr int SR_TABLE = 512;  // padded reference table (the reference uses 500 entries)
 constexpr int SR_TABLE_REF = 500;

@@ -189,7 +189,7 @@ def test_p3m_matches_oracle(gpu_lib, oracle, cs, use_table, eps, a):
     assert abs(pm.p3m_count_pairs() - pairs) <= 2 + 1e-5 * pairs
 
 
-@pytest.mark.parametrize("div,chunking", [(3, 2), (4, 1), (5, 2), (6, 1)])
+@pytest.mark.parametrize("div,chunking", [(1, 2), (3, 2), (4, 1), (5, 2), (6, 1)])
 def test_p3m_cell_division_keeps_the_pair_set(gpu_lib, oracle, div, chunking):
     # finer short-range cells with trimmed rows visit fewer candidates but must
     # accept exactly the same pairs and give the same force
@@ -212,7 +212,8 @@ def test_p3m_cell_division_keeps_the_pair_set(gpu_lib, oracle, div, chunking):
     pm.set_tuning(*k)
     pairs_d, tested_d = pm.p3m_count_pairs(tested=True)
     assert pairs_d == pairs2
-    assert tested_d < tested2
+    if div > 2:
+        assert tested_d < tested2
     pm.pm_step()
     pm.p3m_short_range()
     a_d = pm.get_particles()[2]
