@@ -401,12 +401,15 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr_count(
           const int jend = min(jb + 64, je);
           if (valid) tested += (unsigned long long)(jend - jb);
           issued += (unsigned long long)(jend - jb);
+          unsigned int in_block = 0;  // at most 64 per lane
+#pragma unroll 8
           for (int j = jb; j < jend; ++j) {
             const float4 pj = sr_pm[j];
             const float dx = pi.x - pj.x, dy_ = pi.y - pj.y, dz_ = pi.z - pj.z;
             const float r2 = fmaf(dz_, dz_, fmaf(dy_, dy_, dx * dx));
-            accepted += (valid && r2 < P.rc2 && r2 > 0.0f) ? 1ull : 0ull;
+            in_block += (r2 < P.rc2 && r2 > 0.0f) ? 1u : 0u;
           }
+          if (valid) accepted += in_block;
         }
       }
     }
