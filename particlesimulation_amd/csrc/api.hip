@@ -154,14 +154,16 @@ static void alloc_particle_arrays(pmgpu_ctx* c, int64_t n) {
 // 4, 2, 1 that keeps the grid within SR_CELLS_MAX cells.
 void sr_cells_setup(pmgpu_ctx* c) {
   const double side = (double)c->p3m.cutoff / c->sr_div * (1.0 + 1e-5);
-  int m[3];
-  for (int i = 1; i < 3; ++i) m[i] = std::max(1, (int)std::floor((double)c->p3m.box[i] / side));
+  // cell counts in FP64 first, so that a tiny cutoff cannot overflow an int
+  double d[3];
+  for (int i = 1; i < 3; ++i) d[i] = std::max(1.0, std::floor((double)c->p3m.box[i] / side));
   for (c->sr_xfine = 4;; c->sr_xfine /= 2) {
-    m[0] = std::max(1, (int)std::floor((double)c->p3m.box[0] / (side / c->sr_xfine)));
-    if ((int64_t)m[0] * m[1] * m[2] <= SR_CELLS_MAX || c->sr_xfine == 1) break;
+    d[0] = std::max(1.0, std::floor((double)c->p3m.box[0] / (side / c->sr_xfine)));
+    if (d[0] * d[1] * d[2] <= (double)SR_CELLS_MAX || c->sr_xfine == 1) break;
   }
-  if ((int64_t)m[0] * m[1] * m[2] > SR_CELLS_MAX)
+  if (d[0] * d[1] * d[2] > (double)SR_CELLS_MAX)
     fail(PMGPU_ERR_UNSUPPORTED, "cutoff too small for the box (too many short-range cells)");
+  const int m[3] = {(int)d[0], (int)d[1], (int)d[2]};
   c->mcx = m[0];
   c->mcy = m[1];
   c->mcz = m[2];

@@ -365,3 +365,14 @@ def test_full_size_p3m_properties(gpu_lib):
     assert np.max(np.abs(p)) < 1e-4 * np.abs(m[:, None] * a_pm).sum()
     pairs = pm.p3m_count_pairs()
     assert pairs > 1e9
+
+
+def test_p3m_rejects_a_cutoff_too_small_for_the_box(gpu_lib):
+    # the short-range cell count is checked before it is formed (no overflow)
+    P = gpu_lib
+    grid = (64, 64, 64)
+    pos, vel, m = _particles(1000, grid, 5)
+    pm = _make(P, grid, 2, 0, 2, a=3.0)
+    pm.set_particles(pos, vel, m)
+    with pytest.raises(P.PMGPUError, match="too many short-range cells"):
+        pm.p3m_configure(cutoff=1e-4, particle_diameter=3.0, cloud_shape=1)
