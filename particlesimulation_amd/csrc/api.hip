@@ -147,11 +147,19 @@ static void alloc_particle_arrays(pmgpu_ctx* c, int64_t n) {
   }
 }
 
+// Cells the short-range grid may use: about 128 per particle (every step
+// clears and scans them, which must stay small beside the pair work), at
+// most SR_CELLS_MAX; 16M before the particles are known.
+static double sr_cell_budget(const pmgpu_ctx* c) {
+  if (c->n <= 0) return 16.0 * 1024 * 1024;
+  return std::min((double)SR_CELLS_MAX, std::max(65536.0, 128.0 * (double)c->n));
+}
+
 // The short-range cell grid over the particle box: y and z sides >= cutoff /
 // sr_div, x sides >= cutoff / (sr_div sr_xfine).  Finer x cells trim each
 // neighbour row closer to the cutoff sphere (a row is contiguous in memory,
 // so its length costs nothing but candidates).  sr_xfine is the largest of
-// 4, 2, 1 that keeps the grid within SR_CELLS_MAX cells.
+// 4, 2, 1 that keeps the grid within the cell budget.
 void sr_cells_setup(pmgpu_ctx* c) {
   const double side = (double)c->p3m.cutoff / c->sr_div * (1.0 + 1e-5);
   // cell counts in FP64 first, so that a tiny cutoff cannot overflow an int
@@ -159,7 +167,7 @@ void sr_cells_setup(pmgpu_ctx* c) {
   for (int i = 1; i < 3; ++i) d[i] = std::max(1.0, std::floor((double)c->p3m.box[i] / side));
   for (c->sr_xfine = 4;; c->sr_xfine /= 2) {
     d[0] = std::max(1.0, std::floor((double)c->p3m.box[0] / (side / c->sr_xfine)));
-    if (d[0] * d[1] * d[2] <= (double)SR_CELLS_MAX || c->sr_xfine == 1) break;
+    if (d[0] * d[1] * d[2] <= sr_cell_budget(c) || c->sr_xfine == 1) break;
   }
   if (d[0] * d[1] * d[2] > (double)SR_CELLS_MAX)
     fail(PMGPU_ERR_UNSUPPORTED, "cutoff too small for the box (too many short-range cells)");
@@ -471,8 +479,9 @@ int pmgpu_p3m_configure(pmgpu_ctx* c, const pmgpu_p3m_params* p) {
     // cells of side >= cutoff (the reference's chaining mesh) only when even
     // half that side gives too many cells
     c->sr_div = cells_at(2) <= (double)SR_CELLS_MAX ? 2 : SR_CELL_DIV;
+    const double budget = std::min(16.0 * 1024 * 1024, sr_cell_budget(c));
     for (int d = SR_CELL_DIV_MAX; d > 2; --d) {
-      if (cells_at(d) <= 16.0 * 1024 * 1024) {
+      if (cells_at(d) <= budget) {
         c->sr_div = d;
         break;
       }
