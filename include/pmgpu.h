@@ -247,10 +247,12 @@ int pmgpu_batch_get_tuning(pmgpu_batch* batch, int32_t sim, int32_t* knobs);
    (256..16384, default 2048), knobs[2] workgroups per CU of the persistent
    deposit and interpolation kernels (1..32, default 16), knobs[3] the
    short-range cell division: cells of y and z side >= cutoff / knobs[3]
-   (x cells up to 4 times finer, as the cell count allows) and a
+   (x cells up to 4 times finer while the grid keeps within 128 cells per
+   particle) and a
    (2 knobs[3] + 1)^2 neighbourhood of rows (1..6; unless set here,
    p3m_configure picks the largest from 3 whose y-z cell grid times the x
-   cells at the same side has at most 16M cells, else 2, or 1 when even 2
+   cells at the same side has at most 16M cells (and at most 128 per
+   particle once the particles are set), else 2, or 1 when even 2
    would give more than 128M cells), knobs[4] the
    interpolation variant: 1 tiles staging the potential in LDS, 2 a gather
    from an E grid, 3 a gather straight from the potential (no E grid), 0
