@@ -130,11 +130,11 @@ struct SrParams {
 //           s = r^2 / rc^2 (MODE 3), binned by the float bits of s:
 //           bits >> 16 selects one of 128 bins per binade (log-spaced, so
 //           the r^-3 and r^-1 singularities at small r are resolved to a
-//           fixed relative accuracy), and the low 16 bits, set into the
-//           mantissa of 1.0f, give y in [1, 1 + 2^-7) on which F is linear
-//           within the bin: F = T + S y.  The bits are clamped to
-//           [log_lo, log_hi], 32 binades: closer pairs take the value of the
-//           first bin, and the last bin (and a NaN) is zero.  MODE 2 serves
+//           fixed relative accuracy), and F is linear in s within the bin:
+//           F = B + A s, one FMA on the clamped s.  s is clamped to the
+//           floats of bits [log_lo, log_hi], 32 binades: closer pairs take
+//           the value at the start of the first bin, and the last bin is
+//           zero (A = B = 0).  MODE 2 serves
 //           the unsoftened force with a <= rc, which vanishes for r >= a, so
 //           the zero bin can start at the bin boundary at or above a^2 with
 //           no scale; MODE 3 puts the cutoff s = 1 on a bin boundary with
@@ -158,10 +158,9 @@ __device__ __forceinline__ float pair_factor(float r2, float mj, const float2* t
   } else {
     const float s = MODE == 3 ? r2 * P.log_scale : r2;
     // clamp in the float domain (s >= 0, where float order is bit order)
-    const uint32_t b = __float_as_uint(__builtin_amdgcn_fmed3f(s, __uint_as_float(P.log_lo), __uint_as_float(log_hi_v)));
-    const float2 tv = table[(b >> SR_LOG_SHIFT) & (SR_LOG_WRAP - 1)];
-    const float y = __uint_as_float((b & ((1u << SR_LOG_SHIFT) - 1u)) | 0x3F800000u);
-    return fmaf(tv.y, y, tv.x) * mj;
+    const float sc = __builtin_amdgcn_fmed3f(s, __uint_as_float(P.log_lo), __uint_as_float(log_hi_v));
+    const float2 tv = table[(__float_as_uint(sc) >> SR_LOG_SHIFT) & (SR_LOG_WRAP - 1)];
+    return fmaf(tv.y, sc, tv.x) * mj;
   }
 }
 
@@ -486,10 +485,9 @@ void sr_table_host(float rc, float a, float eps, int cs, std::vector<float>& tab
 // F(s) = (-G/(r^2 + eps^2) + R(r)) / r (shortRangeForce,
 // p3mMethod.cpp:220-238) in the bit-binned layout pair_factor<2, 3> reads,
 // s = r^2 scale: entry i covers the floats s whose bits >> 16 equal
-// (lo >> 16) + i; within it s = s0 + (y - 1) 2^E (E the binade of s0), and
-// the entry is the chord of F through the bin's ends, written as T + S y and
-// lowered by half the sum of its largest deviations above and below F (so
-// the error is centred); the last entry is zero.  With no softening and
+// (lo >> 16) + i, [s0, s1), and holds the chord of F through the bin's ends
+// as (B, A), F = B + A s, lowered by half the sum of its largest deviations
+// above and below F (so the error is centred); the last entry is zero.  With no softening and
 // a <= rc (F = 0 for r >= a) the table runs over r^2 itself (scale 1, the
 // zero bin at the bin boundary at or above a^2); otherwise over r^2 / rc^2
 // (the zero bin at s = 1, the cutoff).  Built in FP64.
@@ -525,7 +523,6 @@ static void sr_table_log_host(float rc, float a, float eps, int cs, std::vector<
   for (int i = 0; i + 1 < SR_TABLE_LOG; ++i) {
     const uint32_t b0 = lo + (uint32_t)i * bin;
     const double s0 = from_bits(b0), s1 = from_bits(b0 + bin);
-    const double unit = std::ldexp(1.0, std::ilogb(s0));
     const double f0 = F(s0), f1 = F(s1);
     double dlo = 0.0, dhi = 0.0;
     for (int t = 1; t < 32; ++t) {
@@ -533,9 +530,9 @@ static void sr_table_log_host(float rc, float a, float eps, int cs, std::vector<
       dlo = std::min(dlo, e);
       dhi = std::max(dhi, e);
     }
-    const double S = (f1 - f0) / (s1 - s0) * unit;
-    table[2 * i] = (float)(f0 - S - 0.5 * (dlo + dhi));
-    table[2 * i + 1] = (float)S;
+    const double A = (f1 - f0) / (s1 - s0);
+    table[2 * i] = (float)(f0 - A * s0 - 0.5 * (dlo + dhi));
+    table[2 * i + 1] = (float)A;
   }
 }
 
