@@ -69,6 +69,7 @@ def lib():
         L.orc_reference_force.restype = C.c_float
         L.orc_sr_table.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, _f32p,
                                    C.POINTER(C.c_float)]
+        L.orc_sr_pair_factor.argtypes = [C.c_int, C.c_double, C.c_double, _f64p, C.c_int64, _f64p]
         L.orc_sr_table_log.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, _f32p, C.POINTER(C.c_float),
                                        C.POINTER(C.c_uint32)]
         L.orc_p3m_sr.argtypes = [C.c_int64, _f32p, _f32p, _f32p, C.c_float, C.c_float, C.c_float, C.c_float,
@@ -215,6 +216,15 @@ def sr_table(rc, a, eps, cs, cnt=500):
     d2 = C.c_float(0.0)
     lib().orc_sr_table(float(rc), float(a), float(eps), cs, cnt, t, C.byref(d2))
     return t, d2.value
+
+
+def sr_pair_factor(cs, a, eps, r2):
+    """FP64 pair factor F of the short-range force for an array of r^2:
+    a_i += m_j F (x_i - x_j) over 0 < r^2 < rc^2 (0 where r^2 <= 0)."""
+    r2 = np.ascontiguousarray(r2, np.float64)
+    out = np.empty_like(r2)
+    lib().orc_sr_pair_factor(int(cs), float(a), float(eps), r2, r2.size, out)
+    return out
 
 
 def sr_table_log(rc, a, eps, cs):

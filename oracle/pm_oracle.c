@@ -651,6 +651,14 @@ static double orc_pair_factor_d(int cs, double s, double a, double e2) {
   return (-G / (s + e2) + ref_force_d(cs, r, a)) / r;
 }
 
+/* The pair factor F(r^2) of shortRangeForce (p3mMethod.cpp:220-238) in
+   double, for n squared separations: a_i += m_j F (x_i - x_j) for
+   0 < r^2 < rc^2.  r2[k] <= 0 gives 0. */
+int orc_sr_pair_factor(int cs, double a, double eps, const double* r2, int64_t n, double* out) {
+  for (int64_t k = 0; k < n; ++k) out[k] = r2[k] > 0.0 ? orc_pair_factor_d(cs, r2[k], a, eps * eps) : 0.0;
+  return 0;
+}
+
 int orc_sr_table_log(float rc, float a, float eps, int cs, float* table, float* scale, uint32_t* lo_bits) {
   const uint32_t bin = 1u << 16;
   float c;

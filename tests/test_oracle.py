@@ -337,6 +337,28 @@ def test_reference_force_is_two_cloud_force(oracle, cs):
         assert abs(oracle.reference_force(cs, r, a) - _two_sphere_force(cs, r, a)) < 2e-5 * GC / a ** 2, r
 
 
+@pytest.mark.parametrize("cs", [0, 1])
+def test_sr_pair_factor_is_two_cloud_residual(oracle, cs):
+    # F r = -G/r^2 + R(r): the Newtonian force less the two-cloud force,
+    # found by integrating the clouds (not from the closed form); zero from
+    # r = a on without softening; the softened force away from the clouds
+    a = 2.0
+    r = np.array([0.3, 0.8, 1.2, 1.9, 2.0, 2.5, 3.1])
+    F = oracle.sr_pair_factor(cs, a, 0.0, r * r)
+    for k, x in enumerate(r):
+        want = -GC / x ** 2 + (_two_sphere_force(cs, x, a) if x < a else GC / x ** 2)
+        assert abs(F[k] * x - want) < 2e-5 * GC / a ** 2, x
+    assert np.all(F[r >= a] == 0.0) or np.max(np.abs(F[r >= a] * r[r >= a] ** 3)) < 1e-12 * GC
+    eps = 0.4
+    Fe = oracle.sr_pair_factor(cs, a, eps, r * r)
+    far = r >= a
+    assert np.allclose(Fe[far] * r[far], GC / r[far] ** 2 - GC / (r[far] ** 2 + eps ** 2), rtol=1e-12, atol=0)
+    # the Newtonian singularity at small r, and no self pair
+    tiny = np.array([1e-6, 1e-4]) * a
+    assert np.allclose(oracle.sr_pair_factor(cs, a, 0.0, tiny ** 2) * tiny ** 3, -GC, rtol=1e-3)
+    assert oracle.sr_pair_factor(cs, a, 0.0, np.zeros(1))[0] == 0.0
+
+
 def test_sr_table_vanishes_beyond_diameter(oracle):
     # with eps=0 and r >= a the reference force is Newtonian: T = 0
     rc, a = 3.0, 2.0
