@@ -155,18 +155,20 @@ static double sr_cell_budget(const pmgpu_ctx* c) {
   return std::min((double)SR_CELLS_MAX, std::max(65536.0, 128.0 * (double)c->n));
 }
 
-// The short-range cell grid over the particle box: y and z sides >= cutoff /
-// sr_div, x sides >= cutoff / (sr_div sr_xfine).  Finer x cells trim each
-// neighbour row closer to the cutoff sphere (a row is contiguous in memory,
-// so its length costs nothing but candidates).  sr_xfine is the largest of
-// 4, 2, 1 that keeps the grid within the cell budget.
+// The short-range cell grid over the particle box: y sides >= cutoff /
+// sr_divy, z sides >= cutoff / sr_divz, x sides >= cutoff / (sr_divx_base
+// sr_xfine).  Finer x cells trim each neighbour row closer to the cutoff
+// sphere (a row is contiguous in memory, so its length costs nothing but
+// candidates).  sr_xfine is the largest of 4, 2, 1 that keeps the grid
+// within the cell budget.
 void sr_cells_setup(pmgpu_ctx* c) {
-  const double side = (double)c->p3m.cutoff / c->sr_div * (1.0 + 1e-5);
+  const double rc = (double)c->p3m.cutoff * (1.0 + 1e-5);
   // cell counts in FP64 first, so that a tiny cutoff cannot overflow an int
   double d[3];
-  for (int i = 1; i < 3; ++i) d[i] = std::max(1.0, std::floor((double)c->p3m.box[i] / side));
+  d[1] = std::max(1.0, std::floor((double)c->p3m.box[1] / (rc / c->sr_divy)));
+  d[2] = std::max(1.0, std::floor((double)c->p3m.box[2] / (rc / c->sr_divz)));
   for (c->sr_xfine = 4;; c->sr_xfine /= 2) {
-    d[0] = std::max(1.0, std::floor((double)c->p3m.box[0] / (side / c->sr_xfine)));
+    d[0] = std::max(1.0, std::floor((double)c->p3m.box[0] / (rc / (c->sr_divx_base * c->sr_xfine))));
     if (d[0] * d[1] * d[2] <= sr_cell_budget(c) || c->sr_xfine == 1) break;
   }
   if (d[0] * d[1] * d[2] > (double)SR_CELLS_MAX)
@@ -185,6 +187,14 @@ void sr_cells_setup(pmgpu_ctx* c) {
   dalloc(&c->sr_chunk_start, c->ncell_sr + 1);
   ensure_scan_scratch(c, c->ncell_sr);
   if (c->n > 0) dalloc(&c->sr_owner, (int64_t)c->ncell_sr + c->n / 64 + 2);
+}
+
+void sr_set_division(pmgpu_ctx* c, int div) {
+  c->sr_div = c->sr_divy = c->sr_divz = c->sr_divx_base = div;
+  if (c->p3m_ready) {
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    sr_cells_setup(c);
+  }
 }
 
 static void set_shift(pmgpu_ctx* c) {
@@ -478,14 +488,17 @@ int pmgpu_p3m_configure(pmgpu_ctx* c, const pmgpu_p3m_params* p) {
     };
     // cells of side >= cutoff (the reference's chaining mesh) only when even
     // half that side gives too many cells
-    c->sr_div = cells_at(2) <= (double)SR_CELLS_MAX ? 2 : SR_CELL_DIV;
+    int div = cells_at(2) <= (double)SR_CELLS_MAX ? 2 : SR_CELL_DIV;
     const double budget = std::min(16.0 * 1024 * 1024, sr_cell_budget(c));
     for (int d = SR_CELL_DIV_MAX; d > 2; --d) {
       if (cells_at(d) <= budget) {
-        c->sr_div = d;
+        div = d;
         break;
       }
     }
+    c->sr_div = c->sr_divy = c->sr_divz = c->sr_divx_base = div;
+  } else {
+    c->sr_divy = c->sr_divz = c->sr_divx_base = c->sr_div;
   }
   sr_cells_setup(c);
   dalloc(&c->sr_table, SR_TABLE_LOG);  // the larger of the two tables
@@ -498,6 +511,7 @@ int pmgpu_p3m_configure(pmgpu_ctx* c, const pmgpu_p3m_params* p) {
     dalloc(&c->sr_idx, c->n);
   }
   p3m_setup(c);
+  if (!c->sr_div_set && c->n > 0) sr_choose_cells(c);
   PMGPU_API_END(c)
 }
 
@@ -657,13 +671,8 @@ int pmgpu_set_tuning(pmgpu_ctx* c, const int32_t* knobs) {
   c->int_chunk = itp;
   c->blocks_per_cu = bpc;
   c->sr_div_set = true;
-  if (div != c->sr_div) {
-    c->sr_div = div;
-    if (c->p3m_ready) {
-      HIP_CHECK(hipStreamSynchronize(c->stream));
-      sr_cells_setup(c);
-    }
-  }
+  if (div != c->sr_div || c->sr_divy != div || c->sr_divz != div || c->sr_divx_base != div)
+    sr_set_division(c, div);
   PMGPU_API_END(c)
 }
 

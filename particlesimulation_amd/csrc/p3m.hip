@@ -108,8 +108,9 @@ __device__ __forceinline__ float reference_force(int cs, float r, float a) {
 }
 
 struct SrParams {
-  int mx, my, mz, div, rowspan;
-  int divx;          // x reach in x cells: div times the x refinement
+  int mx, my, mz, rowspan;
+  int divy, divz;    // y and z reach in cells (cell sides >= cutoff / div)
+  int divx;          // x reach in x cells
   float ihx;         // 1 / cell side in x, as the cell keys use it
   float hx, hy, hz;  // cell sides
   float rc2m;        // rc^2 (1 + 1e-4): row trimming bound with a rounding margin
@@ -200,9 +201,8 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
   __shared__ float3 part[HALF * 64];
   __shared__ int s_w;
   // the j-ranges of the work item's (dz, dy) rows, filled in parallel
-  constexpr int MAXROWS = (2 * SR_CELL_DIV_MAX + 1) * (2 * SR_CELL_DIV_MAX + 1);
-  static_assert(MAXROWS <= SR_THREADS, "one thread per neighbourhood row");
-  __shared__ int2 rows[MAXROWS];
+  static_assert(SR_ROWS_MAX <= SR_THREADS, "one thread per neighbourhood row");
+  __shared__ int2 rows[SR_ROWS_MAX];
   const int total = chunk_start[nunit];
   // the first work item before the table: workgroups that find none (small
   // particle sets leave most of the grid idle) exit without loading it
@@ -253,11 +253,11 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
       cx1 = sr_cell_1d(sr_pm[min(i0 + SR_ICHUNK, re) - 1].x, P.ihx, P.mx);
     }
     const float4 pi = valid ? sr_pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    // row t = (dz + div) (2 div + 1) + dy + div: its j-range, trimmed to the
-    // x cells that can hold a pair (empty outside the cell grid)
-    const int side = 2 * P.div + 1;
-    if (threadIdx.x < side * side) {
-      const int dz = (int)threadIdx.x / side - P.div, dy = (int)threadIdx.x % side - P.div;
+    // row t = (dz + divz) (2 divy + 1) + dy + divy: its j-range, trimmed to
+    // the x cells that can hold a pair (empty outside the cell grid)
+    const int sidey = 2 * P.divy + 1, nrows = sidey * (2 * P.divz + 1);
+    if (threadIdx.x < nrows) {
+      const int dz = (int)threadIdx.x / sidey - P.divz, dy = (int)threadIdx.x % sidey - P.divy;
       const int zc = cz + dz, yc = cy + dy;
       int2 r = make_int2(0, 0);
       int kx;
@@ -271,7 +271,7 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
     __syncthreads();
     float ax = 0.f, ay = 0.f, az = 0.f;
     int blk = 0;
-    for (int t = 0; t < side * side; ++t) {
+    for (int t = 0; t < nrows; ++t) {
       {
         const int2 r = rows[t];
         const int js = __builtin_amdgcn_readfirstlane(r.x), je = __builtin_amdgcn_readfirstlane(r.y);
@@ -342,6 +342,9 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
 // Ordered pairs with 0 < r^2 < rc^2 (accepted), all (i, j) candidates the
 // pass visits (tested) and the lane-slots it issues for them (issued: 64 per
 // j, idle lanes of partly filled i chunks included), over k_sr's traversal.
+// PAIRS = false counts tested and issued only (no pair loop), to compare
+// cell geometries.
+template <bool PAIRS>
 __global__ __launch_bounds__(SR_THREADS) void k_sr_count(
     const float4* __restrict__ sr_pm, const int32_t* __restrict__ sr_start,
     const int32_t* __restrict__ chunk_start, const int32_t* __restrict__ owner, int nunit, SrParams P,
@@ -383,18 +386,26 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr_count(
       cx1 = sr_cell_1d(sr_pm[min(i0 + SR_ICHUNK, re) - 1].x, P.ihx, P.mx);
     }
     const float4 pi = valid ? sr_pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    int blk = 0;
-    for (int dz = -P.div; dz <= P.div; ++dz) {
+    int blk = 0, t = -1;
+    for (int dz = -P.divz; dz <= P.divz; ++dz) {
       const int zc = cz + dz;
-      if (zc < 0 || zc >= P.mz) continue;
-      for (int dy = -P.div; dy <= P.div; ++dy) {
+      for (int dy = -P.divy; dy <= P.divy; ++dy) {
+        ++t;
         const int yc = cy + dy;
-        if (yc < 0 || yc >= P.my) continue;
+        if (zc < 0 || zc >= P.mz || yc < 0 || yc >= P.my) continue;
         int kx;
         if (!row_extent(P, dy, dz, &kx)) continue;
         const int xlo = max(cx0 - kx, 0), xhi = min(cx1 + kx, P.mx - 1);
         const int row = P.mx * (yc + P.my * zc);
         const int js = sr_start[row + xlo], je = sr_start[row + xhi + 1];
+        if constexpr (!PAIRS) {
+          // one wave per row: the whole j-range, once
+          if (t % SR_WAVES == wave) {
+            if (valid) tested += (unsigned long long)(je - js);
+            issued += (unsigned long long)(je - js);
+          }
+          continue;
+        }
         for (int jb = js; jb < je; jb += 64, ++blk) {
           if (blk % SR_WAVES != wave) continue;
           const int jend = min(jb + 64, je);
@@ -563,8 +574,9 @@ static SrParams sr_params(pmgpu_ctx* c) {
   P.mx = c->mcx;
   P.my = c->mcy;
   P.mz = c->mcz;
-  P.div = c->sr_div;
-  P.divx = c->sr_div * c->sr_xfine;
+  P.divy = c->sr_divy;
+  P.divz = c->sr_divz;
+  P.divx = c->sr_divx_base * c->sr_xfine;
   P.rowspan = sr_rowspan(c);
   P.ihx = 1.0f / c->hcx;
   P.hx = c->hcx;
@@ -607,6 +619,8 @@ static void sr_sort(pmgpu_ctx* c, int iteration) {
 
 template <int MODE>
 static void launch_sr(pmgpu_ctx* c, int mode, float kick, int iteration) {
+  if ((2 * c->sr_divy + 1) * (2 * c->sr_divz + 1) > SR_ROWS_MAX)
+    fail(PMGPU_ERR_INVALID_ARGUMENT, "short-range neighbourhood has too many rows");
   const int grid = c->num_cu * 8;
   hipLaunchKernelGGL((k_sr<MODE>), dim3(grid), dim3(SR_THREADS), 0, c->stream, c->sr_pm,
                      c->sr_start, c->sr_chunk_start, c->sr_owner, sr_units(c), c->sr_idx, c->sr_table,
@@ -629,20 +643,65 @@ void p3m_short_range(pmgpu_ctx* c, int mode, float kick, int iteration) {
   stage_end(c, tok);
 }
 
-void p3m_count_pairs(pmgpu_ctx* c, int64_t* accepted, int64_t* tested, int64_t* issued) {
+static void sr_count(pmgpu_ctx* c, bool pairs, unsigned long long h[3]) {
   const int it = -2147483647 - 1;
   sr_sort(c, it);
   zero_fill_i32(c, (int32_t*)c->pair_count, 6);
-  hipLaunchKernelGGL(k_sr_count, dim3(c->num_cu * 8), dim3(SR_THREADS), 0, c->stream, c->sr_pm,
-                     c->sr_start, c->sr_chunk_start, c->sr_owner, sr_units(c), sr_params(c), c->counters,
-                     c->pair_count);
+  if (pairs)
+    hipLaunchKernelGGL(k_sr_count<true>, dim3(c->num_cu * 8), dim3(SR_THREADS), 0, c->stream, c->sr_pm,
+                       c->sr_start, c->sr_chunk_start, c->sr_owner, sr_units(c), sr_params(c), c->counters,
+                       c->pair_count);
+  else
+    hipLaunchKernelGGL(k_sr_count<false>, dim3(c->num_cu * 8), dim3(SR_THREADS), 0, c->stream, c->sr_pm,
+                       c->sr_start, c->sr_chunk_start, c->sr_owner, sr_units(c), sr_params(c), c->counters,
+                       c->pair_count);
   HIP_CHECK(hipGetLastError());
-  unsigned long long h[3] = {0, 0, 0};
-  HIP_CHECK(hipMemcpyAsync(h, c->pair_count, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipMemcpyAsync(h, c->pair_count, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+void p3m_count_pairs(pmgpu_ctx* c, int64_t* accepted, int64_t* tested, int64_t* issued) {
+  unsigned long long h[3] = {0, 0, 0};
+  sr_count(c, true, h);
   *accepted = (int64_t)h[0];
   if (tested) *tested = (int64_t)h[1];
   if (issued) *issued = (int64_t)h[2];
+}
+
+// Anisotropic cells for anisotropic particle sets: in a thin disk, cells one
+// cutoff deep across the disk and finer in the plane visit fewer candidates
+// than cubes (the candidates of a pair are the cells within the cutoff of
+// the i cell, so the excess over the cutoff sphere grows with the cell sides
+// along which the particles extend).  Each geometry -- the isotropic
+// division d, and (4d, 1), (2d, 1), (2d, 2) with y and z either way round,
+// x sides as for d -- is sorted and scored by the candidates it tests plus
+// 16 per cell (the sort's cost per cell, in candidate-pair times); the
+// lowest score wins.  Every geometry visits the same pair set.
+void sr_choose_cells(pmgpu_ctx* c) {
+  const int d = c->sr_div;
+  const int cand[7][2] = {{d, d}, {4 * d, 1}, {1, 4 * d}, {2 * d, 1}, {1, 2 * d}, {2 * d, 2}, {2, 2 * d}};
+  int best = 0;
+  double best_score = 0.0;
+  for (int k = 0; k < 7; ++k) {
+    const int dy = cand[k][0], dz = cand[k][1];
+    if ((2 * dy + 1) * (2 * dz + 1) > SR_ROWS_MAX) continue;
+    if (k > 0 && dy == cand[0][0] && dz == cand[0][1]) continue;
+    c->sr_divy = dy;
+    c->sr_divz = dz;
+    c->sr_divx_base = d;
+    sr_cells_setup(c);
+    unsigned long long h[3] = {0, 0, 0};
+    sr_count(c, false, h);
+    const double score = (double)h[1] + 16.0 * (double)c->ncell_sr;
+    if (k == 0 || score < best_score) {
+      best = k;
+      best_score = score;
+    }
+  }
+  c->sr_divy = cand[best][0];
+  c->sr_divz = cand[best][1];
+  c->sr_divx_base = d;
+  sr_cells_setup(c);
 }
 
 }  // namespace pmgpu

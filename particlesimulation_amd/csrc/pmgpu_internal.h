@@ -36,6 +36,9 @@ constexpr int INTERP_GATHER_TILES_PER_CU = 4;
 
 // P3M short range: cell side >= cutoff / div, stencil +-div cells (div tunable)
 constexpr int SR_CELL_DIV = 1, SR_CELL_DIV_MAX = 6;
+// neighbourhood rows (2 divy + 1)(2 divz + 1) of any cell geometry: the
+// isotropic maximum, which also bounds the anisotropic geometries tried
+constexpr int SR_ROWS_MAX = (2 * SR_CELL_DIV_MAX + 1) * (2 * SR_CELL_DIV_MAX + 1);
 constexpr int64_t SR_CELLS_MAX = 128ll * 1024 * 1024;  // short-range cells, x refinement included
 constexpr int SR_TABLE = 512;  // padded reference table (the reference uses 500 entries)
 constexpr int SR_TABLE_REF = 500;
@@ -111,9 +114,13 @@ struct pmgpu_ctx {
   int dep_chunk = pmgpu::DEPOSIT_CHUNK;  // tunables (pmgpu_set_tuning)
   int int_chunk = pmgpu::INTERP_CHUNK;
   int blocks_per_cu = pmgpu::BLOCKS_PER_CU;
-  int sr_div = pmgpu::SR_CELL_DIV;
+  int sr_div = pmgpu::SR_CELL_DIV;      // the division knob (pmgpu_set_tuning knobs[3])
   bool sr_div_set = false;              // set explicitly (else chosen at p3m_configure)
-  int sr_xfine = 1;                     // x cells are this many times finer than y and z cells
+  // the short-range cell geometry: cell sides >= cutoff / sr_divy in y,
+  // cutoff / sr_divz in z and cutoff / (sr_divx_base sr_xfine) in x; all three
+  // equal sr_div unless p3m_configure picked anisotropic cells for the particles
+  int sr_divy = pmgpu::SR_CELL_DIV, sr_divz = pmgpu::SR_CELL_DIV, sr_divx_base = pmgpu::SR_CELL_DIV;
+  int sr_xfine = 1;                     // x cells are this many times finer than sr_divx_base gives
   int interp_kernel = 0;                // 0 auto, 1 tiled (LDS halo), 2 gather from an E grid
   int sr_chunking = 0;                  // 0 auto (rows), 1 cells, 2 rows of cells
   float4* efield = nullptr;             // M, the gather variant's E grid
@@ -201,7 +208,9 @@ void interpolate(pmgpu_ctx* c, int mode, float kick, int iteration);
 void field_grid(pmgpu_ctx* c);
 void leapfrog_op(pmgpu_ctx* c, int op, float dt, int iteration);
 void p3m_setup(pmgpu_ctx* c);
-void sr_cells_setup(pmgpu_ctx* c);  // short-range cell grid for c->p3m and c->sr_div
+void sr_cells_setup(pmgpu_ctx* c);  // short-range cell grid for c->p3m and the divisions
+void sr_set_division(pmgpu_ctx* c, int div);  // isotropic division, cells rebuilt if configured
+void sr_choose_cells(pmgpu_ctx* c);           // the cell geometry with the fewest candidates
 void p3m_short_range(pmgpu_ctx* c, int mode, float kick, int iteration);
 void p3m_count_pairs(pmgpu_ctx* c, int64_t* accepted, int64_t* tested, int64_t* issued);
 // learned correction (mlp.hip)

@@ -224,6 +224,45 @@ def test_p3m_cell_division_keeps_the_pair_set(gpu_lib, oracle, div, chunking):
     assert _rel(a_d, oracle.pm_accel(pos, m, grid, 2, 0, G) + ref_sr) <= FORCE_TOL
 
 
+def test_p3m_anisotropic_cells_for_a_thin_disk(gpu_lib, oracle):
+    # with the division left to p3m_configure, a thin disk gets cells finer in
+    # the disk's plane and one cutoff deep across it: fewer candidates than
+    # cubes of the same division, the same pair set and the same force
+    grid = (64, 64, 64)
+    r = _rng(33)
+    n = 20000
+    rad = r.exponential(5.0, n)
+    th = r.uniform(0.0, 2 * np.pi, n)
+    pos = np.stack([32 + rad * np.cos(th), 32 + rad * np.sin(th), 32 + 0.3 * r.standard_normal(n)], 1)
+    pos = np.clip(pos, 2.0, 62.0).astype(np.float32)
+    vel = np.zeros_like(pos)
+    m = r.uniform(0.5, 1.5, n).astype(np.float32) / n
+    out = []
+    for explicit in (False, True):
+        pm = _make(gpu_lib, grid, 2, 0, 2, a=3.0)
+        pm.set_particles(pos, vel, m)
+        pm.init_greens()
+        if explicit:
+            k = list(pm.get_tuning())
+            k[3] = out[0][0]
+            pm.set_tuning(*k)
+        pm.p3m_configure(cutoff=3.0, particle_diameter=3.0, cloud_shape=1, use_table=2)
+        pairs, tested = pm.p3m_count_pairs(tested=True)
+        pm.pm_step()
+        pm.p3m_short_range()
+        out.append((pm.get_tuning()[3], pairs, tested, pm.get_particles()[2]))
+    (d, pairs_a, tested_a, a_auto), (d2, pairs_i, tested_i, a_iso) = out
+    assert d == d2 and pairs_a == pairs_i
+    assert tested_a < 0.95 * tested_i
+    # the cells order each i's j-stream differently: FP32 rounding only
+    assert _rel(a_auto, a_iso) <= 1e-5
+    G = oracle.green_grid(2, 2, 0, 3.0, grid)
+    H = 0.25
+    ref_sr, _ = oracle.p3m_short_range(pos, m, np.array(grid, np.float32) * H, 3 * H, H, 3.0, 3.0, 0.0, 1,
+                                       use_table=2)
+    assert _rel(a_auto, oracle.pm_accel(pos, m, grid, 2, 0, G) + ref_sr) <= FORCE_TOL
+
+
 def test_p3m_short_range_conserves_momentum_at_scale(gpu_lib):
     grid = (128, 128, 128)
     pos, vel, m = _particles(200000, grid, 40, clustered=True)
