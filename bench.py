@@ -234,7 +234,8 @@ def main():
                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                     "traffic": None, "avg_launch_ms": avg_s * 1e3, "ordered_pairs": pairs,
                     "tested_pairs": tested, "flop_per_pair": SR_FLOP_PER_PAIR,
-                    "sr_table_mode": w.sr_table, "sr_cell_division": pm.get_tuning()[3]}
+                    "sr_table_mode": w.sr_table, "sr_cell_division": pm.get_tuning()[3],
+                    "sr_cells": dict(zip(("div_x", "div_y", "div_z", "cells"), pm.p3m_cell_geometry()))}
         roofline["traffic"], roofline["traffic_source"] = measured_traffic(w.name)
     else:
         roofline = None

@@ -253,7 +253,8 @@ int pmgpu_batch_get_tuning(pmgpu_batch* batch, int32_t sim, int32_t* knobs);
    p3m_configure picks the largest from 3 whose y-z cell grid times the x
    cells at the same side has at most 16M cells (and at most 128 per
    particle once the particles are set), else 2, or 1 when even 2
-   would give more than 128M cells), knobs[4] the
+   would give more than 128M cells, and with the particles set may then
+   shape the cells to them: pmgpu_p3m_cell_geometry), knobs[4] the
    interpolation variant: 1 tiles staging the potential in LDS, 2 a gather
    from an E grid, 3 a gather straight from the potential (no E grid), 0
    automatic (the gather for grids of fewer than 4 tiles per CU, else the
@@ -283,6 +284,15 @@ int pmgpu_p3m_count_pairs(pmgpu_ctx* ctx, int64_t* pairs, int64_t* tested);
 /* stats[0] accepted pairs, stats[1] tested pairs, stats[2] lane-slots issued
    (64 per visited j, idle lanes of partly filled i chunks included). */
 int pmgpu_p3m_pair_stats(pmgpu_ctx* ctx, int64_t* stats);
+/* No counterpart in the reference.  The short-range cell geometry in use:
+   geo[0..2] the x, y and z divisions (cell sides >= cutoff / geo[a]; the
+   neighbourhood reaches geo[a] cells each way), geo[3] the number of cells.
+   Unless knobs[3] of pmgpu_set_tuning fixes an isotropic division,
+   pmgpu_p3m_configure (with particles set) picks, among the isotropic
+   division d and cells d times 4 or 2 finer in y or z and one cutoff (or
+   half of one) deep in the other, the geometry whose candidate pairs plus
+   16 per cell are fewest; all visit the same pair set. */
+int pmgpu_p3m_cell_geometry(pmgpu_ctx* ctx, int32_t* geo);
 
 /* ---- learned correction: standalone MLP evaluation ------------------- */
 /* No counterpart in the reference.  inputs is n x 4 (host); out[m] = y(m).

@@ -52,6 +52,7 @@ EXPORTS = (
     "pmgpu_greens_value", "pmgpu_diagnostics", "pmgpu_mlp_forward", "pmgpu_mlp_backward",
     "pmgpu_trainer_create", "pmgpu_trainer_destroy", "pmgpu_trainer_last_error", "pmgpu_trainer_add_snapshot",
     "pmgpu_trainer_loss_grad", "pmgpu_train", "pmgpu_direct_sum", "pmgpu_set_tuning", "pmgpu_get_tuning", "pmgpu_p3m_pair_stats",
+    "pmgpu_p3m_cell_geometry",
     "pmgpu_batch_create", "pmgpu_batch_destroy", "pmgpu_batch_last_error", "pmgpu_batch_size",
     "pmgpu_batch_set_particles", "pmgpu_batch_prepare", "pmgpu_batch_step", "pmgpu_batch_synchronize",
     "pmgpu_batch_time", "pmgpu_batch_member_latency", "pmgpu_batch_get_particles", "pmgpu_batch_get_tuning",
@@ -135,6 +136,7 @@ def load_library(path: str = LIB_PATH):
                                      C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
     L.pmgpu_p3m_count_pairs.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.pmgpu_p3m_pair_stats.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.pmgpu_p3m_cell_geometry.argtypes = [vp, C.POINTER(C.c_int32)]
     L.pmgpu_greens_value.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_double] + [C.c_int32] * 6
     L.pmgpu_greens_value.restype = C.c_double
     L.pmgpu_diagnostics.argtypes = [vp, C.POINTER(C.c_double)]
@@ -288,6 +290,12 @@ class PM:
         s = (C.c_int64 * 3)()
         self._check(self._lib.pmgpu_p3m_pair_stats(self._ctx, s))
         return tuple(s)
+
+    def p3m_cell_geometry(self):
+        """(x, y, z divisions, cells) of the short-range cell grid in use."""
+        g = (C.c_int32 * 4)()
+        self._check(self._lib.pmgpu_p3m_cell_geometry(self._ctx, g))
+        return tuple(g)
 
     def leapfrog(self, op, dt=1.0):
         self._check(self._lib.pmgpu_leapfrog(self._ctx, int(op), float(dt)))

@@ -542,6 +542,17 @@ int pmgpu_p3m_pair_stats(pmgpu_ctx* c, int64_t* stats) {
   PMGPU_API_END(c)
 }
 
+int pmgpu_p3m_cell_geometry(pmgpu_ctx* c, int32_t* geo) {
+  PMGPU_API_BEGIN(c)
+  if (!c->p3m_ready) fail(PMGPU_ERR_NOT_READY, "call pmgpu_p3m_configure first");
+  if (!geo) fail(PMGPU_ERR_INVALID_ARGUMENT, "geo is NULL");
+  geo[0] = c->sr_divx_base * c->sr_xfine;
+  geo[1] = c->sr_divy;
+  geo[2] = c->sr_divz;
+  geo[3] = (int32_t)c->ncell_sr;
+  PMGPU_API_END(c)
+}
+
 int pmgpu_leapfrog(pmgpu_ctx* c, int32_t op, float dt) {
   PMGPU_API_BEGIN(c)
   require_particles(c);

@@ -250,9 +250,11 @@ def test_p3m_anisotropic_cells_for_a_thin_disk(gpu_lib, oracle):
         pairs, tested = pm.p3m_count_pairs(tested=True)
         pm.pm_step()
         pm.p3m_short_range()
-        out.append((pm.get_tuning()[3], pairs, tested, pm.get_particles()[2]))
-    (d, pairs_a, tested_a, a_auto), (d2, pairs_i, tested_i, a_iso) = out
+        out.append((pm.get_tuning()[3], pairs, tested, pm.get_particles()[2], pm.p3m_cell_geometry()))
+    (d, pairs_a, tested_a, a_auto, geo_a), (d2, pairs_i, tested_i, a_iso, geo_i) = out
     assert d == d2 and pairs_a == pairs_i
+    assert geo_i[1] == geo_i[2] == d  # explicit: cubes in y and z
+    assert geo_a[1] > geo_a[2]  # the disk lies in x-y: finer in y, deeper in z
     assert tested_a < 0.95 * tested_i
     # the cells order each i's j-stream differently: FP32 rounding only
     assert _rel(a_auto, a_iso) <= 1e-5
