@@ -159,15 +159,15 @@ static double sr_cell_budget(const pmgpu_ctx* c) {
 // sr_divy, z sides >= cutoff / sr_divz, x sides >= cutoff / (sr_divx_base
 // sr_xfine).  Finer x cells trim each neighbour row closer to the cutoff
 // sphere (a row is contiguous in memory, so its length costs nothing but
-// candidates).  sr_xfine is the largest of 4, 2, 1 that keeps the grid
-// within the cell budget.
+// candidates).  sr_xfine is the largest of sr_xfine_max (4 or 8), its
+// halvings and 1 that keeps the grid within the cell budget.
 void sr_cells_setup(pmgpu_ctx* c) {
   const double rc = (double)c->p3m.cutoff * (1.0 + 1e-5);
   // cell counts in FP64 first, so that a tiny cutoff cannot overflow an int
   double d[3];
   d[1] = std::max(1.0, std::floor((double)c->p3m.box[1] / (rc / c->sr_divy)));
   d[2] = std::max(1.0, std::floor((double)c->p3m.box[2] / (rc / c->sr_divz)));
-  for (c->sr_xfine = 4;; c->sr_xfine /= 2) {
+  for (c->sr_xfine = c->sr_xfine_max;; c->sr_xfine /= 2) {
     d[0] = std::max(1.0, std::floor((double)c->p3m.box[0] / (rc / (c->sr_divx_base * c->sr_xfine))));
     if (d[0] * d[1] * d[2] <= sr_cell_budget(c) || c->sr_xfine == 1) break;
   }
@@ -191,6 +191,7 @@ void sr_cells_setup(pmgpu_ctx* c) {
 
 void sr_set_division(pmgpu_ctx* c, int div) {
   c->sr_div = c->sr_divy = c->sr_divz = c->sr_divx_base = div;
+  c->sr_xfine_max = 4;
   if (c->p3m_ready) {
     HIP_CHECK(hipStreamSynchronize(c->stream));
     sr_cells_setup(c);
@@ -500,6 +501,7 @@ int pmgpu_p3m_configure(pmgpu_ctx* c, const pmgpu_p3m_params* p) {
   } else {
     c->sr_divy = c->sr_divz = c->sr_divx_base = c->sr_div;
   }
+  c->sr_xfine_max = 4;
   sr_cells_setup(c);
   dalloc(&c->sr_table, SR_TABLE_LOG);  // the larger of the two tables
   dalloc(&c->pair_count, 3);
@@ -682,7 +684,8 @@ int pmgpu_set_tuning(pmgpu_ctx* c, const int32_t* knobs) {
   c->int_chunk = itp;
   c->blocks_per_cu = bpc;
   c->sr_div_set = true;
-  if (div != c->sr_div || c->sr_divy != div || c->sr_divz != div || c->sr_divx_base != div)
+  if (div != c->sr_div || c->sr_divy != div || c->sr_divz != div || c->sr_divx_base != div ||
+      c->sr_xfine_max != 4)
     sr_set_division(c, div);
   PMGPU_API_END(c)
 }

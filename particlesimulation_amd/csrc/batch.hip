@@ -83,7 +83,8 @@ void apply_knobs(pmgpu_ctx* c, const int32_t k[6]) {
   c->blocks_per_cu = k[2];
   c->interp_kernel = k[4];
   c->sr_chunking = k[5];
-  if (k[3] != c->sr_div || c->sr_divy != k[3] || c->sr_divz != k[3] || c->sr_divx_base != k[3])
+  if (k[3] != c->sr_div || c->sr_divy != k[3] || c->sr_divz != k[3] || c->sr_divx_base != k[3] ||
+      c->sr_xfine_max != 4)
     sr_set_division(c, k[3]);
 }
 

@@ -30,6 +30,8 @@
 #include <cmath>
 #include <cstring>
 #include <algorithm>
+#include <array>
+#include <vector>
 
 namespace pmgpu {
 
@@ -673,34 +675,57 @@ void p3m_count_pairs(pmgpu_ctx* c, int64_t* accepted, int64_t* tested, int64_t* 
 // than cubes (the candidates of a pair are the cells within the cutoff of
 // the i cell, so the excess over the cutoff sphere grows with the cell sides
 // along which the particles extend).  Each geometry -- the isotropic
-// division d, and (4d, 1), (2d, 1), (2d, 2) with y and z either way round,
-// x sides as for d -- is sorted and scored by the candidates it tests plus
-// 16 per cell (the sort's cost per cell, in candidate-pair times); the
-// lowest score wins.  Every geometry visits the same pair set.
+// division d, y or z cells 2d, 3d or 4d times finer with the other one
+// cutoff deep, or 2d times finer with the other half a cutoff deep, each with
+// x cells 4 or 8 times finer than d (as the cell budget allows) -- is sorted
+// and scored by the candidates it tests plus 16 per cell (the per-step
+// sort's cost per cell, in pair-evaluation times); the lowest score wins.
+// Every geometry visits the same pair set.
 void sr_choose_cells(pmgpu_ctx* c) {
   const int d = c->sr_div;
-  const int cand[7][2] = {{d, d}, {4 * d, 1}, {1, 4 * d}, {2 * d, 1}, {1, 2 * d}, {2 * d, 2}, {2, 2 * d}};
-  int best = 0;
-  double best_score = 0.0;
-  for (int k = 0; k < 7; ++k) {
-    const int dy = cand[k][0], dz = cand[k][1];
-    if ((2 * dy + 1) * (2 * dz + 1) > SR_ROWS_MAX) continue;
-    if (k > 0 && dy == cand[0][0] && dz == cand[0][1]) continue;
-    c->sr_divy = dy;
-    c->sr_divz = dz;
+  std::vector<std::array<int, 3>> cand = {{d, d, 4}};
+  for (int xf : {4, 8}) {
+    if (xf == 8) cand.push_back({d, d, 8});
+    for (int k : {2, 3, 4}) {
+      cand.push_back({k * d, 1, xf});
+      cand.push_back({1, k * d, xf});
+    }
+    cand.push_back({2 * d, 2, xf});
+    cand.push_back({2, 2 * d, xf});
+  }
+  // cells at x refinement 1, the fewest sr_cells_setup can fall back to
+  const double rc = (double)c->p3m.cutoff * (1.0 + 1e-5);
+  auto cells_min = [&](int y, int z) {
+    return std::max(1.0, std::floor((double)c->p3m.box[0] / (rc / d))) *
+           std::max(1.0, std::floor((double)c->p3m.box[1] / (rc / y))) *
+           std::max(1.0, std::floor((double)c->p3m.box[2] / (rc / z)));
+  };
+  std::vector<std::array<int, 3>> seen;  // (y, z, x) divisions scored
+  std::array<int, 3> best = cand[0];
+  double best_score = -1.0;
+  for (const auto& g : cand) {
+    if ((2 * g[0] + 1) * (2 * g[1] + 1) > SR_ROWS_MAX) continue;
+    if (&g != &cand[0] && cells_min(g[0], g[1]) > (double)SR_CELLS_MAX) continue;
+    c->sr_divy = g[0];
+    c->sr_divz = g[1];
     c->sr_divx_base = d;
+    c->sr_xfine_max = g[2];
     sr_cells_setup(c);
+    const std::array<int, 3> got = {g[0], g[1], d * c->sr_xfine};
+    if (std::find(seen.begin(), seen.end(), got) != seen.end()) continue;
+    seen.push_back(got);
     unsigned long long h[3] = {0, 0, 0};
     sr_count(c, false, h);
     const double score = (double)h[1] + 16.0 * (double)c->ncell_sr;
-    if (k == 0 || score < best_score) {
-      best = k;
+    if (best_score < 0.0 || score < best_score) {
+      best = g;
       best_score = score;
     }
   }
-  c->sr_divy = cand[best][0];
-  c->sr_divz = cand[best][1];
+  c->sr_divy = best[0];
+  c->sr_divz = best[1];
   c->sr_divx_base = d;
+  c->sr_xfine_max = best[2];
   sr_cells_setup(c);
 }
 

@@ -121,6 +121,7 @@ struct pmgpu_ctx {
   // equal sr_div unless p3m_configure picked anisotropic cells for the particles
   int sr_divy = pmgpu::SR_CELL_DIV, sr_divz = pmgpu::SR_CELL_DIV, sr_divx_base = pmgpu::SR_CELL_DIV;
   int sr_xfine = 1;                     // x cells are this many times finer than sr_divx_base gives
+  int sr_xfine_max = 4;                 // the x refinement sr_cells_setup starts from
   int interp_kernel = 0;                // 0 auto, 1 tiled (LDS halo), 2 gather from an E grid
   int sr_chunking = 0;                  // 0 auto (rows), 1 cells, 2 rows of cells
   float4* efield = nullptr;             // M, the gather variant's E grid
