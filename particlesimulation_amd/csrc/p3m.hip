@@ -4,18 +4,22 @@
 // The reference walks a chaining mesh of cells >= cutoff with linked lists,
 // a 13-cell half shell and per-neighbour accumulation slots on CPU threads.
 // Here:
-//  - particles are counting-sorted by a short-range cell of y and z side
-//    >= cutoff/div and x side up to 4 times finer (div 1..6, chosen from the
-//    cell count or tuned; non-periodic, as the chaining mesh; out-of-box
+//  - particles are counting-sorted by a short-range cell of sides >=
+//    cutoff/div per axis, into a compact float4 (x, y, z, m) copy: cubes of
+//    division div in y and z with x up to 4 times finer (div 1..6, chosen
+//    from the cell count or tuned), or, when the division is not tuned, the
+//    shape among a few anisotropic ones (finer across a thin particle set's
+//    plane, one cutoff deep through it) that tests the fewest candidates
+//    (sr_choose_cells); non-periodic, as the chaining mesh; out-of-box
 //    particles clamp to the edge cells, which never loses a pair because
-//    clamping is 1-Lipschitz), into a compact float4 (x, y, z, m) copy;
+//    clamping is 1-Lipschitz;
 //  - one work item = 64 consecutive i-particles of one cell, or of one
 //    (y, z) row of cells (spanning x cells cx0..cx1, so sparse cells do not
 //    leave lanes idle; tunable);
 //    its j-stream is the contiguous (dz, dy) rows of the neighbourhood, each
 //    trimmed to the x cells that can hold a pair within the cutoff (corner
-//    rows drop out for div >= 3), their ranges computed in parallel into
-//    LDS.  The SR_WAVES
+//    rows drop out for div >= 3), their ranges (at most SR_ROWS_MAX)
+//    computed in parallel into LDS.  The SR_WAVES
 //    waves of a workgroup take interleaved 64-particle blocks of that stream
 //    and read each j through the scalar unit (uniform address), so the inner
 //    loop is pure FP32 VALU with scalar operands;
