@@ -142,7 +142,7 @@ static void alloc_particle_arrays(pmgpu_ctx* c, int64_t n) {
     dalloc(&c->sr_owner, (int64_t)c->ncell_sr + n / 64 + 2);
     dalloc(&c->sr_key, n);
     dalloc(&c->sr_rank, n);
-    dalloc(&c->sr_pm, n);
+    dalloc(&c->sr_pm, n + SR_JGROUP);
     dalloc(&c->sr_idx, n);
   }
 }
@@ -509,7 +509,7 @@ int pmgpu_p3m_configure(pmgpu_ctx* c, const pmgpu_p3m_params* p) {
   if (c->n > 0) {
     dalloc(&c->sr_key, c->n);
     dalloc(&c->sr_rank, c->n);
-    dalloc(&c->sr_pm, c->n);
+    dalloc(&c->sr_pm, c->n + SR_JGROUP);
     dalloc(&c->sr_idx, c->n);
   }
   p3m_setup(c);

@@ -284,15 +284,35 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
         for (int jb = js; jb < je; jb += 64, ++blk) {
           if (blk % SR_WAVES != wave) continue;
           const int jend = min(jb + 64, je);
-#pragma unroll 8
-          for (int j = jb; j < jend; ++j) {
-            const float4 pj = sr_pm[j];  // uniform address: scalar load
-            const float dx = pi.x - pj.x, dy_ = pi.y - pj.y, dz_ = pi.z - pj.z;
-            const float r2 = fmaf(dz_, dz_, fmaf(dy_, dy_, dx * dx));
-            const float F = pair_factor<MODE>(r2, pj.w, table, P, log_hi_v);
-            ax = fmaf(F, dx, ax);
-            ay = fmaf(F, dy_, ay);
-            az = fmaf(F, dz_, az);
+          // whole groups of SR_JGROUP j; the last, partial group of a block
+          // reads a whole group too, and its j past jend (particles of other
+          // rows, or the zeroed pad after the last particle) enter with zero
+          // mass, so they add exactly nothing
+          int j0 = jb;
+          for (; j0 + SR_JGROUP <= jend; j0 += SR_JGROUP) {
+#pragma unroll
+            for (int k = 0; k < SR_JGROUP; ++k) {
+              const float4 pj = sr_pm[j0 + k];  // uniform address: scalar load
+              const float dx = pi.x - pj.x, dy_ = pi.y - pj.y, dz_ = pi.z - pj.z;
+              const float r2 = fmaf(dz_, dz_, fmaf(dy_, dy_, dx * dx));
+              const float F = pair_factor<MODE>(r2, pj.w, table, P, log_hi_v);
+              ax = fmaf(F, dx, ax);
+              ay = fmaf(F, dy_, ay);
+              az = fmaf(F, dz_, az);
+            }
+          }
+          if (j0 < jend) {
+#pragma unroll
+            for (int k = 0; k < SR_JGROUP; ++k) {
+              const float4 pj = sr_pm[j0 + k];
+              const float mj = j0 + k < jend ? pj.w : 0.0f;
+              const float dx = pi.x - pj.x, dy_ = pi.y - pj.y, dz_ = pi.z - pj.z;
+              const float r2 = fmaf(dz_, dz_, fmaf(dy_, dy_, dx * dx));
+              const float F = pair_factor<MODE>(r2, mj, table, P, log_hi_v);
+              ax = fmaf(F, dx, ax);
+              ay = fmaf(F, dy_, ay);
+              az = fmaf(F, dz_, az);
+            }
           }
         }
       }

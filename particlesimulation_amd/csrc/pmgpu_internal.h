@@ -39,6 +39,9 @@ constexpr int SR_CELL_DIV = 1, SR_CELL_DIV_MAX = 6;
 // neighbourhood rows (2 divy + 1)(2 divz + 1) of any cell geometry: the
 // isotropic maximum, which also bounds the anisotropic geometries tried
 constexpr int SR_ROWS_MAX = (2 * SR_CELL_DIV_MAX + 1) * (2 * SR_CELL_DIV_MAX + 1);
+// the pair loop reads j in groups of SR_JGROUP; the sorted particle copy
+// carries SR_JGROUP - 1 zeroed entries past the last particle for it
+constexpr int SR_JGROUP = 8;
 constexpr int64_t SR_CELLS_MAX = 128ll * 1024 * 1024;  // short-range cells, x refinement included
 constexpr int SR_TABLE = 512;  // padded reference table (the reference uses 500 entries)
 constexpr int SR_TABLE_REF = 500;
