@@ -288,7 +288,8 @@ int pmgpu_p3m_pair_stats(pmgpu_ctx* ctx, int64_t* stats);
    geo[0..2] the x, y and z divisions (cell sides >= cutoff / geo[a]; the
    neighbourhood reaches geo[a] cells each way), geo[3] the number of cells.
    Unless knobs[3] of pmgpu_set_tuning fixes an isotropic division,
-   pmgpu_p3m_configure (with particles set) picks, among the isotropic
+   pmgpu_p3m_configure (with particles set), and pmgpu_set_particles when it
+   changes the particle count of a configured context, pick, among the isotropic
    division d and cells d times 4 or 2 finer in y or z and one cutoff (or
    half of one) deep in the other, the geometry whose candidate pairs plus
    16 per cell are fewest; all visit the same pair set. */

@@ -189,6 +189,29 @@ void sr_cells_setup(pmgpu_ctx* c) {
   if (c->n > 0) dalloc(&c->sr_owner, (int64_t)c->ncell_sr + c->n / 64 + 2);
 }
 
+// The division p3m_configure picks when none is set: the finest whose
+// isotropic cell grid stays modest (finer cells visit fewer candidate pairs;
+// the pair set is the same for every division).
+static void sr_auto_division(pmgpu_ctx* c) {
+  auto cells_at = [&](int d) {
+    double cells = 1.0;
+    for (int i = 0; i < 3; ++i)
+      cells *= std::floor((double)c->p3m.box[i] / ((double)c->p3m.cutoff / d * (1.0 + 1e-5)));
+    return cells;
+  };
+  // cells of side >= cutoff (the reference's chaining mesh) only when even
+  // half that side gives too many cells
+  int div = cells_at(2) <= (double)SR_CELLS_MAX ? 2 : SR_CELL_DIV;
+  const double budget = std::min(16.0 * 1024 * 1024, sr_cell_budget(c));
+  for (int d = SR_CELL_DIV_MAX; d > 2; --d) {
+    if (cells_at(d) <= budget) {
+      div = d;
+      break;
+    }
+  }
+  c->sr_div = div;
+}
+
 void sr_set_division(pmgpu_ctx* c, int div) {
   c->sr_div = c->sr_divy = c->sr_divz = c->sr_divx_base = div;
   c->sr_xfine_max = 4;
@@ -355,6 +378,7 @@ int pmgpu_set_particles(pmgpu_ctx* c, int64_t n, const float* pos, const float* 
     fail(PMGPU_ERR_INVALID_ARGUMENT, "pos and mass are required");
   if (n > (int64_t)INT_MAX / 2) fail(PMGPU_ERR_UNSUPPORTED, "too many particles");
   HIP_CHECK(hipStreamSynchronize(c->stream));
+  const int64_t old_n = c->n;
   alloc_particle_arrays(c, n);
   c->n = n;
   std::vector<float4> hp(n), hv(n);
@@ -378,6 +402,17 @@ int pmgpu_set_particles(pmgpu_ctx* c, int64_t n, const float* pos, const float* 
   c->sorted_valid = false;
   c->run_iter = 0;
   reset_escape(c);
+  // the short-range cells depend on the particles: their budget on the
+  // count, and (unless the division is set) their shape on the positions
+  if (c->p3m_ready && n != old_n) {
+    if (!c->sr_div_set) {
+      sr_auto_division(c);
+      c->sr_divy = c->sr_divz = c->sr_divx_base = c->sr_div;
+      c->sr_xfine_max = 4;
+    }
+    sr_cells_setup(c);
+    if (!c->sr_div_set && n > 0) sr_choose_cells(c);
+  }
   PMGPU_API_END(c)
 }
 
@@ -479,28 +514,8 @@ int pmgpu_p3m_configure(pmgpu_ctx* c, const pmgpu_p3m_params* p) {
     if (!(p->box[i] > 0.0f)) fail(PMGPU_ERR_INVALID_ARGUMENT, "box must be > 0");
   HIP_CHECK(hipStreamSynchronize(c->stream));
   c->p3m = *p;
-  if (!c->sr_div_set) {
-    // the finest division whose cell grid stays modest (finer cells visit
-    // fewer candidate pairs; the pair set is the same for every division)
-    auto cells_at = [&](int d) {
-      double cells = 1.0;
-      for (int i = 0; i < 3; ++i) cells *= std::floor((double)p->box[i] / ((double)p->cutoff / d * (1.0 + 1e-5)));
-      return cells;
-    };
-    // cells of side >= cutoff (the reference's chaining mesh) only when even
-    // half that side gives too many cells
-    int div = cells_at(2) <= (double)SR_CELLS_MAX ? 2 : SR_CELL_DIV;
-    const double budget = std::min(16.0 * 1024 * 1024, sr_cell_budget(c));
-    for (int d = SR_CELL_DIV_MAX; d > 2; --d) {
-      if (cells_at(d) <= budget) {
-        div = d;
-        break;
-      }
-    }
-    c->sr_div = c->sr_divy = c->sr_divz = c->sr_divx_base = div;
-  } else {
-    c->sr_divy = c->sr_divz = c->sr_divx_base = c->sr_div;
-  }
+  if (!c->sr_div_set) sr_auto_division(c);
+  c->sr_divy = c->sr_divz = c->sr_divx_base = c->sr_div;
   c->sr_xfine_max = 4;
   sr_cells_setup(c);
   dalloc(&c->sr_table, SR_TABLE_LOG);  // the larger of the two tables

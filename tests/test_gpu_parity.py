@@ -238,21 +238,27 @@ def test_p3m_anisotropic_cells_for_a_thin_disk(gpu_lib, oracle):
     vel = np.zeros_like(pos)
     m = r.uniform(0.5, 1.5, n).astype(np.float32) / n
     out = []
-    for explicit in (False, True):
+    for mode in ("auto", "explicit", "configured_first"):
         pm = _make(gpu_lib, grid, 2, 0, 2, a=3.0)
+        if mode == "configured_first":  # the cells follow the particles set later
+            pm.set_particles(pos[:100], vel[:100], m[:100])
+            pm.p3m_configure(cutoff=3.0, particle_diameter=3.0, cloud_shape=1, use_table=2)
         pm.set_particles(pos, vel, m)
         pm.init_greens()
-        if explicit:
+        if mode == "explicit":
             k = list(pm.get_tuning())
             k[3] = out[0][0]
             pm.set_tuning(*k)
-        pm.p3m_configure(cutoff=3.0, particle_diameter=3.0, cloud_shape=1, use_table=2)
+        if mode != "configured_first":
+            pm.p3m_configure(cutoff=3.0, particle_diameter=3.0, cloud_shape=1, use_table=2)
         pairs, tested = pm.p3m_count_pairs(tested=True)
         pm.pm_step()
         pm.p3m_short_range()
         out.append((pm.get_tuning()[3], pairs, tested, pm.get_particles()[2], pm.p3m_cell_geometry()))
-    (d, pairs_a, tested_a, a_auto, geo_a), (d2, pairs_i, tested_i, a_iso, geo_i) = out
+    (d, pairs_a, tested_a, a_auto, geo_a), (d2, pairs_i, tested_i, a_iso, geo_i), late = out
     assert d == d2 and pairs_a == pairs_i
+    assert late[4] == geo_a and late[1] == pairs_a and late[2] == tested_a
+    assert _rel(late[3], a_auto) <= 1e-5
     assert geo_i[1] == geo_i[2] == d  # explicit: cubes in y and z
     assert geo_a[1] > geo_a[2]  # the disk lies in x-y: finer in y, deeper in z
     assert tested_a < 0.95 * tested_i
