@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "pmgpu_internal.h"
@@ -221,6 +222,44 @@ void sr_set_division(pmgpu_ctx* c, int div) {
   }
 }
 
+// The most common mass and the particles off it (original ids ascending),
+// if there are at most SR_ODD_MAX of them.
+static void set_mass_classes(pmgpu_ctx* c, const float* mass, int64_t n) {
+  c->sr_nodd = -1;
+  if (n <= 0) return;
+  std::unordered_map<uint32_t, int64_t> count;
+  for (int64_t i = 0; i < n && (int64_t)count.size() <= SR_ODD_MAX + 1; ++i) {
+    uint32_t b;
+    std::memcpy(&b, &mass[i], sizeof b);
+    ++count[b];
+  }
+  if ((int64_t)count.size() > SR_ODD_MAX + 1) return;
+  uint32_t best = 0;
+  int64_t best_n = -1;
+  for (const auto& kv : count)
+    if (kv.second > best_n || (kv.second == best_n && kv.first < best)) {
+      best = kv.first;
+      best_n = kv.second;
+    }
+  if (n - best_n > SR_ODD_MAX) return;
+  float m0;
+  std::memcpy(&m0, &best, sizeof m0);
+  std::vector<int32_t> odd;
+  for (int64_t i = 0; i < n; ++i) {
+    uint32_t b;
+    std::memcpy(&b, &mass[i], sizeof b);
+    if (b != best) odd.push_back((int32_t)i);
+  }
+  if (!c->sr_odd_ids) {
+    dalloc(&c->sr_odd_ids, SR_ODD_MAX);
+    dalloc(&c->sr_odd_pos, SR_ODD_MAX);
+  }
+  if (!odd.empty())
+    HIP_CHECK(hipMemcpy(c->sr_odd_ids, odd.data(), sizeof(int32_t) * odd.size(), hipMemcpyHostToDevice));
+  c->sr_m0 = m0;
+  c->sr_nodd = (int)odd.size();
+}
+
 static void set_shift(pmgpu_ctx* c) {
   // every cell sum of the deposit is bounded by 1.5 * sum|m| (the largest
   // weight product of any scheme is 1.125^3 < 1.5); keep it below 2^62
@@ -349,6 +388,7 @@ void pmgpu_destroy(pmgpu_ctx* c) {
   dfree(&c->tw_x); dfree(&c->tw_y); dfree(&c->tw_z); dfree(&c->tw_r2c);
   dfree(&c->sr_table); dfree(&c->sr_count); dfree(&c->sr_start); dfree(&c->sr_key);
   dfree(&c->sr_rank); dfree(&c->sr_pm); dfree(&c->sr_idx); dfree(&c->sr_nchunk);
+  dfree(&c->sr_odd_ids); dfree(&c->sr_odd_pos);
   dfree(&c->sr_chunk_start); dfree(&c->pair_count); dfree(&c->escape); dfree(&c->diag_scratch);
   for (auto& p : c->pending) {
     (void)hipEventDestroy(p.a);
@@ -398,6 +438,7 @@ int pmgpu_set_particles(pmgpu_ctx* c, int64_t n, const float* pos, const float* 
     HIP_CHECK(hipMemcpy(c->vel[0], hv.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
   }
   c->mass_abs_sum = s;
+  set_mass_classes(c, mass, n);
   set_shift(c);
   c->sorted_valid = false;
   c->run_iter = 0;

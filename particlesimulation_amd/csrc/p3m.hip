@@ -71,16 +71,33 @@ __global__ void k_sr_keys(const float4* __restrict__ pm, int64_t n, float ihx, f
   }
 }
 
-__global__ void k_sr_scatter(const float4* __restrict__ pm, int64_t n, const int32_t* __restrict__ key,
+// Also records the sorted positions of the particles off the common mass m0
+// (original ids ascending in odd_ids, nodd of them; nodd <= 0: none kept).
+// The particle arrays are in the tile sort's order: the original id of the
+// particle at i is the int in vel[i].w.
+__global__ void k_sr_scatter(const float4* __restrict__ pm, const float4* __restrict__ vel, int64_t n,
+                             const int32_t* __restrict__ key,
                              const int32_t* __restrict__ rank, const int32_t* __restrict__ start,
-                             float4* __restrict__ sr_pm, int32_t* __restrict__ sr_idx,
+                             float4* __restrict__ sr_pm, int32_t* __restrict__ sr_idx, float m0, int nodd,
+                             const int32_t* __restrict__ odd_ids, int32_t* __restrict__ odd_pos,
                              const int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int64_t d = (int64_t)start[key[i]] + rank[i];
-  sr_pm[d] = pm[i];
+  const float4 p = pm[i];
+  sr_pm[d] = p;
   sr_idx[d] = (int32_t)i;
+  if (nodd > 0 && __float_as_uint(p.w) != __float_as_uint(m0)) {
+    const int32_t id = __float_as_int(vel[i].w);
+    int lo = 0, hi = nodd - 1;  // odd_ids holds id
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (odd_ids[mid] < id) lo = mid + 1;
+      else hi = mid;
+    }
+    odd_pos[lo] = (int32_t)d;
+  }
 }
 
 // Work items per unit in chunks of SR_ICHUNK particles; a unit is a cell
@@ -190,13 +207,17 @@ __device__ __forceinline__ bool row_extent(const SrParams& P, int dy, int dz, in
   return true;
 }
 
-template <int MODE>
+// UNIFORM: every j enters the pair loop at the common mass m0 (no multiply
+// per pair); the sum is scaled by m0 once, and the nodd particles off m0
+// (sorted positions in odd_pos) add their (m_j - m0) share per i.
+template <int MODE, bool UNIFORM>
 __global__ __launch_bounds__(SR_THREADS) void k_sr(
     const float4* __restrict__ sr_pm, const int32_t* __restrict__ sr_start,
     const int32_t* __restrict__ chunk_start, const int32_t* __restrict__ owner, int nunit,
     const int32_t* __restrict__ sr_idx,
     const float2* __restrict__ table_g, SrParams P, float4* __restrict__ pm,
     float4* __restrict__ vel, float4* __restrict__ acc, int mode, float kick, float3 box,
+    float m0, int nodd, const int32_t* __restrict__ odd_pos,
     int32_t* __restrict__ counters, int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
   // The partial sums of half of the waves at a time.
@@ -295,7 +316,7 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
               const float4 pj = sr_pm[j0 + k];  // uniform address: scalar load
               const float dx = pi.x - pj.x, dy_ = pi.y - pj.y, dz_ = pi.z - pj.z;
               const float r2 = fmaf(dz_, dz_, fmaf(dy_, dy_, dx * dx));
-              const float F = pair_factor<MODE>(r2, pj.w, table, P, log_hi_v);
+              const float F = pair_factor<MODE>(r2, UNIFORM ? 1.0f : pj.w, table, P, log_hi_v);
               ax = fmaf(F, dx, ax);
               ay = fmaf(F, dy_, ay);
               az = fmaf(F, dz_, az);
@@ -305,7 +326,7 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
 #pragma unroll
             for (int k = 0; k < SR_JGROUP; ++k) {
               const float4 pj = sr_pm[j0 + k];
-              const float mj = j0 + k < jend ? pj.w : 0.0f;
+              const float mj = j0 + k < jend ? (UNIFORM ? 1.0f : pj.w) : 0.0f;
               const float dx = pi.x - pj.x, dy_ = pi.y - pj.y, dz_ = pi.z - pj.z;
               const float r2 = fmaf(dz_, dz_, fmaf(dy_, dy_, dx * dx));
               const float F = pair_factor<MODE>(r2, mj, table, P, log_hi_v);
@@ -337,6 +358,20 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
         s.x += o.x;
         s.y += o.y;
         s.z += o.z;
+      }
+      if constexpr (UNIFORM) {
+        s.x *= m0;
+        s.y *= m0;
+        s.z *= m0;
+        for (int q = 0; q < nodd; ++q) {
+          const float4 pj = sr_pm[odd_pos[q]];
+          const float dx = pi.x - pj.x, dy_ = pi.y - pj.y, dz_ = pi.z - pj.z;
+          const float r2 = fmaf(dz_, dz_, fmaf(dy_, dy_, dx * dx));
+          const float F = pair_factor<MODE>(r2, pj.w - m0, table, P, log_hi_v);
+          s.x = fmaf(F, dx, s.x);
+          s.y = fmaf(F, dy_, s.y);
+          s.z = fmaf(F, dz_, s.z);
+        }
       }
       const int idx = sr_idx[i];
       float4 a = acc[idx];
@@ -630,8 +665,10 @@ static void sr_sort(pmgpu_ctx* c, int iteration) {
                      c->sr_rank, c->sr_count, c->escape, iteration);
   HIP_CHECK(hipGetLastError());
   exclusive_scan_i32(c, c->sr_count, c->sr_start, c->ncell_sr);
-  hipLaunchKernelGGL(k_sr_scatter, dim3(g), dim3(256), 0, c->stream, c->pm[c->cur], c->n, c->sr_key,
-                     c->sr_rank, c->sr_start, c->sr_pm, c->sr_idx, c->escape, iteration);
+  hipLaunchKernelGGL(k_sr_scatter, dim3(g), dim3(256), 0, c->stream, c->pm[c->cur], c->vel[c->cur], c->n,
+                     c->sr_key,
+                     c->sr_rank, c->sr_start, c->sr_pm, c->sr_idx, c->sr_m0, c->sr_nodd, c->sr_odd_ids,
+                     c->sr_odd_pos, c->escape, iteration);
   HIP_CHECK(hipGetLastError());
   const int nunit = sr_units(c), stride = sr_rowspan(c) ? c->mcx : 1;
   hipLaunchKernelGGL(k_sr_plan, dim3((nunit + 255) / 256), dim3(256), 0, c->stream, c->sr_start, stride, nunit,
@@ -643,17 +680,23 @@ static void sr_sort(pmgpu_ctx* c, int iteration) {
   stage_end(c, tok);
 }
 
+template <int MODE, bool UNIFORM>
+static void launch_sr_mass(pmgpu_ctx* c, int mode, float kick, int iteration) {
+  const int grid = c->num_cu * 8;
+  hipLaunchKernelGGL((k_sr<MODE, UNIFORM>), dim3(grid), dim3(SR_THREADS), 0, c->stream, c->sr_pm,
+                     c->sr_start, c->sr_chunk_start, c->sr_owner, sr_units(c), c->sr_idx, c->sr_table,
+                     sr_params(c), c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick,
+                     make_float3(c->prm.box[0], c->prm.box[1], c->prm.box[2]), c->sr_m0,
+                     std::max(c->sr_nodd, 0), c->sr_odd_pos, c->counters, c->escape, iteration);
+  HIP_CHECK(hipGetLastError());
+}
+
 template <int MODE>
 static void launch_sr(pmgpu_ctx* c, int mode, float kick, int iteration) {
   if ((2 * c->sr_divy + 1) * (2 * c->sr_divz + 1) > SR_ROWS_MAX)
     fail(PMGPU_ERR_INVALID_ARGUMENT, "short-range neighbourhood has too many rows");
-  const int grid = c->num_cu * 8;
-  hipLaunchKernelGGL((k_sr<MODE>), dim3(grid), dim3(SR_THREADS), 0, c->stream, c->sr_pm,
-                     c->sr_start, c->sr_chunk_start, c->sr_owner, sr_units(c), c->sr_idx, c->sr_table,
-                     sr_params(c), c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick,
-                     make_float3(c->prm.box[0], c->prm.box[1], c->prm.box[2]), c->counters,
-                     c->escape, iteration);
-  HIP_CHECK(hipGetLastError());
+  if (c->sr_nodd >= 0) launch_sr_mass<MODE, true>(c, mode, kick, iteration);
+  else launch_sr_mass<MODE, false>(c, mode, kick, iteration);
 }
 
 void p3m_short_range(pmgpu_ctx* c, int mode, float kick, int iteration) {

@@ -42,6 +42,7 @@ constexpr int SR_ROWS_MAX = (2 * SR_CELL_DIV_MAX + 1) * (2 * SR_CELL_DIV_MAX + 1
 // the pair loop reads j in groups of SR_JGROUP; the sorted particle copy
 // carries SR_JGROUP - 1 zeroed entries past the last particle for it
 constexpr int SR_JGROUP = 8;
+constexpr int SR_ODD_MAX = 64;  // particles off the common mass that the uniform-mass pair loop corrects
 constexpr int64_t SR_CELLS_MAX = 128ll * 1024 * 1024;  // short-range cells, x refinement included
 constexpr int SR_TABLE = 512;  // padded reference table (the reference uses 500 entries)
 constexpr int SR_TABLE_REF = 500;
@@ -96,6 +97,14 @@ struct pmgpu_ctx {
   int32_t* key = nullptr;
   int32_t* rank = nullptr;
   double mass_abs_sum = 0.0;
+  // mass classes of the short-range pass: when all but at most SR_ODD_MAX
+  // particles have the mass sr_m0, the pair loop sums at that mass and the
+  // others (original ids ascending in sr_odd_ids, their sorted positions in
+  // sr_odd_pos) are corrected per work item; sr_nodd = -1 otherwise
+  float sr_m0 = 0.0f;
+  int sr_nodd = -1;
+  int32_t* sr_odd_ids = nullptr;        // SR_ODD_MAX
+  int32_t* sr_odd_pos = nullptr;        // SR_ODD_MAX
   int fx_shift = 0;                     // fixed-point scale 2^fx_shift
   bool sorted_valid = false;            // tile sort matches current positions
 

@@ -224,6 +224,31 @@ def test_p3m_cell_division_keeps_the_pair_set(gpu_lib, oracle, div, chunking):
     assert _rel(a_d, oracle.pm_accel(pos, m, grid, 2, 0, G) + ref_sr) <= FORCE_TOL
 
 
+@pytest.mark.parametrize("use_table", [0, 1, 2])
+@pytest.mark.parametrize("n_heavy", [0, 5])
+def test_p3m_common_mass_matches_oracle(gpu_lib, oracle, use_table, n_heavy):
+    # particles of one mass (but for a few heavy ones) take the pair loop
+    # without a mass multiply and a per-i correction for the heavy ones
+    grid = (64, 64, 64)
+    pos, vel, _ = _particles(6000, grid, 41, clustered=True)
+    m = np.full(len(pos), np.float32(1.0 / len(pos)))
+    m[_rng(42).choice(len(pos), n_heavy, replace=False)] = np.float32(20.0 / len(pos))
+    pm = _make(gpu_lib, grid, 2, 0, 2, a=3.0)
+    pm.set_particles(pos, vel, m)
+    pm.init_greens()
+    pm.p3m_configure(cutoff=3.0, particle_diameter=3.0, cloud_shape=1, use_table=use_table)
+    pm.pm_step()
+    _, _, a_pm = pm.get_particles()
+    pm.p3m_short_range()
+    acc = pm.get_particles()[2]
+    H = 0.25
+    ref_sr, _ = oracle.p3m_short_range(pos, m, np.array(grid, np.float32) * H, 3 * H, H, 3.0, 3.0, 0.0, 1,
+                                       use_table=use_table)
+    assert _rel(acc - a_pm, ref_sr) <= 10 * FORCE_TOL
+    G = oracle.green_grid(2, 2, 0, 3.0, grid)
+    assert _rel(acc, oracle.pm_accel(pos, m, grid, 2, 0, G) + ref_sr) <= FORCE_TOL
+
+
 def test_p3m_anisotropic_cells_for_a_thin_disk(gpu_lib, oracle):
     # with the division left to p3m_configure, a thin disk gets cells finer in
     # the disk's plane and one cutoff deep across it: fewer candidates than
