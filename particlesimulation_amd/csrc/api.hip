@@ -5,7 +5,6 @@
 #include <cmath>
 #include <cstring>
 #include <string>
-#include <unordered_map>
 #include <vector>
 
 #include "pmgpu_internal.h"
@@ -222,33 +221,35 @@ void sr_set_division(pmgpu_ctx* c, int div) {
   }
 }
 
-// The most common mass and the particles off it (original ids ascending),
-// if there are at most SR_ODD_MAX of them.
+// The common mass and the particles off it (original ids ascending), if
+// there are at most SR_ODD_MAX of them: a majority vote over the mass bits
+// (Boyer-Moore) and a count.  A mass held by all but SR_ODD_MAX particles
+// is the majority whenever n > 2 SR_ODD_MAX; below that one may be missed,
+// which leaves the pair loop's per-pair mass multiply on.
 static void set_mass_classes(pmgpu_ctx* c, const float* mass, int64_t n) {
   c->sr_nodd = -1;
   if (n <= 0) return;
-  std::unordered_map<uint32_t, int64_t> count;
-  for (int64_t i = 0; i < n && (int64_t)count.size() <= SR_ODD_MAX + 1; ++i) {
+  auto bits = [&](int64_t i) {
     uint32_t b;
     std::memcpy(&b, &mass[i], sizeof b);
-    ++count[b];
-  }
-  if ((int64_t)count.size() > SR_ODD_MAX + 1) return;
-  uint32_t best = 0;
-  int64_t best_n = -1;
-  for (const auto& kv : count)
-    if (kv.second > best_n || (kv.second == best_n && kv.first < best)) {
-      best = kv.first;
-      best_n = kv.second;
+    return b;
+  };
+  uint32_t cand = 0;
+  int64_t votes = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const uint32_t b = bits(i);
+    if (votes == 0) {
+      cand = b;
+      votes = 1;
+    } else {
+      votes += b == cand ? 1 : -1;
     }
-  if (n - best_n > SR_ODD_MAX) return;
-  float m0;
-  std::memcpy(&m0, &best, sizeof m0);
+  }
   std::vector<int32_t> odd;
   for (int64_t i = 0; i < n; ++i) {
-    uint32_t b;
-    std::memcpy(&b, &mass[i], sizeof b);
-    if (b != best) odd.push_back((int32_t)i);
+    if (bits(i) == cand) continue;
+    if ((int64_t)odd.size() == SR_ODD_MAX) return;
+    odd.push_back((int32_t)i);
   }
   if (!c->sr_odd_ids) {
     dalloc(&c->sr_odd_ids, SR_ODD_MAX);
@@ -256,7 +257,7 @@ static void set_mass_classes(pmgpu_ctx* c, const float* mass, int64_t n) {
   }
   if (!odd.empty())
     HIP_CHECK(hipMemcpy(c->sr_odd_ids, odd.data(), sizeof(int32_t) * odd.size(), hipMemcpyHostToDevice));
-  c->sr_m0 = m0;
+  std::memcpy(&c->sr_m0, &cand, sizeof cand);
   c->sr_nodd = (int)odd.size();
 }
 
