@@ -59,6 +59,8 @@ def lib():
         L.orc_alias_sum_bruteforce.restype = C.c_double
         L.orc_alias_sum_closed.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
         L.orc_alias_sum_closed.restype = C.c_double
+        L.orc_fd_d.argtypes = [C.c_int, C.c_double]
+        L.orc_fd_d.restype = C.c_double
         L.orc_field.argtypes = [_f64p, C.c_int, C.c_int, C.c_int, C.c_int, _f64p]
         L.orc_interpolate.argtypes = [C.c_int64, _f32p, _f64p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                       C.c_int, C.c_void_p, _f64p]
@@ -152,6 +154,11 @@ def alias_sum_bruteforce(scheme, k, lim):
 
 def alias_sum_closed(scheme, k):
     return lib().orc_alias_sum_closed(scheme, float(k[0]), float(k[1]), float(k[2]))
+
+
+def fd_d(fds, k):
+    """D(k)/i of the 2-point or 4-point gradient the optimal G assumes."""
+    return lib().orc_fd_d(int(fds), float(k))
 
 
 def field(phi, fds):

@@ -298,6 +298,10 @@ static inline double fd_d(int fds, double k) {
   return alpha * sin(k) + (1.0 - alpha) * sin(2.0 * k) / 2.0;
 }
 
+/* D(k)/i for one axis, exported so tests can hold it against the field
+   stencils of orc_field (a plane wave through the stencil). */
+double orc_fd_d(int fds, double k) { return fd_d(fds, k); }
+
 /* Exact alias sum of U^2 with window [-lim, lim]^3: used by tests to pin the
    closed forms of alias_sum() (they must agree as lim grows). */
 double orc_alias_sum_bruteforce(int is, double kx, double ky, double kz, int lim) {

@@ -195,7 +195,37 @@ def test_s_fourier_series_branch_is_continuous(oracle):
         assert abs(oracle.s_fourier(cs, 1e-6, a) - 1.0) < 1e-12
 
 
-def test_optimal_green_reproduces_reference_force(oracle):
+def _point_mass_samples(oracle, scheme, fds, G, n, a, rng, reps, rmin=1.0):
+    """PM force of a unit point mass at seeded positions on targets at seeded
+    directions and distances in [rmin, 2a): radial force, full force vector,
+    separation vector and distance per target."""
+    rad, acc_all, dvec, dist = [], [], [], []
+    for _ in range(reps):
+        c = 16.0 + rng.random(3)
+        dirs = rng.standard_normal((60, 3))
+        dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+        r = rng.uniform(rmin, 2 * a, 60)
+        pos = np.vstack([c, c + dirs * r[:, None]]).astype(np.float32)
+        m = np.zeros(len(pos), np.float32)
+        m[0] = 1.0
+        acc = oracle.pm_accel(pos, m, (n, n, n), scheme, fds, G)
+        d = pos[1:].astype(np.float64) - pos[0]
+        dd = np.linalg.norm(d, axis=1)
+        rad.append(-(acc[1:] * d).sum(1) / dd)
+        acc_all.append(acc[1:])
+        dvec.append(d)
+        dist.append(dd)
+    return tuple(map(np.concatenate, (rad, acc_all, dvec, dist)))
+
+
+# (scheme, fds, bias, rms, band): NGP's force scatters over the source's cell
+# by tens of per cent, so its averages carry looser bounds
+_OPT_CASES = [(1, 0, 0.015, 0.06, 0.03), (2, 0, 0.015, 0.06, 0.03), (1, 1, 0.015, 0.06, 0.03),
+              (2, 1, 0.015, 0.06, 0.03), (0, 0, 0.09, 0.22, 0.03), (0, 1, 0.09, 0.22, 0.03)]
+
+
+@pytest.mark.parametrize("scheme,fds,bias,rms,band", _OPT_CASES)
+def test_optimal_green_reproduces_reference_force(oracle, scheme, fds, bias, rms, band):
     # The reference's own check (script/pm_optimal.py): the PM force of a point
     # mass with the optimal G follows the S1/S2 reference force R(r).  The
     # influence function is optimal on average over source and target
@@ -203,31 +233,67 @@ def test_optimal_green_reproduces_reference_force(oracle):
     # Periodic images enter at leading order as the background term +m r/(3N^3).
     n, a = 32, 4.0
     rng = _rng(12)
-    for scheme in (1, 2):
-        for gf, cs in ((1, 0), (2, 1)):
-            G = oracle.green_grid(gf, scheme, 0, a, (n, n, n))
-            rad, ref, dist = [], [], []
-            for _ in range(8):
-                c = 16.0 + rng.random(3)
-                dirs = rng.standard_normal((60, 3))
-                dirs /= np.linalg.norm(dirs, axis=1)[:, None]
-                r = rng.uniform(1.0, 2 * a, 60)
-                pos = np.vstack([c, c + dirs * r[:, None]]).astype(np.float32)
-                m = np.zeros(len(pos), np.float32)
-                m[0] = 1.0
-                acc = oracle.pm_accel(pos, m, (n, n, n), scheme, 0, G)
-                d = pos[1:].astype(np.float64) - pos[0]
-                dd = np.linalg.norm(d, axis=1)
-                rad.append(-(acc[1:] * d).sum(1) / dd)
-                ref.append(np.array([oracle.reference_force(cs, float(x), a) for x in dd]) - dd / (3 * n ** 3))
-                dist.append(dd)
-            rad, ref, dist = map(np.concatenate, (rad, ref, dist))
-            scale = np.sqrt(np.mean(ref ** 2))
-            assert abs(np.mean(rad - ref)) / scale < 0.015, (scheme, gf)
-            assert np.sqrt(np.mean((rad - ref) ** 2)) / scale < 0.06, (scheme, gf)
-            for lo in (2.0, 4.0, 6.0):
-                sel = (dist >= lo) & (dist < lo + 2.0)
-                assert abs(np.mean(rad[sel] / ref[sel]) - 1.0) < 0.03, (scheme, gf, lo)
+    for gf, cs in ((1, 0), (2, 1)):
+        G = oracle.green_grid(gf, scheme, fds, a, (n, n, n))
+        rad, _, _, dist = _point_mass_samples(oracle, scheme, fds, G, n, a, rng, 8)
+        ref = np.array([oracle.reference_force(cs, float(x), a) for x in dist]) - dist / (3 * n ** 3)
+        scale = np.sqrt(np.mean(ref ** 2))
+        assert abs(np.mean(rad - ref)) / scale < bias, (scheme, fds, gf)
+        assert np.sqrt(np.mean((rad - ref) ** 2)) / scale < rms, (scheme, fds, gf)
+        for lo in (4.0, 6.0) if scheme == 0 else (2.0, 4.0, 6.0):
+            sel = (dist >= lo) & (dist < lo + 2.0)
+            assert abs(np.mean(rad[sel] / ref[sel]) - 1.0) < band, (scheme, fds, gf, lo)
+
+
+# (scheme, fds) with the optimal G built for a mismatched (scheme', fds'):
+# a G that assumes another gradient stencil or another assignment window
+_MISMATCH = [((2, 1), (2, 0)), ((2, 0), (2, 1)), ((1, 1), (1, 0)), ((0, 0), (1, 0)), ((0, 1), (1, 1)),
+             ((1, 0), (0, 0)), ((1, 0), (2, 0)), ((2, 1), (1, 1))]
+
+
+@pytest.mark.parametrize("used,built", _MISMATCH)
+def test_optimal_green_is_optimal_for_its_scheme_and_stencil(oracle, used, built):
+    # GreenOptimal (source/greensFunctions.cpp:122-189) minimises the mean
+    # square deviation of the PM force from the reference force over source
+    # and target positions.  The G built for the scheme and stencil in use
+    # must therefore beat, on the same seeded samples, a G built for another
+    # stencil D(k) or another assignment window U(k): a mistake in D4, in the
+    # NGP window or in the alias sum of either makes the mismatched G win.
+    n, a = 32, 4.0
+    scheme, fds = used
+
+    def err(G):
+        _, acc, d, dist = _point_mass_samples(oracle, scheme, fds, G, n, a, _rng(21), 12, rmin=0.5)
+        R = np.array([oracle.reference_force(1, float(x), a) for x in dist]) - dist / (3 * n ** 3)
+        want = -R[:, None] * d / dist[:, None]
+        return np.sqrt(((acc - want) ** 2).sum() / (R ** 2).sum())
+    good = err(oracle.green_grid(2, scheme, fds, a, (n, n, n)))
+    other = err(oracle.green_grid(2, built[0], built[1], a, (n, n, n)))
+    assert good < 0.9 * other, (used, built, good, other)
+
+
+@pytest.mark.parametrize("fds", [0, 1])
+def test_fd_d_is_the_field_stencil_on_plane_waves(oracle, fds):
+    # D(k) inside GreenOptimal (D2Fourier / D4Fourier, source/greensFunctions.cpp:
+    # 70-89) must be the symbol of the field stencil (source/pmMethod.cpp:355-365):
+    # for phi = cos(k x) the stencil gives E = -grad phi = d(k) sin(k x).
+    n = 32
+    x = np.arange(n, dtype=np.float64)
+    for m in (1, 3, 7, 12, 16):
+        k = 2 * np.pi * m / n
+        d = oracle.fd_d(fds, k)
+        for ax in range(3):
+            shape = [1, 1, 1]
+            shape[2 - ax] = n
+            phi = np.broadcast_to(np.cos(k * x).reshape(shape), (n, n, n)).copy()
+            E = oracle.field(phi, fds)
+            want = np.broadcast_to((d * np.sin(k * x)).reshape(shape), (n, n, n))
+            assert np.allclose(E[..., ax], want, atol=1e-12), (fds, m, ax)
+            others = [c for c in range(3) if c != ax]
+            assert np.allclose(E[..., others], 0.0, atol=1e-12)
+    # the symbols themselves: 2-point sin k; 4-point (8 sin k - sin 2k)/6
+    assert abs(oracle.fd_d(0, 0.7) - np.sin(0.7)) < 1e-15
+    assert abs(oracle.fd_d(1, 0.7) - (8 * np.sin(0.7) - np.sin(1.4)) / 6) < 1e-15
 
 
 # --------------------------------------------------------------------------
@@ -366,6 +432,24 @@ def test_sr_table_vanishes_beyond_diameter(oracle):
     r = np.sqrt(np.arange(500) * d2)
     assert np.all(np.abs(t[r >= a * 1.0001]) < 1e-6 * np.abs(t[1:]).max())
     assert t[0] == 0.0
+
+
+@pytest.mark.parametrize("cs,eps", [(0, 0.4), (1, 0.4), (1, 1.1)])
+def test_sr_reference_table_with_softening(oracle, cs, eps):
+    # initSRForceTable (source/p3mMethod.cpp:275-294) divides by sqrt(r^2+eps^2)
+    # rather than r.  Each entry is therefore the direct pair factor F (pinned
+    # above by the two-cloud integration and the softened far field) times
+    # r/sqrt(r^2+eps^2): a dropped softening term, a wrong sign of R or an
+    # off-by-one in r^2 = i * delta^2 fails this.
+    rc, a = 3.0, 2.0
+    t, d2 = oracle.sr_table(rc, a, eps, cs)
+    assert abs(d2 - np.float32(rc * rc) / np.float32(499)) <= 1e-7 * d2
+    r2 = np.arange(500, dtype=np.float64) * np.float64(np.float32(d2))
+    r = np.sqrt(r2)
+    F = oracle.sr_pair_factor(cs, a, eps, r2[1:])
+    want = F * r[1:] / np.sqrt(r2[1:] + eps * eps)
+    assert t[0] == 0.0
+    assert np.allclose(t[1:], want, rtol=2e-5, atol=2e-6 * np.abs(want).max())
 
 
 def _brute_sr(pos, m, rc, a, cs):
