@@ -120,10 +120,28 @@ def measured_learned():
             **{k: large[k] for k in ("plain_pm", "learned_pm", "plain_p3m") if k in large}}
 
 
-def cpu_baseline(w, pos, vel, m, pairs_total, budget_s=15.0):
-    """The oracle (CPU restatement of the reference path) on this host."""
+def host_threads():
+    """Threads for the CPU baseline: this process's CPU share (the affinity
+    mask, capped by OMP_NUM_THREADS when that is set), and the counts behind it."""
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = nproc
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = affinity
+    if omp and omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    return threads, {"nproc": nproc, "affinity": affinity, "omp_num_threads": omp}
+
+
+def cpu_baseline(w, pos, vel, m, pairs_total, budget_s=15.0, nsamples=3):
+    """The oracle (CPU restatement of the reference path) on this host: the
+    full PM step, plus the short-range pass on `nsamples` disjoint random sets
+    of chaining cells, each scaled to the whole pass by its pair count; the
+    spread of the per-sample estimates is the sampling error."""
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads, counts = host_threads()
     O.set_num_threads(threads)
     grid = w.grid
     t0 = time.perf_counter()
@@ -132,31 +150,53 @@ def cpu_baseline(w, pos, vel, m, pairs_total, budget_s=15.0):
     t0 = time.perf_counter()
     O.pm_accel(pos, m, grid, w.interpolation, w.finite_diff, G, wrap=True)
     t_pm = time.perf_counter() - t0
-    t_sr, sample = 0.0, "full PM step"
+    t_sr, sample, sr_err = 0.0, "full PM step", None
     if w.p3m:
         H = w.H
         box_orig = np.array([w.box] * 3, np.float32)
         dims = O.p3m_mesh_dims(box_orig, w.rc_cells * H)
         ncell = dims[0] * dims[1] * dims[2]
-        rng = np.random.default_rng(11)
-        frac, dt, p_s, mask = 0.002, 0.0, 0, None
-        while True:
-            mask = (rng.random(ncell) < frac).astype(np.uint8)
+        perm = np.random.default_rng(11).permutation(ncell)
+
+        def run(cells):
+            mask = np.zeros(ncell, np.uint8)
+            mask[cells] = 1
             t0 = time.perf_counter()
-            _, p_s = O.p3m_short_range(pos, m, box_orig, w.rc_cells * H, H, w.rc_cells, w.a_cells, w.softening,
-                                       w.cloud_shape, use_table=w.sr_table, ysort=True, cell_mask=mask)
-            dt = time.perf_counter() - t0
-            if (dt > budget_s / 3 and p_s > 0) or frac >= 1.0:
+            _, p = O.p3m_short_range(pos, m, box_orig, w.rc_cells * H, H, w.rc_cells, w.a_cells, w.softening,
+                                     w.cloud_shape, use_table=w.sr_table, ysort=True, cell_mask=mask)
+            return time.perf_counter() - t0, p
+
+        # pilot: grow a sample until it takes a measurable time, to size the others
+        k = max(1, ncell // 1000)
+        while True:
+            dt, p = run(perm[:k])
+            if (dt > 0.5 and p > 0) or k >= ncell // (nsamples + 1):
                 break
-            frac = min(1.0, frac * (10.0 if p_s == 0 else max(2.0, (budget_s / 2) / max(dt, 1e-3))))
-        if p_s == 0:
+            k = min(ncell // (nsamples + 1), k * 4)
+        if p == 0:
             raise RuntimeError("short-range sample found no pairs")
-        t_sr = dt * (pairs_total / max(p_s, 1))
-        sample = (f"full PM step ({t_pm:.2f}s) + short range on {mask.sum()} of {ncell} chaining cells "
-                  f"({p_s:.3e} of {pairs_total:.3e} ordered pairs, {dt:.2f}s), scaled by pair count")
+        per_sample = budget_s / nsamples
+        k = int(min(ncell // (nsamples + 1), max(1, k * per_sample / max(dt, 1e-3))))
+        est, used_pairs, used_cells, used_s = [], 0, 0, 0.0
+        for j in range(nsamples):
+            cells = perm[ncell - (j + 1) * k: ncell - j * k]  # disjoint from each other and from the pilot
+            dt, p = run(cells)
+            if p == 0:
+                continue
+            est.append(dt * pairs_total / p)
+            used_pairs, used_cells, used_s = used_pairs + p, used_cells + len(cells), used_s + dt
+        if not est:
+            raise RuntimeError("short-range samples found no pairs")
+        est = np.array(est)
+        t_sr = float(est.mean())
+        sr_err = float(est.std(ddof=1) / np.sqrt(len(est)) / t_sr) if len(est) > 1 else None
+        sample = (f"full PM step ({t_pm:.2f}s) + short range on {len(est)} disjoint random samples of {k} of "
+                  f"{ncell} chaining cells ({used_pairs:.3e} of {pairs_total:.3e} ordered pairs in all, "
+                  f"{used_s:.1f}s), each scaled to the whole pass by its pair count")
     t_step = t_pm + t_sr
     return {"value": len(m) / t_step, "unit": "particle-updates/s", "cores": threads, "kind": "port",
-            "sample": sample, "s_per_step": t_step, "green_init_s": t_green}
+            "sample": sample, "s_per_step": t_step, "pm_s": t_pm, "sr_s": t_sr,
+            "sr_sampling_rel_err": sr_err, "green_init_s": t_green, **counts}
 
 
 def main():
@@ -212,18 +252,24 @@ def main():
     pm.profile_reset()
     pm.synchronize()
     t0 = time.perf_counter()
-    pm.run(args.steps, initial=False, p3m=w.p3m, sync=False)
-    pm.synchronize()
+    # the run ends with a 4-byte read of the escape word (the first iteration
+    # whose drift left the box) and a stream synchronisation
+    done = pm.run(args.steps, initial=False, p3m=w.p3m, sync=True)
     t1 = time.perf_counter()
     stages = pm.profile_read()
     pm.profile(False)
-    done = pm.run(0, initial=False, p3m=w.p3m)  # iterations_done check: 0 requested
+    escaped = pm.escaped()
+    if escaped or done != args.steps:
+        # a particle outside the box stops every later kernel of the run, so
+        # the timed steps would not all have been computed
+        log(f"[bench] invalid run: {done} of {args.steps} iterations done, escaped={escaped}")
+        sys.exit(3)
     pairs1, tested1 = pm.p3m_count_pairs(tested=True) if w.p3m else (0, 0)
     pairs = 0.5 * (pairs0 + pairs1)
     tested = 0.5 * (tested0 + tested1)
     elapsed = t1 - t0
     value = w.n * args.steps / elapsed
-    log(f"[bench] {args.steps} steps in {elapsed:.4f}s -> {value:.4e} upd/s; escaped={pm.escaped()}")
+    log(f"[bench] {args.steps} steps in {elapsed:.4f}s -> {value:.4e} upd/s; iterations={done} escaped={escaped}")
 
     per_step = {k: v[0] / max(1, args.steps) for k, v in stages.items()}
     if w.p3m:
@@ -278,9 +324,10 @@ def main():
         "stage_ms_per_step": per_step,
         "hbm_stages": hbm,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "iterations": done,
+        "escaped": escaped,
     }
     print(json.dumps(out), flush=True)
-    _ = done
 
 
 if __name__ == "__main__":

@@ -33,6 +33,7 @@
 #define ORC_NGP 0
 #define ORC_CIC 1
 #define ORC_TSC 2
+#define ORC_TSC_CENTRED 3 /* textbook TSC (nearest grid point as the centre); not in the reference */
 
 #define ORC_TWO_POINT 0
 #define ORC_FOUR_POINT 1
@@ -106,6 +107,13 @@ static inline int axis_stencil(int scheme, double x, int* first, double w[3]) {
     w[1] = d;
     return 2;
   }
+  if (scheme == ORC_TSC_CENTRED) {
+    /* Hockney & Eastwood's TSC: the same per-axis quadratics about the
+       nearest grid point, d = x - round(x) in [-1/2, 1/2), so no weight is
+       negative (an extension; the reference anchors at floor(x), Q1) */
+    c = (int)floor(x + 0.5);
+    d = x - c;
+  }
   *first = c - 1;
   w[0] = 0.5 * tsc_f(d, -1);
   w[1] = 0.5 * tsc_f(d, 0);
@@ -122,7 +130,7 @@ static inline int axis_stencil(int scheme, double x, int* first, double w[3]) {
  */
 int orc_deposit(int64_t n, const float* pos, const float* mass, int nx, int ny, int nz,
                 int scheme, int wrap, double* rho) {
-  if (scheme < 0 || scheme > 2) return -2;
+  if (scheme < 0 || scheme > 3) return -2;
   int64_t m = (int64_t)nx * ny * nz;
   memset(rho, 0, sizeof(double) * (size_t)m);
   int bad = 0;
@@ -412,7 +420,8 @@ static double green_at(int gf, int is, int fds, double a, int kx, int ky, int kz
 /* PMMethod::initGreensFunction, source/pmMethod.cpp:164-185: G on the full
    grid, G[kx + nx*(ky + ny*kz)], unfolded indices, no symmetrisation. */
 int orc_green_grid(int gf, int is, int fds, double a, int nx, int ny, int nz, double* G) {
-  if (gf < 0 || gf > 3 || is < 0 || is > 2 || fds < 0 || fds > 1) return -2;
+  if (gf < 0 || gf > 3 || is < 0 || is > 3 || fds < 0 || fds > 1) return -2;
+  if (is == ORC_TSC_CENTRED) is = ORC_TSC; /* the same assignment function, only re-centred */
 #pragma omp parallel for collapse(2) schedule(dynamic, 1)
   for (int kz = 0; kz < nz; ++kz)
     for (int ky = 0; ky < ny; ++ky)

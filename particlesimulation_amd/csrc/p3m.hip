@@ -96,7 +96,10 @@ __global__ void k_sr_scatter(const float4* __restrict__ pm, const float4* __rest
       if (odd_ids[mid] < id) lo = mid + 1;
       else hi = mid;
     }
-    odd_pos[lo] = (int32_t)d;
+    // set_particles lists every particle off m0, so the search finds the id;
+    // the guard keeps a mass changed behind the list's back from writing
+    // another particle's slot
+    if (odd_ids[lo] == id) odd_pos[lo] = (int32_t)d;
   }
 }
 

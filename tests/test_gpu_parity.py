@@ -224,14 +224,18 @@ def test_p3m_cell_division_keeps_the_pair_set(gpu_lib, oracle, div, chunking):
     assert _rel(a_d, oracle.pm_accel(pos, m, grid, 2, 0, G) + ref_sr) <= FORCE_TOL
 
 
-@pytest.mark.parametrize("use_table", [0, 1, 2])
-@pytest.mark.parametrize("n_heavy", [0, 5])
-def test_p3m_common_mass_matches_oracle(gpu_lib, oracle, use_table, n_heavy):
-    # particles of one mass (but for a few heavy ones) take the pair loop
-    # without a mass multiply and a per-i correction for the heavy ones
+@pytest.mark.parametrize("use_table,n_heavy,m0", [(0, 0, 1.0), (1, 0, 1.0), (2, 0, 1.0), (0, 5, 1.0), (1, 5, 1.0),
+                                                  (2, 5, 1.0), (2, 64, 1.0), (2, 65, 1.0), (1, 64, 1.0),
+                                                  (2, 40, 0.0), (0, 64, 0.0)])
+def test_p3m_common_mass_matches_oracle(gpu_lib, oracle, use_table, n_heavy, m0):
+    # particles of one mass m0 / n (but for a few heavy ones) take the pair
+    # loop without a mass multiply and a per-i correction for the heavy ones;
+    # 64 heavy particles is the most the correction list holds, 65 takes the
+    # per-pair mass multiply, and a zero common mass leaves only the heavy
+    # particles' forces
     grid = (64, 64, 64)
     pos, vel, _ = _particles(6000, grid, 41, clustered=True)
-    m = np.full(len(pos), np.float32(1.0 / len(pos)))
+    m = np.full(len(pos), np.float32(m0 / len(pos)))
     m[_rng(42).choice(len(pos), n_heavy, replace=False)] = np.float32(20.0 / len(pos))
     pm = _make(gpu_lib, grid, 2, 0, 2, a=3.0)
     pm.set_particles(pos, vel, m)
