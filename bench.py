@@ -8,14 +8,23 @@ FFT x G, field, interpolation, short range for P3M), kick.  Inputs are
 resident in HBM before timing starts; the timed region is exactly K steps
 between two stream synchronisations.  Prints one JSON line on stdout.
 
-The line also carries
+The line carries, for the headline workload (the 1M-particle P3M disk on
+128^3 with a 3-cell cutoff):
   roofline      the dominant kernel's achieved rate vs the MI355X peak,
                 measured with HIP events on the library's stream,
   cpu_baseline  the CPU restatement of the reference (oracle/, OpenMP) timed
-                on a bounded sample of the same workload on this host,
+                on this host: one whole step (PM and the whole short-range
+                pass),
   force_error   RMS and mean relative error of the GPU force against an FP64
-                direct sum on a seeded subsample,
-  learned_vs_plain  the recorded learned-vs-plain PM comparison (profiles/).
+                direct sum (host CPU) on a seeded subsample of targets,
+and two more measured blocks:
+  at_error_target   the same disk and grid at the smallest cutoff whose
+                force error is below 1e-3 (textbook TSC, 4-point gradient,
+                a = rc = 5 cells): throughput, roofline, error, CPU baseline,
+  learned_vs_plain  the learned influence-function correction trained on the
+                GPU within this run, then plain PM, learned PM and P3M on the
+                4M-particle clustered workload on 256^3: error against the
+                periodic direct sum and throughput.
 """
 from __future__ import annotations
 
@@ -34,7 +43,9 @@ METRIC = "particle-updates/s, P3M 1M on 128^3 at fixed force err; learned vs pla
 PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 vector peak
 PEAK_HBM_GBS = 8000.0
 SR_FLOP_PER_PAIR = 20      # per ordered pair within the cutoff
-GC = 1.0 / (4.0 * np.pi)
+SCHEMES = ["NGP", "CIC", "TSC", "TSC_CENTRED"]
+FDS = ["TWO_POINT", "FOUR_POINT"]
+GREENS = ["DISCRETE_LAPLACIAN", "S1_OPTIMAL", "S2_OPTIMAL", "POOR_MAN"]
 
 
 def log(*a):
@@ -50,78 +61,8 @@ def load():
     return P
 
 
-def make_context(P, w, pos, vel, m):
-    box = (0, 0, 0) if w.periodic else w.grid
-    pm = P.PM(w.grid, interpolation=w.interpolation, finite_diff=w.finite_diff,
-              greens_function=w.greens_function, particle_diameter=w.a_cells, box=box)
-    pm.set_particles(pos, vel, m)
-    pm.init_greens()
-    if w.p3m:
-        pm.p3m_configure(cutoff=w.rc_cells, particle_diameter=w.a_cells, softening=w.softening,
-                         cloud_shape=w.cloud_shape, use_table=w.sr_table, box=w.grid)
-    return pm
-
-
-def force_error(P, w, pm, pos, m, n_targets=512, seed=7):
-    """e_rms = sqrt(sum|F - F_ds|^2 / sum|F_ds|^2) and the reference's mean relative
-    error (script/pm_accuracy.py:44-48) against a direct sum over all particles, on a
-    seeded subsample of targets.  Isolated workloads: FP64 Newtonian direct sum (numpy).
-    Periodic workloads: the GPU's periodic direct sum (nearest image + Ewald
-    correction, FP32 with compensated sums; checked against the FP64 oracle)."""
-    pm.pm_step()
-    if w.p3m:
-        pm.p3m_short_range()
-    _, _, acc = pm.get_particles()
-    rng = np.random.default_rng(seed)
-    t = rng.choice(len(m), size=min(n_targets, len(m)), replace=False)
-    if w.periodic:
-        ref = P.direct_sum(pos, m, pos[t], box=w.grid).astype(np.float64)
-        kind = "gpu fp32 periodic (Ewald) direct sum"
-    else:
-        x = pos.astype(np.float64)
-        mm = m.astype(np.float64)
-        ref = np.zeros((len(t), 3))
-        for k, i in enumerate(t):
-            d = x[i] - x
-            r2 = (d * d).sum(1)
-            r2[i] = np.inf
-            ref[k] = -GC * ((mm / (r2 * np.sqrt(r2)))[:, None] * d).sum(0)
-        kind = "fp64 direct sum"
-    diff = acc[t].astype(np.float64) - ref
-    e_rms = float(np.sqrt((diff ** 2).sum() / (ref ** 2).sum()))
-    mean_rel = float(np.mean(np.linalg.norm(diff, axis=1) / (np.linalg.norm(ref, axis=1) + 1e-30)))
-    return {"e_rms": e_rms, "mean_rel": mean_rel, "targets": int(len(t)), "reference": kind}
-
-
-def measured_traffic(workload):
-    """HBM bytes per launch of the short-range kernel on this workload, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE measurement (profiles/), or None."""
-    path = os.path.join(ROOT, "profiles", f"k_sr_{workload}_traffic.json")
-    if not os.path.exists(path):
-        return None, None
-    with open(path) as f:
-        d = json.load(f)
-    return d["traffic_bytes_per_launch"], f"profiles/k_sr_{workload}_traffic.json ({d['kernel']})"
-
-
-def measured_learned():
-    """The learned-vs-plain comparison (error against the periodic direct sum
-    and throughput of plain PM, learned PM and P3M on the 4M-particle clumps
-    workload), from the committed measurement in profiles/, or None.  Training
-    takes tens of seconds, so the bench reports the recorded run."""
-    path = os.path.join(ROOT, "profiles", "learned_pm_clumps_4M_256.json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        d = json.load(f)
-    large = d["large"]
-    return {"source": f"profiles/learned_pm_clumps_4M_256.json ({d['command']})", "workload": large["workload"],
-            "targets": large["targets"], "reference": large["reference"],
-            **{k: large[k] for k in ("plain_pm", "learned_pm", "plain_p3m") if k in large}}
-
-
 def host_threads():
-    """Threads for the CPU baseline: this process's CPU share (the affinity
+    """Threads for the host-side work: this process's CPU share (the affinity
     mask, capped by OMP_NUM_THREADS when that is set), and the counts behind it."""
     nproc = os.cpu_count() or 1
     try:
@@ -135,11 +76,152 @@ def host_threads():
     return threads, {"nproc": nproc, "affinity": affinity, "omp_num_threads": omp}
 
 
-def cpu_baseline(w, pos, vel, m, pairs_total, budget_s=15.0, nsamples=3):
-    """The oracle (CPU restatement of the reference path) on this host: the
-    full PM step, plus the short-range pass on `nsamples` disjoint random sets
-    of chaining cells, each scaled to the whole pass by its pair count; the
-    spread of the per-sample estimates is the sampling error."""
+def workload_config(w, seed):
+    return {"workload": w.name, "n": w.n, "grid": list(w.grid), "box": w.box,
+            "interpolation": SCHEMES[w.interpolation], "finite_diff": FDS[w.finite_diff],
+            "greens_function": GREENS[w.greens_function], "a_cells": w.a_cells, "cutoff_cells": w.rc_cells,
+            "dt": w.dt, "seed": seed, "p3m": w.p3m, "periodic": w.periodic}
+
+
+def make_context(P, w, pos, vel, m):
+    box = (0, 0, 0) if w.periodic else w.grid
+    pm = P.PM(w.grid, interpolation=w.interpolation, finite_diff=w.finite_diff,
+              greens_function=w.greens_function, particle_diameter=w.a_cells, box=box)
+    pm.set_particles(pos, vel, m)
+    pm.init_greens()
+    if w.p3m:
+        pm.p3m_configure(cutoff=w.rc_cells, particle_diameter=w.a_cells, softening=w.softening,
+                         cloud_shape=w.cloud_shape, use_table=w.sr_table, box=w.grid)
+    return pm
+
+
+class DirectSum:
+    """Direct-sum accelerations of a seeded target subsample, computed once
+    per particle set and shared by the workloads that use it.  Isolated
+    workloads: FP64 on the host CPU over every particle.  Periodic ones: the
+    GPU's periodic sum (nearest image + Ewald correction, FP32 with
+    compensated sums) on every target, cross-checked by the FP64 host sum on
+    the first `fp64_check` targets."""
+
+    def __init__(self, P, pos, m, n_targets, seed=7, box=None, fp64_check=256):
+        threads, _ = host_threads()
+        rng = np.random.default_rng(seed)
+        self.targets = np.sort(rng.choice(len(m), size=min(n_targets, len(m)), replace=False))
+        t0 = time.perf_counter()
+        if box is None:
+            self.ref = P.direct_sum_host(pos, m, pos[self.targets], threads=threads)
+            self.kind, self.check = "fp64 direct sum (host)", None
+        else:
+            self.ref = P.direct_sum(pos, m, pos[self.targets], box=box).astype(np.float64)
+            k = self.targets[:fp64_check]
+            ref64 = P.direct_sum_host(pos, m, pos[k], box=box, threads=threads)
+            self.check = {"targets": int(len(k)),
+                          "rms_rel_fp32_vs_fp64": float(np.linalg.norm(self.ref[:len(k)] - ref64)
+                                                        / np.linalg.norm(ref64))}
+            self.kind = "gpu fp32 periodic (Ewald) direct sum, checked against the fp64 host sum"
+        self.seconds = time.perf_counter() - t0
+        self.threads = threads
+
+    def error(self, acc):
+        """e_rms = sqrt(sum|F - F_ds|^2 / sum|F_ds|^2) and the reference's mean
+        relative error (script/pm_accuracy.py:44-48)."""
+        diff = np.asarray(acc, np.float64)[self.targets] - self.ref
+        e_rms = float(np.sqrt((diff ** 2).sum() / (self.ref ** 2).sum()))
+        mean_rel = float(np.mean(np.linalg.norm(diff, axis=1) / (np.linalg.norm(self.ref, axis=1) + 1e-30)))
+        out = {"e_rms": e_rms, "mean_rel": mean_rel, "targets": int(len(self.targets)), "reference": self.kind,
+               "direct_sum_s": self.seconds, "host_threads": self.threads}
+        if self.check:
+            out["fp64_check"] = self.check
+        return out
+
+
+def force_of(pm, w):
+    pm.pm_step()
+    if w.p3m:
+        pm.p3m_short_range()
+    return pm.get_particles()[2]
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch of the short-range kernel on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE measurement (profiles/), or None."""
+    path = os.path.join(ROOT, "profiles", f"k_sr_{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        d = json.load(f)
+    return d["traffic_bytes_per_launch"], f"profiles/k_sr_{workload}_traffic.json ({d['kernel']})"
+
+
+def measure(pm, w, steps, warmup, pos, vel, m):
+    """Warm-up, then exactly `steps` run-loop iterations timed between two
+    stream synchronisations; per-stage times from HIP events on the library
+    stream, roofline of the dominant kernel.  Exits when the run escaped."""
+    pm.set_particles(pos, vel, m)
+    pm.run(warmup, initial=True, p3m=w.p3m)
+    pairs0, tested0 = pm.p3m_count_pairs(tested=True) if w.p3m else (0, 0)
+    pm.profile(True)
+    pm.profile_reset()
+    pm.synchronize()
+    t0 = time.perf_counter()
+    # the run ends with a 4-byte read of the escape word (the first iteration
+    # whose drift left the box) and a stream synchronisation
+    done = pm.run(steps, initial=False, p3m=w.p3m, sync=True)
+    t1 = time.perf_counter()
+    stages = pm.profile_read()
+    pm.profile(False)
+    escaped = pm.escaped()
+    if escaped or done != steps:
+        # a particle outside the box stops every later kernel of the run, so
+        # the timed steps would not all have been computed
+        log(f"[bench] invalid run of {w.name}: {done} of {steps} iterations done, escaped={escaped}")
+        sys.exit(3)
+    pairs1, tested1 = pm.p3m_count_pairs(tested=True) if w.p3m else (0, 0)
+    pairs = 0.5 * (pairs0 + pairs1)
+    tested = 0.5 * (tested0 + tested1)
+    elapsed = t1 - t0
+    value = w.n * steps / elapsed
+    log(f"[bench] {w.name}: {steps} steps in {elapsed:.4f}s -> {value:.4e} upd/s; iterations={done} "
+        f"escaped={escaped}")
+    per_step = {k: v[0] / max(1, steps) for k, v in stages.items()}
+    roofline = None
+    if w.p3m:
+        ms, cnt = stages["sr_pairs"]
+        avg_s = ms / cnt * 1e-3
+        achieved = SR_FLOP_PER_PAIR * pairs / avg_s / 1e12
+        roofline = {"bound": "valu", "kernel": "k_sr (short-range pairs)", "achieved": achieved,
+                    "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                    "traffic": None, "avg_launch_ms": avg_s * 1e3, "ordered_pairs": pairs,
+                    "tested_pairs": tested, "flop_per_pair": SR_FLOP_PER_PAIR,
+                    "sr_table_mode": w.sr_table, "sr_cell_division": pm.get_tuning()[3],
+                    "sr_cells": dict(zip(("div_x", "div_y", "div_z", "cells"), pm.p3m_cell_geometry()))}
+        roofline["traffic"], roofline["traffic_source"] = measured_traffic(w.name)
+    # algorithmic-byte rates of the bandwidth-bound stages (per launch); the
+    # interpolation reads x (16 B) and writes a (16 B) per particle, and in a
+    # PM step also reads and writes v and writes x (48 B more)
+    nx, ny, nz = w.grid
+    M, K, n = nx * ny * nz, (nx // 2 + 1) * ny * nz, w.n
+    alg = {"deposit": 16 * n + 4 * M, "fft_x_r2c": 4 * M + 8 * K, "fft_y_fwd": 16 * K,
+           "fft_z_green": 20 * K, "fft_y_inv": 16 * K, "fft_x_c2r": 8 * K + 4 * M,
+           "interp": 4 * M + (32 if w.p3m else 80) * n}
+    hbm = {}
+    for k, b in alg.items():
+        if k in stages and stages[k][1]:
+            s = stages[k][0] / stages[k][1] * 1e-3
+            hbm[k] = {"ms": s * 1e3, "GB/s": b / s / 1e9, "frac": b / s / 1e9 / PEAK_HBM_GBS}
+    return {"value": value, "ms_per_step": elapsed / steps * 1e3, "steps": steps, "iterations": done,
+            "escaped": escaped, "roofline": roofline, "stage_ms_per_step": per_step, "hbm_stages": hbm,
+            "pairs": pairs}
+
+
+def cpu_baseline(w, pos, m, sr_scale_from=None):
+    """The oracle (CPU restatement of the reference path, OpenMP) on this
+    host: one whole step, the PM force and the whole short-range pass (the
+    reference's chaining mesh with y-sorted cells and the accurate table).
+    With sr_scale_from = (seconds, pairs) of a measured pass on the same
+    particles, the short-range time is that pass's time per ordered pair
+    times this workload's pair count instead (the pass on a larger cutoff
+    would take minutes); the sample says so."""
     from oracle import oracle as O
     threads, counts = host_threads()
     O.set_num_threads(threads)
@@ -150,53 +232,54 @@ def cpu_baseline(w, pos, vel, m, pairs_total, budget_s=15.0, nsamples=3):
     t0 = time.perf_counter()
     O.pm_accel(pos, m, grid, w.interpolation, w.finite_diff, G, wrap=True)
     t_pm = time.perf_counter() - t0
-    t_sr, sample, sr_err = 0.0, "full PM step", None
-    if w.p3m:
+    t_sr, pairs, sample = 0.0, 0, f"one whole PM step ({t_pm:.2f}s)"
+    if w.p3m and sr_scale_from is None:
         H = w.H
         box_orig = np.array([w.box] * 3, np.float32)
-        dims = O.p3m_mesh_dims(box_orig, w.rc_cells * H)
-        ncell = dims[0] * dims[1] * dims[2]
-        perm = np.random.default_rng(11).permutation(ncell)
-
-        def run(cells):
-            mask = np.zeros(ncell, np.uint8)
-            mask[cells] = 1
-            t0 = time.perf_counter()
-            _, p = O.p3m_short_range(pos, m, box_orig, w.rc_cells * H, H, w.rc_cells, w.a_cells, w.softening,
-                                     w.cloud_shape, use_table=w.sr_table, ysort=True, cell_mask=mask)
-            return time.perf_counter() - t0, p
-
-        # pilot: grow a sample until it takes a measurable time, to size the others
-        k = max(1, ncell // 1000)
-        while True:
-            dt, p = run(perm[:k])
-            if (dt > 0.5 and p > 0) or k >= ncell // (nsamples + 1):
-                break
-            k = min(ncell // (nsamples + 1), k * 4)
-        if p == 0:
-            raise RuntimeError("short-range sample found no pairs")
-        per_sample = budget_s / nsamples
-        k = int(min(ncell // (nsamples + 1), max(1, k * per_sample / max(dt, 1e-3))))
-        est, used_pairs, used_cells, used_s = [], 0, 0, 0.0
-        for j in range(nsamples):
-            cells = perm[ncell - (j + 1) * k: ncell - j * k]  # disjoint from each other and from the pilot
-            dt, p = run(cells)
-            if p == 0:
-                continue
-            est.append(dt * pairs_total / p)
-            used_pairs, used_cells, used_s = used_pairs + p, used_cells + len(cells), used_s + dt
-        if not est:
-            raise RuntimeError("short-range samples found no pairs")
-        est = np.array(est)
-        t_sr = float(est.mean())
-        sr_err = float(est.std(ddof=1) / np.sqrt(len(est)) / t_sr) if len(est) > 1 else None
-        sample = (f"full PM step ({t_pm:.2f}s) + short range on {len(est)} disjoint random samples of {k} of "
-                  f"{ncell} chaining cells ({used_pairs:.3e} of {pairs_total:.3e} ordered pairs in all, "
-                  f"{used_s:.1f}s), each scaled to the whole pass by its pair count")
+        t0 = time.perf_counter()
+        _, pairs = O.p3m_short_range(pos, m, box_orig, w.rc_cells * H, H, w.rc_cells, w.a_cells, w.softening,
+                                     w.cloud_shape, use_table=w.sr_table, ysort=True)
+        t_sr = time.perf_counter() - t0
+        sample = (f"one whole step: PM ({t_pm:.2f}s) + the whole short-range pass ({pairs:.3e} ordered pairs, "
+                  f"{t_sr:.1f}s)")
+    elif w.p3m:
+        t_ref, pairs_ref, pairs = sr_scale_from
+        t_sr = t_ref * pairs / pairs_ref
+        sample = (f"PM step measured ({t_pm:.2f}s) + short-range pass estimated as {pairs:.3e} ordered pairs at "
+                  f"the measured time per pair of the 3-cell pass on the same particles ({t_ref:.1f}s for "
+                  f"{pairs_ref:.3e} pairs)")
     t_step = t_pm + t_sr
     return {"value": len(m) / t_step, "unit": "particle-updates/s", "cores": threads, "kind": "port",
-            "sample": sample, "s_per_step": t_step, "pm_s": t_pm, "sr_s": t_sr,
-            "sr_sampling_rel_err": sr_err, "green_init_s": t_green, **counts}
+            "sample": sample, "s_per_step": t_step, "pm_s": t_pm, "sr_s": t_sr, "sr_pairs": pairs,
+            "green_init_s": t_green, **counts}
+
+
+def learned_live(P, seed, budget_steps=150, snapshots=24, n_targets=65536):
+    """Train the correction on the GPU in this run and compare plain PM,
+    learned PM and P3M on the 4M-particle clustered workload on 256^3."""
+    from particlesimulation_amd import learned as Lm
+    t0 = time.perf_counter()
+    trainers = []
+    for gi, g in enumerate((64, 128)):
+        seeds = range(1000 * (gi + 1), 1000 * (gi + 1) + snapshots)
+        trainers.append(Lm.build_trainer((g, g, g), seeds, 65536))
+    t_data = time.perf_counter() - t0
+    params = P.mlp_init(seed)
+    t0 = time.perf_counter()
+    params, hist = P.train(trainers, params, budget_steps, 2e-3)
+    t_train = time.perf_counter() - t0
+    for t in trainers:
+        t.close()
+    large = Lm.evaluate_large(params, n_targets=n_targets)
+    pm_, lp, p3 = large["plain_pm"], large["learned_pm"], large["plain_p3m"]
+    out = {"training": {"grids": [64, 128], "snapshots_per_grid": snapshots, "particles": 65536,
+                        "adam_steps": budget_steps, "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
+                        "data_s": t_data, "train_s": t_train},
+           **large,
+           "learned_error_over_plain": lp["e_rms"] / pm_["e_rms"],
+           "learned_throughput_over_p3m": lp["updates_per_s"] / p3["updates_per_s"],
+           "goal_met": bool(lp["e_rms"] <= p3["e_rms"] and lp["updates_per_s"] >= 2 * p3["updates_per_s"])}
+    return out
 
 
 def main():
@@ -206,8 +289,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="p3m_disk_1M_128")
     ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--targets", type=int, default=65536, help="direct-sum targets of the force error")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--no-at-error", action="store_true", help="skip the block at the 1e-3 error target")
+    ap.add_argument("--no-learned", action="store_true", help="skip the live learned-vs-plain block")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings) and exit")
@@ -242,90 +328,62 @@ def main():
         k = list(pm.get_tuning())
         k[3] = args.sr_div
         pm.set_tuning(*k)
-    err = None if args.no_error else force_error(P, w, pm, pos, m)
+    ds = None if args.no_error else DirectSum(P, pos, m, args.targets, box=w.grid if w.periodic else None)
+    err = ds.error(force_of(pm, w)) if ds else None
     if err:
         log(f"[bench] force error vs direct sum: {err}")
-    pm.set_particles(pos, vel, m)
-    pm.run(args.warmup, initial=True, p3m=w.p3m)
-    pairs0, tested0 = pm.p3m_count_pairs(tested=True) if w.p3m else (0, 0)
-    pm.profile(True)
-    pm.profile_reset()
-    pm.synchronize()
-    t0 = time.perf_counter()
-    # the run ends with a 4-byte read of the escape word (the first iteration
-    # whose drift left the box) and a stream synchronisation
-    done = pm.run(args.steps, initial=False, p3m=w.p3m, sync=True)
-    t1 = time.perf_counter()
-    stages = pm.profile_read()
-    pm.profile(False)
-    escaped = pm.escaped()
-    if escaped or done != args.steps:
-        # a particle outside the box stops every later kernel of the run, so
-        # the timed steps would not all have been computed
-        log(f"[bench] invalid run: {done} of {args.steps} iterations done, escaped={escaped}")
-        sys.exit(3)
-    pairs1, tested1 = pm.p3m_count_pairs(tested=True) if w.p3m else (0, 0)
-    pairs = 0.5 * (pairs0 + pairs1)
-    tested = 0.5 * (tested0 + tested1)
-    elapsed = t1 - t0
-    value = w.n * args.steps / elapsed
-    log(f"[bench] {args.steps} steps in {elapsed:.4f}s -> {value:.4e} upd/s; iterations={done} escaped={escaped}")
-
-    per_step = {k: v[0] / max(1, args.steps) for k, v in stages.items()}
-    if w.p3m:
-        ms, cnt = stages["sr_pairs"]
-        avg_s = ms / cnt * 1e-3
-        achieved = SR_FLOP_PER_PAIR * pairs / avg_s / 1e12
-        roofline = {"bound": "valu", "kernel": "k_sr (short-range pairs)", "achieved": achieved,
-                    "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                    "traffic": None, "avg_launch_ms": avg_s * 1e3, "ordered_pairs": pairs,
-                    "tested_pairs": tested, "flop_per_pair": SR_FLOP_PER_PAIR,
-                    "sr_table_mode": w.sr_table, "sr_cell_division": pm.get_tuning()[3],
-                    "sr_cells": dict(zip(("div_x", "div_y", "div_z", "cells"), pm.p3m_cell_geometry()))}
-        roofline["traffic"], roofline["traffic_source"] = measured_traffic(w.name)
-    else:
-        roofline = None
-    # algorithmic-byte rates of the bandwidth-bound stages (per launch); the
-    # interpolation reads x (16 B) and writes a (16 B) per particle, and in a
-    # PM step also reads and writes v and writes x (48 B more)
-    nx, ny, nz = w.grid
-    M, K, n = nx * ny * nz, (nx // 2 + 1) * ny * nz, w.n
-    alg = {"deposit": 16 * n + 4 * M, "fft_x_r2c": 4 * M + 8 * K, "fft_y_fwd": 16 * K,
-           "fft_z_green": 20 * K, "fft_y_inv": 16 * K, "fft_x_c2r": 8 * K + 4 * M,
-           "interp": 4 * M + (32 if w.p3m else 80) * n}
-    hbm = {}
-    for k, b in alg.items():
-        if k in stages and stages[k][1]:
-            s = stages[k][0] / stages[k][1] * 1e-3
-            hbm[k] = {"ms": s * 1e3, "GB/s": b / s / 1e9, "frac": b / s / 1e9 / PEAK_HBM_GBS}
+    res = measure(pm, w, args.steps, args.warmup, pos, vel, m)
+    pm.close()
 
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(w, pos, vel, m, pairs if w.p3m else 0)
+            cpu = cpu_baseline(w, pos, m)
             log(f"[bench] cpu baseline: {cpu}")
         except Exception as e:  # the baseline must not hide the GPU measurement
             log(f"[bench] cpu baseline failed: {e!r}")
+
+    at_err = None
+    if not args.no_at_error and w.name == ics.C2_P3M_DISK.name:
+        we = ics.C2_P3M_DISK_ERR
+        pe = make_context(P, we, pos, vel, m)
+        e_err = ds.error(force_of(pe, we)) if ds else None
+        log(f"[bench] {we.name} force error: {e_err}")
+        r = measure(pe, we, args.steps, args.warmup, pos, vel, m)
+        pe.close()
+        c = None
+        if cpu and cpu["sr_pairs"]:
+            c = cpu_baseline(we, pos, m, sr_scale_from=(cpu["sr_s"], cpu["sr_pairs"], r["pairs"]))
+        at_err = {"config": workload_config(we, args.seed), "value": r["value"], "unit": "particle-updates/s",
+                  "ms_per_step": r["ms_per_step"], "force_error": e_err,
+                  "error_target_met": bool(e_err and e_err["e_rms"] <= 1e-3), "roofline": r["roofline"],
+                  "stage_ms_per_step": r["stage_ms_per_step"], "cpu_baseline": c,
+                  "gpu_vs_cpu": (r["value"] / c["value"]) if c else None, "iterations": r["iterations"],
+                  "escaped": r["escaped"]}
+
+    learned = None
+    if not args.no_learned and w.name == ics.C2_P3M_DISK.name:
+        try:
+            learned = learned_live(P, args.seed)
+            log(f"[bench] learned vs plain: {learned}")
+        except Exception as e:
+            log(f"[bench] learned block failed: {e!r}")
+
     out = {
-        "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+        "metric": METRIC, "value": res["value"], "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": w.name, "n": w.n, "grid": list(w.grid), "box": w.box,
-                   "interpolation": ["NGP", "CIC", "TSC"][w.interpolation],
-                   "finite_diff": ["TWO_POINT", "FOUR_POINT"][w.finite_diff],
-                   "greens_function": ["DISCRETE_LAPLACIAN", "S1_OPTIMAL", "S2_OPTIMAL", "POOR_MAN"][
-                       w.greens_function],
-                   "a_cells": w.a_cells, "cutoff_cells": w.rc_cells, "dt": w.dt, "seed": args.seed,
-                   "p3m": w.p3m},
-        "roofline": roofline,
+        "config": workload_config(w, args.seed),
+        "roofline": res["roofline"],
         "cpu_baseline": cpu,
         "force_error": err,
-        "learned_vs_plain": measured_learned(),
-        "stage_ms_per_step": per_step,
-        "hbm_stages": hbm,
-        "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
-        "iterations": done,
-        "escaped": escaped,
+        "at_error_target": at_err,
+        "learned_vs_plain": learned,
+        "stage_ms_per_step": res["stage_ms_per_step"],
+        "hbm_stages": res["hbm_stages"],
+        "gpu_vs_cpu": (res["value"] / cpu["value"]) if cpu else None,
+        "iterations": res["iterations"],
+        "escaped": res["escaped"],
     }
     print(json.dumps(out), flush=True)
 

@@ -45,6 +45,12 @@ extern "C" {
 #define PMGPU_NGP 0
 #define PMGPU_CIC 1
 #define PMGPU_TSC 2
+/* An extension: TSC about the nearest grid point (Hockney & Eastwood's
+   assignment, whose weights are never negative).  PMGPU_TSC keeps the
+   reference's anchor at the floor cell (source/pmMethod.cpp:250-269), whose
+   centre weight 3/4 - d^2 is negative for d > sqrt(3)/2.  Same stencil,
+   Green's functions and errors otherwise. */
+#define PMGPU_TSC_CENTRED 3
 /* FiniteDiffScheme, include/pmConfig.h:4 */
 #define PMGPU_TWO_POINT 0
 #define PMGPU_FOUR_POINT 1
@@ -129,6 +135,12 @@ int pmgpu_set_particles(pmgpu_ctx* ctx, int64_t n, const float* pos, const float
 /* Replaces PMMethod::getParticles (include/pmMethod.h:33): copies the
    state back in the caller's particle order.  Any pointer may be NULL. */
 int pmgpu_get_particles(pmgpu_ctx* ctx, float* pos, float* vel, float* acc);
+/* Adds accelerations evaluated by the host (code units, caller's particle
+   order, n x 3) to those of the last force evaluation: the external field
+   term of PMMethod::updateAccelerations (source/pmMethod.cpp:384-390) for a
+   std::function the device cannot call.  Synchronises (acc is read before
+   the call returns). */
+int pmgpu_add_accelerations(pmgpu_ctx* ctx, const float* acc);
 /* Overwrites positions / velocities / accelerations (caller order) without
    changing masses; NULL leaves a field as it is.  Lets a host that edited
    its AoS mirror push it back (PMMethod::getParticles returns a mutable
@@ -337,6 +349,11 @@ int pmgpu_train(pmgpu_trainer** trainers, int32_t n_trainers, float* params, int
    condition of the FFT force step). */
 int pmgpu_direct_sum(int64_t n, const float* pos, const float* mass, int64_t nt, const float* tpos,
                      float softening, const float* box, float* out);
+/* The same sums on the host CPU in FP64 over `threads` threads (double
+   accumulation; the periodic correction from the same Ewald table), for
+   error statements against a direct sum.  out: nt*3 doubles. */
+int pmgpu_direct_sum_host(int64_t n, const float* pos, const float* mass, int64_t nt, const float* tpos,
+                          float softening, const float* box, int32_t threads, double* out);
 
 #ifdef __cplusplus
 }

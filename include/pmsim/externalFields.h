@@ -1,11 +1,13 @@
 // External fields.  The reference passes a std::function<Vec3(Vec3)> to
-// PMMethod; a GPU cannot call a host closure, so the drop-in recognises the
-// callables it can evaluate on the device:
+// PMMethod.  A GPU cannot call a host closure, so the drop-in recognises by
+// type the callables it evaluates on the device:
 //   - SphRadDecr (sphRadDecrField, source/externalFields.cpp:4-16, the
-//     field of the reference's galaxy demos) and ZeroField, by type;
-//   - any other callable that returns exactly zero on every probe point is
-//     taken as a zero field (the reference's demos pass such lambdas);
-// anything else makes the PMMethod constructor throw std::invalid_argument.
+//     field of the reference's galaxy demos), in the interpolation kernel;
+//   - ZeroField, and an empty std::function: no field.
+// Any other callable (the reference's demos pass lambdas, e.g. one wrapping
+// sphRadDecrField) is evaluated on the host for every particle after each
+// force evaluation and added on the device (pmgpu_add_accelerations): exact,
+// at the cost of one position read-back per step.
 #pragma once
 
 #include "vec3.h"

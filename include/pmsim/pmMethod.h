@@ -63,6 +63,7 @@ class PMMethod {
  private:
   void upload();
   uint64_t mirrorHash() const;
+  void applyHostField();  // adds a host-evaluated external field to the last force
 
   Grid& grid;
   float effectiveBoxSizeX, effectiveBoxSizeY, effectiveBoxSizeZ;
@@ -81,4 +82,5 @@ class PMMethod {
   bool mirrorStale = false;   // device newer than the mirror
   uint64_t mirrorSeen = 0;    // hash of the mirror when it was last synchronised
   bool greensReady = false;
+  bool hostField = false;     // the external field is a host callable (not SphRadDecr / ZeroField)
 };

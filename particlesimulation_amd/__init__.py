@@ -27,6 +27,7 @@ LIB_PATH = os.path.join(_HERE, "libpmgpu.so")
 
 # enums, same numbering as the reference's enum classes (include/pmConfig.h)
 NGP, CIC, TSC = 0, 1, 2
+TSC_CENTRED = 3  # TSC about the nearest grid point (an extension; TSC is the reference's floor anchor)
 TWO_POINT, FOUR_POINT = 0, 1
 DISCRETE_LAPLACIAN, S1_OPTIMAL, S2_OPTIMAL, POOR_MAN = 0, 1, 2, 3
 S1, S2 = 0, 1
@@ -51,8 +52,8 @@ EXPORTS = (
     "pmgpu_profile_enable", "pmgpu_profile_reset", "pmgpu_profile_read", "pmgpu_p3m_count_pairs",
     "pmgpu_greens_value", "pmgpu_diagnostics", "pmgpu_mlp_forward", "pmgpu_mlp_backward",
     "pmgpu_trainer_create", "pmgpu_trainer_destroy", "pmgpu_trainer_last_error", "pmgpu_trainer_add_snapshot",
-    "pmgpu_trainer_loss_grad", "pmgpu_train", "pmgpu_direct_sum", "pmgpu_set_tuning", "pmgpu_get_tuning", "pmgpu_p3m_pair_stats",
-    "pmgpu_p3m_cell_geometry",
+    "pmgpu_trainer_loss_grad", "pmgpu_train", "pmgpu_direct_sum", "pmgpu_direct_sum_host", "pmgpu_set_tuning", "pmgpu_get_tuning", "pmgpu_p3m_pair_stats",
+    "pmgpu_p3m_cell_geometry", "pmgpu_add_accelerations",
     "pmgpu_batch_create", "pmgpu_batch_destroy", "pmgpu_batch_last_error", "pmgpu_batch_size",
     "pmgpu_batch_set_particles", "pmgpu_batch_prepare", "pmgpu_batch_step", "pmgpu_batch_synchronize",
     "pmgpu_batch_time", "pmgpu_batch_member_latency", "pmgpu_batch_get_particles", "pmgpu_batch_get_tuning",
@@ -116,6 +117,7 @@ def load_library(path: str = LIB_PATH):
     L.pmgpu_synchronize.argtypes = [vp]
     L.pmgpu_set_particles.argtypes = [vp, C.c_int64, fp, fp, fp]
     L.pmgpu_get_particles.argtypes = [vp, fp, fp, fp]
+    L.pmgpu_add_accelerations.argtypes = [vp, fp]
     L.pmgpu_update_particles.argtypes = [vp, fp, fp, fp]
     L.pmgpu_num_particles.argtypes = [vp]
     L.pmgpu_num_particles.restype = C.c_int64
@@ -168,6 +170,8 @@ def load_library(path: str = LIB_PATH):
     L.pmgpu_trainer_loss_grad.argtypes = [vp, fp, C.POINTER(C.c_double), fp, fp]
     L.pmgpu_train.argtypes = [C.POINTER(vp), C.c_int32, fp, C.c_int32, C.c_float, fp]
     L.pmgpu_direct_sum.argtypes = [C.c_int64, fp, fp, C.c_int64, fp, C.c_float, fp, fp]
+    L.pmgpu_direct_sum_host.argtypes = [C.c_int64, fp, fp, C.c_int64, fp, C.c_float, fp, C.c_int32,
+                                        C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -252,6 +256,12 @@ class PM:
         acc = np.zeros((self.n, 3), np.float32)
         self._check(self._lib.pmgpu_get_particles(self._ctx, _fptr(pos), _fptr(vel), _fptr(acc)))
         return pos, vel, acc
+
+    def add_accelerations(self, acc):
+        """Adds host accelerations (code units, (n, 3), caller order) to those
+        of the last force evaluation."""
+        a = _f32(acc, (-1, 3))
+        self._check(self._lib.pmgpu_add_accelerations(self._ctx, _fptr(a)))
 
     def update_particles(self, pos=None, vel=None, acc=None):
         a = [None if x is None else _f32(x, (-1, 3)) for x in (pos, vel, acc)]
@@ -453,6 +463,25 @@ def direct_sum(pos, mass, targets, softening=0.0, box=None):
                             _fptr(b), _fptr(out))
     if rc != OK:
         _raise(rc, "pmgpu_direct_sum failed")
+    return out
+
+
+def direct_sum_host(pos, mass, targets, softening=0.0, box=None, threads=None):
+    """The same direct sum on the host CPU in FP64 (float64 (nt, 3)), over
+    `threads` threads (default: the CPUs this process may run on)."""
+    L = load_library()
+    pos = _f32(pos, (-1, 3))
+    mass = _f32(mass)
+    targets = _f32(targets, (-1, 3))
+    if threads is None:
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    out = np.zeros((len(targets), 3), np.float64)
+    b = None if box is None else _f32(box)
+    rc = L.pmgpu_direct_sum_host(len(mass), _fptr(pos), _fptr(mass), len(targets), _fptr(targets),
+                                 float(softening), _fptr(b), int(threads),
+                                 out.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc != OK:
+        _raise(rc, "pmgpu_direct_sum_host failed")
     return out
 
 

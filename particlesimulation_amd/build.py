@@ -5,7 +5,8 @@
 Produces, in-tree:
     particlesimulation_amd/libpmgpu.so   HIP kernels + the C ABI of include/pmgpu.h
     particlesimulation_amd/libpmsim.so   C++ drop-in classes (include/pmsim/*.h)
-    tests/dropin/dropin_check            a program using the drop-in classes (test input)
+    tests/dropin/dropin_check            programs using the drop-in classes (test inputs)
+    tests/dropin/galaxy_timing
 """
 from __future__ import annotations
 
@@ -73,9 +74,9 @@ def build_pmsim() -> str | None:
     return lib
 
 
-def build_dropin_check() -> str | None:
-    """The C++ program the drop-in tests run (tests/dropin/dropin_check.cpp)."""
-    src = os.path.join(ROOT, "tests", "dropin", "dropin_check.cpp")
+def build_dropin_check(name: str = "dropin_check") -> str | None:
+    """A C++ program the drop-in tests run (tests/dropin/<name>.cpp)."""
+    src = os.path.join(ROOT, "tests", "dropin", name + ".cpp")
     if not os.path.isfile(src):
         return None
     exe = src[:-4]
@@ -91,6 +92,7 @@ def build_all() -> None:
     build_pmgpu()
     build_pmsim()
     build_dropin_check()
+    build_dropin_check("galaxy_timing")
 
 
 if __name__ == "__main__":

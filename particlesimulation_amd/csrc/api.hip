@@ -90,7 +90,7 @@ static void validate(const pmgpu_pm_params* p) {
   for (int i = 0; i < 3; ++i)
     if (!is_pow2_in(p->grid[i], 32, 1024))
       fail(PMGPU_ERR_UNSUPPORTED, "grid dimensions must be powers of two in [32, 1024]");
-  if (p->interpolation < 0 || p->interpolation > 2)
+  if (p->interpolation < 0 || p->interpolation > PMGPU_TSC_CENTRED)
     fail(PMGPU_ERR_INVALID_ARGUMENT, "Unkown interpolation scheme");
   if (p->finite_diff < 0 || p->finite_diff > 1)
     fail(PMGPU_ERR_INVALID_ARGUMENT, "Unknown finite difference type.");
@@ -132,6 +132,7 @@ static void alloc_particle_arrays(pmgpu_ctx* c, int64_t n) {
   dalloc(&c->key, n);
   dalloc(&c->rank, n);
   dfree(&c->vint);
+  dfree(&c->ext_stage);
   c->fx_slots = 8 * n / DEPOSIT_CHUNK_MIN + 2;  // enough for any tunable chunk size
   dalloc(&c->fx_accum, c->fx_slots * TILE_CELLS);
   dalloc(&c->slot_tile, c->fx_slots);
@@ -321,6 +322,10 @@ int pmgpu_create(const pmgpu_pm_params* params, pmgpu_ctx** out) {
     validate(params);
     c = new pmgpu_ctx();
     c->prm = *params;
+    if (c->prm.interpolation == PMGPU_TSC_CENTRED) {
+      c->prm.interpolation = PMGPU_TSC;
+      c->tsc_centred = 1;
+    }
     c->nx = params->grid[0];
     c->ny = params->grid[1];
     c->nz = params->grid[2];
@@ -380,7 +385,7 @@ void pmgpu_destroy(pmgpu_ctx* c) {
     dfree(&c->pm[b]);
     dfree(&c->vel[b]);
   }
-  dfree(&c->acc); dfree(&c->vint); dfree(&c->key); dfree(&c->rank);
+  dfree(&c->acc); dfree(&c->vint); dfree(&c->ext_stage); dfree(&c->key); dfree(&c->rank);
   dfree(&c->cls_count); dfree(&c->cls_start); dfree(&c->tile_nchunk); dfree(&c->tile_chunk_start);
   dfree(&c->tile_slot); dfree(&c->slot_tile); dfree(&c->int_nchunk); dfree(&c->int_chunk_start);
   dfree(&c->dep_table); dfree(&c->counters); dfree(&c->fx_accum); dfree(&c->scan_tmp);
@@ -619,6 +624,19 @@ int pmgpu_leapfrog(pmgpu_ctx* c, int32_t op, float dt) {
   if (op == PMGPU_INTEGER_STEP && !c->vint) dalloc(&c->vint, c->n);
   leapfrog_op(c, op, dt, INT_MIN);
   if (op == PMGPU_DRIFT) c->sorted_valid = false;
+  PMGPU_API_END(c)
+}
+
+int pmgpu_add_accelerations(pmgpu_ctx* c, const float* acc) {
+  PMGPU_API_BEGIN(c)
+  require_particles(c);
+  if (!acc) fail(PMGPU_ERR_INVALID_ARGUMENT, "acc is NULL");
+  if (!c->ext_stage) dalloc(&c->ext_stage, 3 * c->n);
+  // the copy is ordered on the stream after the force that wrote c->acc and
+  // returns only once the host array has been read
+  HIP_CHECK(hipMemcpyAsync(c->ext_stage, acc, sizeof(float) * 3 * (size_t)c->n, hipMemcpyHostToDevice, c->stream));
+  add_accelerations(c, c->ext_stage);
+  HIP_CHECK(hipStreamSynchronize(c->stream));
   PMGPU_API_END(c)
 }
 

@@ -34,8 +34,8 @@ __device__ __forceinline__ int wrapc(int a, int n) {
   return a < 0 ? a + n : a;
 }
 
-__device__ __forceinline__ int anchor_of(float x, int scheme) {
-  float f = scheme == PMGPU_NGP ? roundf(x) : floorf(x);
+__device__ __forceinline__ int anchor_of(float x, int scheme, int centred) {
+  float f = (scheme == PMGPU_NGP || centred) ? roundf(x) : floorf(x);
   f = fminf(fmaxf(f, -1.0e9f), 1.0e9f);
   return (int)f;
 }
@@ -134,7 +134,8 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit(
     int ntiles, int nranges, const int4* __restrict__ table,
     const int32_t* __restrict__ cls_start, const int32_t* __restrict__ chunk_start,
     const int32_t* __restrict__ owner, const int32_t* __restrict__ slot, long long* __restrict__ fx_accum,
-    int shift, float* __restrict__ rho, int dep_chunk, const int32_t* __restrict__ escape, int iteration) {
+    int shift, float* __restrict__ rho, int dep_chunk, int centred, const int32_t* __restrict__ escape,
+    int iteration) {
   if (escape[0] <= iteration) return;
   __shared__ unsigned long long acc[TILE_CELLS];
   __shared__ int32_t rstart[MAX_RANGES], rpre[MAX_RANGES + 1];
@@ -209,7 +210,7 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit(
         while (rpre[a + 1] <= vn) ++a;
         pnext = pm[rstart[a] + (vn - rpre[a])];
       }
-      const int ax = anchor_of(p.x, SCHEME), ay = anchor_of(p.y, SCHEME), az = anchor_of(p.z, SCHEME);
+      const int ax = anchor_of(p.x, SCHEME, centred), ay = anchor_of(p.y, SCHEME, centred), az = anchor_of(p.z, SCHEME, centred);
       float wx[3], wy[3], wz[3];
       axis_weights(SCHEME, p.x - (float)ax, wx);
       axis_weights(SCHEME, p.y - (float)ay, wy);
@@ -337,8 +338,7 @@ void deposit(pmgpu_ctx* c) {
     hipLaunchKernelGGL(kern, dim3(grid), dim3(DEP_THREADS), 0, c->stream, c->pm[c->cur], c->nx, c->ny, c->nz,
                        c->ntx, c->nty, c->ntz, c->ntiles, nranges, c->dep_table, c->cls_start,
                        c->tile_chunk_start, c->dep_owner, c->tile_slot, c->fx_accum, c->fx_shift, c->rho,
-                       c->dep_chunk,
-                       c->escape, c->iter);
+                       c->dep_chunk, c->tsc_centred, c->escape, c->iter);
   };
   if (c->prm.interpolation == PMGPU_TSC) go(k_deposit<PMGPU_TSC>);
   else if (c->prm.interpolation == PMGPU_CIC) go(k_deposit<PMGPU_CIC>);

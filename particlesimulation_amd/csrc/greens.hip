@@ -158,8 +158,9 @@ extern "C" double pmgpu_greens_value(int32_t greens_function, int32_t interpolat
                                      double particle_diameter, int32_t kx, int32_t ky, int32_t kz,
                                      int32_t nx, int32_t ny, int32_t nz) {
   if (nx <= 0 || ny <= 0 || nz <= 0 || greens_function < 0 || greens_function > 3 || interpolation < 0 ||
-      interpolation > 2 || finite_diff < 0 || finite_diff > 1)
+      interpolation > PMGPU_TSC_CENTRED || finite_diff < 0 || finite_diff > 1)
     return NAN;
+  if (interpolation == PMGPU_TSC_CENTRED) interpolation = PMGPU_TSC;  // same assignment function
   return pmgpu::green_at(greens_function, interpolation, finite_diff, particle_diameter, kx, ky, kz,
                          nx, ny, nz);
 }

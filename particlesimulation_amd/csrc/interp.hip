@@ -21,8 +21,8 @@ __device__ __forceinline__ int wrapc(int a, int n) {
   return a < 0 ? a + n : a;
 }
 
-__device__ __forceinline__ int anchor_of(float x, int scheme) {
-  float f = scheme == PMGPU_NGP ? roundf(x) : floorf(x);
+__device__ __forceinline__ int anchor_of(float x, int scheme, int centred) {
+  float f = (scheme == PMGPU_NGP || centred) ? roundf(x) : floorf(x);
   f = fminf(fmaxf(f, -1.0e9f), 1.0e9f);
   return (int)f;
 }
@@ -90,7 +90,8 @@ __global__ __launch_bounds__(256) void k_interp(
     const float* __restrict__ phi, int nx, int ny, int nz, int ntx, int nty, int ntiles,
     const int32_t* __restrict__ cls_start, const int32_t* __restrict__ chunk_start,
     const int32_t* __restrict__ owner, float4* __restrict__ pm, float4* __restrict__ vel, float4* __restrict__ acc, int mode,
-    float kick, float3 box, ExtField ext, int int_chunk, int32_t* __restrict__ escape, int iteration) {
+    float kick, float3 box, ExtField ext, int int_chunk, int centred, int32_t* __restrict__ escape,
+    int iteration) {
   if (escape[0] <= iteration) return;
   using Gm = Geo<SCHEME, FDS>;
   __shared__ float sphi[Gm::PX * Gm::PY * Gm::PZ];
@@ -146,7 +147,7 @@ __global__ __launch_bounds__(256) void k_interp(
     const int pend = min(pbeg + int_chunk, cls_start[t * NCLASS + NCLASS]);
     for (int i = pbeg + (int)threadIdx.x; i < pend; i += 256) {
       float4 p = pm[i];
-      const int ax = anchor_of(p.x, SCHEME), ay = anchor_of(p.y, SCHEME), az = anchor_of(p.z, SCHEME);
+      const int ax = anchor_of(p.x, SCHEME, centred), ay = anchor_of(p.y, SCHEME, centred), az = anchor_of(p.z, SCHEME, centred);
       float wx[3], wy[3], wz[3];
       interp_weights<SCHEME>(p.x - (float)ax, wx);
       interp_weights<SCHEME>(p.y - (float)ay, wy);
@@ -247,14 +248,14 @@ template <int SCHEME>
 __global__ __launch_bounds__(256) void k_interp_gather(
     const float4* __restrict__ E, int nx, int ny, int nz, int64_t n, float4* __restrict__ pm,
     float4* __restrict__ vel, float4* __restrict__ acc, int mode, float kick, float3 box, ExtField ext,
-    int32_t* __restrict__ escape, int iteration) {
+    int centred, int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   constexpr int SLO = SCHEME == PMGPU_TSC ? -1 : 0;
   constexpr int CNT = SCHEME == PMGPU_TSC ? 3 : (SCHEME == PMGPU_CIC ? 2 : 1);
   float4 p = pm[i];
-  const int ax = anchor_of(p.x, SCHEME), ay = anchor_of(p.y, SCHEME), az = anchor_of(p.z, SCHEME);
+  const int ax = anchor_of(p.x, SCHEME, centred), ay = anchor_of(p.y, SCHEME, centred), az = anchor_of(p.z, SCHEME, centred);
   float wx[3], wy[3], wz[3];
   interp_weights<SCHEME>(p.x - (float)ax, wx);
   interp_weights<SCHEME>(p.y - (float)ay, wy);
@@ -314,7 +315,7 @@ template <int SCHEME, int FDS>
 __global__ __launch_bounds__(256) void k_interp_phi(
     const float* __restrict__ phi, int nx, int ny, int nz, int64_t n, float4* __restrict__ pm,
     float4* __restrict__ vel, float4* __restrict__ acc, int mode, float kick, float3 box, ExtField ext,
-    int32_t* __restrict__ escape, int iteration) {
+    int centred, int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -326,7 +327,7 @@ __global__ __launch_bounds__(256) void k_interp_phi(
   constexpr float S2[3] = {-0.5f, 0.0f, 0.5f};
   constexpr float S4[5] = {1.0f / 12, -8.0f / 12, 0.0f, 8.0f / 12, -1.0f / 12};
   float4 p = pm[i];
-  const int ax = anchor_of(p.x, SCHEME), ay = anchor_of(p.y, SCHEME), az = anchor_of(p.z, SCHEME);
+  const int ax = anchor_of(p.x, SCHEME, centred), ay = anchor_of(p.y, SCHEME, centred), az = anchor_of(p.z, SCHEME, centred);
   float wx[3], wy[3], wz[3];
   interp_weights<SCHEME>(p.x - (float)ax, wx);
   interp_weights<SCHEME>(p.y - (float)ay, wy);
@@ -446,8 +447,8 @@ void launch_interp(pmgpu_ctx* c, int grid, int mode, float kick, float3 box, Ext
                    int iteration) {
   hipLaunchKernelGGL((k_interp<S, F>), dim3(grid), dim3(256), 0, c->stream, c->phi, c->nx, c->ny,
                      c->nz, c->ntx, c->nty, c->ntiles, c->cls_start, c->int_chunk_start, c->int_owner,
-                     c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick, box, e, c->int_chunk, c->escape,
-                     iteration);
+                     c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick, box, e, c->int_chunk, c->tsc_centred,
+                     c->escape, iteration);
 }
 
 }  // namespace
@@ -485,7 +486,8 @@ static void interpolate_gather(pmgpu_ctx* c, int mode, float kick, int iteration
   const float3 box = box_of(c);
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(gp), dim3(256), 0, c->stream, (const float4*)c->efield, c->nx, c->ny, c->nz, c->n,
-                       c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick, box, e, c->escape, iteration);
+                       c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick, box, e, c->tsc_centred, c->escape,
+                       iteration);
   };
   if (c->prm.interpolation == PMGPU_NGP) go(k_interp_gather<PMGPU_NGP>);
   else if (c->prm.interpolation == PMGPU_CIC) go(k_interp_gather<PMGPU_CIC>);
@@ -500,7 +502,8 @@ static void interpolate_phi(pmgpu_ctx* c, int mode, float kick, int iteration) {
   const float3 box = box_of(c);
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(gp), dim3(256), 0, c->stream, (const float*)c->phi, c->nx, c->ny, c->nz, c->n,
-                       c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick, box, e, c->escape, iteration);
+                       c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick, box, e, c->tsc_centred, c->escape,
+                       iteration);
   };
   const int s = c->prm.interpolation, f = c->prm.finite_diff;
   if (s == PMGPU_NGP && f == PMGPU_TWO_POINT) go(k_interp_phi<PMGPU_NGP, PMGPU_TWO_POINT>);
@@ -561,6 +564,24 @@ void leapfrog_op(pmgpu_ctx* c, int op, float dt, int iteration) {
 void escape_check(pmgpu_ctx* c, int32_t* flag_dev) {
   hipLaunchKernelGGL(k_escape, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0, c->stream,
                      c->pm[c->cur], c->n, box_of(c), flag_dev);
+  HIP_CHECK(hipGetLastError());
+}
+
+__global__ void k_add_accelerations(float4* __restrict__ acc, const float4* __restrict__ vel,
+                                    const float* __restrict__ add, int64_t n) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int64_t id = __float_as_int(vel[s].w);
+  float4 a = acc[s];
+  a.x += add[3 * id];
+  a.y += add[3 * id + 1];
+  a.z += add[3 * id + 2];
+  acc[s] = a;
+}
+
+void add_accelerations(pmgpu_ctx* c, const float* add) {
+  hipLaunchKernelGGL(k_add_accelerations, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0, c->stream, c->acc,
+                     c->vel[c->cur], add, c->n);
   HIP_CHECK(hipGetLastError());
 }
 

@@ -80,7 +80,8 @@ void check_hip(hipError_t e, const char* what);
 
 struct pmgpu_ctx {
   // configuration
-  pmgpu_pm_params prm{};
+  pmgpu_pm_params prm{};  // prm.interpolation holds the assignment order (NGP, CIC or TSC)
+  int tsc_centred = 0;     // PMGPU_TSC_CENTRED requested: TSC about the nearest grid point
   int nx = 0, ny = 0, nz = 0, nxh = 0;  // nxh = nx/2 + 1
   int64_t M = 0, K = 0;                  // cells, half-spectrum modes
   int ntx = 0, nty = 0, ntz = 0, ntiles = 0;
@@ -93,6 +94,7 @@ struct pmgpu_ctx {
   float4* vel[2] = {nullptr, nullptr};  // vx, vy, vz, original index (int bits)
   float4* acc = nullptr;                // ax, ay, az, -
   float4* vint = nullptr;               // integer-step velocities (lazy)
+  float* ext_stage = nullptr;           // host-evaluated accelerations, caller order, 3 n (lazy)
   int cur = 0;                          // which buffer holds the state
   int32_t* key = nullptr;
   int32_t* rank = nullptr;
@@ -218,6 +220,8 @@ void fft_r2c_forward(pmgpu_ctx* c, const float* in, float2* spec);
 void fft_mul_c2r_inverse(pmgpu_ctx* c, const float2* src, const float* g, float2* work, float* out);
 void greens_compute(pmgpu_ctx* c, const float* mlp_dev);
 void interpolate(pmgpu_ctx* c, int mode, float kick, int iteration);
+// acc[s] += add[3 id(s) ..], id(s) the original index of sorted particle s
+void add_accelerations(pmgpu_ctx* c, const float* add);
 void field_grid(pmgpu_ctx* c);
 void leapfrog_op(pmgpu_ctx* c, int op, float dt, int iteration);
 void p3m_setup(pmgpu_ctx* c);

@@ -132,9 +132,10 @@ __device__ __forceinline__ int wrap_cell(int a, int n) {
 }
 
 // Anchor cell of one coordinate: round for NGP (source/pmMethod.cpp:207),
-// floor for CIC/TSC ((int) truncation for x >= 0, pmMethod.cpp:219, 250).
-__device__ __forceinline__ int anchor_of(float x, int scheme) {
-  float f = scheme == PMGPU_NGP ? roundf(x) : floorf(x);
+// floor for CIC/TSC ((int) truncation for x >= 0, pmMethod.cpp:219, 250),
+// round for the textbook (centred) TSC.
+__device__ __forceinline__ int anchor_of(float x, int scheme, int centred) {
+  float f = (scheme == PMGPU_NGP || centred) ? roundf(x) : floorf(x);
   f = fminf(fmaxf(f, -1.0e9f), 1.0e9f);  // NaN -> -1e9, keeps the int conversion defined
   return (int)f;
 }
@@ -144,16 +145,16 @@ __device__ __forceinline__ int face_class(int local, int extent) {
 }
 
 __global__ void k_tile_keys(const float4* __restrict__ pm, int64_t n, int nx, int ny, int nz,
-                            int ntx, int nty, int scheme, int32_t* __restrict__ key,
+                            int ntx, int nty, int scheme, int centred, int32_t* __restrict__ key,
                             int32_t* __restrict__ rank, int32_t* __restrict__ count,
                             const int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = i < n;
   float4 p = active ? pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-  int ax = wrap_cell(anchor_of(p.x, scheme), nx);
-  int ay = wrap_cell(anchor_of(p.y, scheme), ny);
-  int az = wrap_cell(anchor_of(p.z, scheme), nz);
+  int ax = wrap_cell(anchor_of(p.x, scheme, centred), nx);
+  int ay = wrap_cell(anchor_of(p.y, scheme, centred), ny);
+  int az = wrap_cell(anchor_of(p.z, scheme, centred), nz);
   int tx = ax / TILE_X, ty = ay / TILE_Y, tz = az / TILE_Z;
   int cls = face_class(ax - tx * TILE_X, TILE_X) + 3 * face_class(ay - ty * TILE_Y, TILE_Y) +
             9 * face_class(az - tz * TILE_Z, TILE_Z);
@@ -249,7 +250,7 @@ void tile_sort(pmgpu_ctx* c) {
   zero_fill_i32(c, c->cls_count, nkeys);
   unsigned g = (unsigned)((c->n + 255) / 256);
   hipLaunchKernelGGL(k_tile_keys, dim3(g), dim3(256), 0, c->stream, c->pm[c->cur], c->n, c->nx,
-                     c->ny, c->nz, c->ntx, c->nty, c->prm.interpolation, c->key, c->rank,
+                     c->ny, c->nz, c->ntx, c->nty, c->prm.interpolation, c->tsc_centred, c->key, c->rank,
                      c->cls_count, c->escape, c->iter);
   HIP_CHECK(hipGetLastError());
   exclusive_scan_i32(c, c->cls_count, c->cls_start, nkeys);

@@ -25,6 +25,7 @@
 #include "pmgpu_internal.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <array>
 #include <map>
@@ -394,7 +395,7 @@ using namespace pmgpu;
 
 struct pmgpu_trainer {
   pmgpu_ctx* ctx = nullptr;
-  int scheme = 0, fds = 0, S = 1;
+  int scheme = 0, centred = 0, fds = 0, S = 1;
   std::vector<Snap> snaps;
   float* geff = nullptr;   // K
   float* gradc = nullptr;  // K
@@ -414,15 +415,15 @@ namespace {
 constexpr int NPARAM = 8705;
 
 // Host copy of the deposit's per-axis weights (csrc/deposit.hip), floor
-// anchor for CIC/TSC, round for NGP.
-void axis_stencil(int scheme, float x, int* first, float w[3], int* cnt) {
+// anchor for CIC/TSC, round for NGP and the centred TSC.
+void axis_stencil(int scheme, int centred, float x, int* first, float w[3], int* cnt) {
   if (scheme == PMGPU_NGP) {
     *first = (int)std::round(x);
     w[0] = 1.0f;
     *cnt = 1;
     return;
   }
-  const float f = std::floor(x);
+  const float f = centred ? std::round(x) : std::floor(x);
   const float d = x - f;
   if (scheme == PMGPU_CIC) {
     *first = (int)f;
@@ -546,7 +547,8 @@ int pmgpu_trainer_create(const pmgpu_pm_params* params, pmgpu_trainer** out) {
   pmgpu_trainer* t = new pmgpu_trainer();
   t->ctx = c;
   try {
-    t->scheme = params->interpolation;
+    t->scheme = c->prm.interpolation;
+    t->centred = c->tsc_centred;
     t->fds = params->finite_diff;
     t->S = t->scheme == PMGPU_NGP ? 1 : (t->scheme == PMGPU_CIC ? 8 : 27);
     greens_compute(c, nullptr);  // analytic G_sym / M
@@ -623,7 +625,7 @@ int pmgpu_trainer_add_snapshot(pmgpu_trainer* t, int64_t n, const float* pos, co
     const int64_t j = target_index[i];
     int f[3], cnt[3];
     float w[3][3];
-    for (int a = 0; a < 3; ++a) axis_stencil(t->scheme, pos[3 * j + a], &f[a], w[a], &cnt[a]);
+    for (int a = 0; a < 3; ++a) axis_stencil(t->scheme, t->centred, pos[3 * j + a], &f[a], w[a], &cnt[a]);
     int e = 0;
     for (int a = 0; a < cnt[0]; ++a)
       for (int b = 0; b < cnt[1]; ++b)
@@ -805,6 +807,96 @@ int pmgpu_direct_sum(int64_t n, const float* pos, const float* mass, int64_t nt,
   dfree(res);
   dfree(tab);
   return rc;
+}
+
+// The same sums on the host CPU in FP64 (an error evaluator, the FP64
+// counterpart of the reference's direct PP sum, source/ppMethod.cpp:23-36):
+// sources as doubles, one target per task over `threads` std::threads; the
+// periodic correction is read from the same Ewald table, interpolated in FP64.
+int pmgpu_direct_sum_host(int64_t n, const float* pos, const float* mass, int64_t nt, const float* tpos,
+                          float softening, const float* box, int32_t threads, double* out) {
+  if (n < 0 || nt < 0 || (n > 0 && (!pos || !mass)) || (nt > 0 && (!tpos || !out))) return PMGPU_ERR_INVALID_ARGUMENT;
+  const bool periodic = box != nullptr;
+  if (periodic && !(box[0] > 0.f && box[1] > 0.f && box[2] > 0.f)) return PMGPU_ERR_INVALID_ARGUMENT;
+  if (nt == 0) return PMGPU_OK;
+  try {
+    std::vector<double> sx(n), sy(n), sz(n), sm(n);
+    for (int64_t j = 0; j < n; ++j) {
+      sx[j] = pos[3 * j];
+      sy[j] = pos[3 * j + 1];
+      sz[j] = pos[3 * j + 2];
+      sm[j] = mass[j];
+    }
+    const std::vector<float4>* tab = periodic ? &ewald_table(box) : nullptr;
+    double L[3] = {0, 0, 0}, scale[3] = {0, 0, 0};
+    if (periodic)
+      for (int c = 0; c < 3; ++c) {
+        L[c] = box[c];
+        scale[c] = EW_N / (0.5 * box[c]);
+      }
+    const double eps2 = (double)softening * softening, Gc = 1.0 / (4.0 * M_PI);
+    constexpr int S1 = EW_N + 1;
+    auto corr = [&](double dx, double dy, double dz, double c[3]) {
+      const double u[3] = {std::min(std::fabs(dx) * scale[0], EW_N - 1e-9),
+                           std::min(std::fabs(dy) * scale[1], EW_N - 1e-9),
+                           std::min(std::fabs(dz) * scale[2], EW_N - 1e-9)};
+      const int ix = (int)u[0], iy = (int)u[1], iz = (int)u[2];
+      const double fx = u[0] - ix, fy = u[1] - iy, fz = u[2] - iz;
+      double acc[3] = {0, 0, 0};
+      for (int k = 0; k < 8; ++k) {
+        const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
+        const double w = (ox ? fx : 1 - fx) * (oy ? fy : 1 - fy) * (oz ? fz : 1 - fz);
+        const float4 v = (*tab)[(ix + ox) + (size_t)S1 * ((iy + oy) + (size_t)S1 * (iz + oz))];
+        acc[0] += w * v.x;
+        acc[1] += w * v.y;
+        acc[2] += w * v.z;
+      }
+      c[0] = dx < 0 ? -acc[0] : acc[0];
+      c[1] = dy < 0 ? -acc[1] : acc[1];
+      c[2] = dz < 0 ? -acc[2] : acc[2];
+    };
+    std::atomic<int64_t> next{0};
+    auto work = [&]() {
+      for (int64_t t = next++; t < nt; t = next++) {
+        const double px = tpos[3 * t], py = tpos[3 * t + 1], pz = tpos[3 * t + 2];
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        for (int64_t j = 0; j < n; ++j) {
+          double dx = px - sx[j], dy = py - sy[j], dz = pz - sz[j];
+          if (periodic) {
+            dx -= L[0] * std::nearbyint(dx / L[0]);
+            dy -= L[1] * std::nearbyint(dy / L[1]);
+            dz -= L[2] * std::nearbyint(dz / L[2]);
+          }
+          const double r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 <= 0.0) continue;
+          const double s = r2 + eps2;
+          const double f = sm[j] / (s * std::sqrt(s));
+          ax += f * dx;
+          ay += f * dy;
+          az += f * dz;
+          if (periodic) {
+            double c[3];
+            corr(dx, dy, dz, c);
+            ax -= sm[j] * c[0];
+            ay -= sm[j] * c[1];
+            az -= sm[j] * c[2];
+          }
+        }
+        out[3 * t] = -Gc * ax;
+        out[3 * t + 1] = -Gc * ay;
+        out[3 * t + 2] = -Gc * az;
+      }
+    };
+    const int nth = std::max(1, (int)threads);
+    std::vector<std::thread> th;
+    for (int i = 0; i < nth; ++i) th.emplace_back(work);
+    for (auto& x : th) x.join();
+  } catch (const Error& e) {
+    return e.code;
+  } catch (const std::exception&) {
+    return PMGPU_ERR_OUT_OF_MEMORY;
+  }
+  return PMGPU_OK;
 }
 
 }  // extern "C"

@@ -1,7 +1,6 @@
 // PMMethod on the GPU (reference source/pmMethod.cpp), driving libpmgpu.
 #include <cstring>
 #include <iostream>
-#include <random>
 #include <stdexcept>
 #include <string>
 
@@ -19,20 +18,6 @@ void check(int rc, pmgpu_ctx* ctx, const char* what) {
   const std::string msg = std::string(what) + ": " + pmgpu_last_error(ctx);
   if (rc == PMGPU_ERR_INVALID_ARGUMENT) throw std::invalid_argument(msg);
   throw std::runtime_error(msg);
-}
-
-// A zero field is recognised by type (ZeroField) or, for arbitrary callables
-// such as the reference's `[](Vec3) { return Vec3::zero(); }`, by returning
-// exactly zero at every seeded probe point of the box and its surroundings.
-bool probesZero(const std::function<Vec3(Vec3)>& f, float bx, float by, float bz) {
-  std::mt19937 rng(12345);
-  std::uniform_real_distribution<float> u(-0.5f, 1.5f);
-  for (int k = 0; k < 64; ++k) {
-    Vec3 p(u(rng) * bx, u(rng) * by, u(rng) * bz);
-    Vec3 g = f(p);
-    if (g.x != 0.0f || g.y != 0.0f || g.z != 0.0f) return false;
-  }
-  return true;
 }
 
 }  // namespace
@@ -79,12 +64,12 @@ PMMethod::PMMethod(const std::vector<Vec3>& state, const std::vector<float>& mas
     p.ext_kind = PMGPU_EXT_SPH_RAD_DECR;
     float prm[6] = {f->center.x, f->center.y, f->center.z, f->R, f->M, f->G};
     std::memcpy(p.ext_params, prm, sizeof(prm));
-  } else if (!externalField || externalField.target<ZeroField>() ||
-             probesZero(externalField, effectiveBoxSizeX, effectiveBoxSizeY, effectiveBoxSizeZ)) {
-    p.ext_kind = PMGPU_EXT_NONE;
   } else {
-    throw std::invalid_argument(
-        "external field: the GPU path evaluates ZeroField, SphRadDecr or callables that are zero");
+    // ZeroField and an empty function add nothing; any other callable (the
+    // reference's demos pass lambdas) is evaluated on the host after every
+    // force evaluation, as the reference evaluates it per particle
+    p.ext_kind = PMGPU_EXT_NONE;
+    hostField = externalField && !externalField.target<ZeroField>();
   }
   check(pmgpu_create(&p, &ctx), nullptr, "pmgpu_create");
   grid.attach(ctx);
@@ -187,8 +172,27 @@ void PMMethod::pmMethodStep() {
   uploadIfModified();
   if (!greensReady) initGreensFunction();
   check(pmgpu_pm_step(ctx), ctx, "pmgpu_pm_step");
+  applyHostField();
   mirrorStale = true;
   grid.markDeviceNewer();
+}
+
+// The external-field term of updateAccelerations (source/pmMethod.cpp:
+// 384-390) for a callable the device cannot evaluate: the positions come
+// back in code units, the field is evaluated at positionToOriginalUnits and
+// its accelerationToCodeUnits is added on the device.
+void PMMethod::applyHostField() {
+  if (!hostField) return;
+  std::vector<float> pos(3 * (size_t)N), add(3 * (size_t)N);
+  check(pmgpu_get_particles(ctx, pos.data(), nullptr, nullptr), ctx, "pmgpu_get_particles");
+  for (int i = 0; i < N; ++i) {
+    const Vec3 x(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    const Vec3 a = accelerationToCodeUnits(externalField(positionToOriginalUnits(x, H)), H, DT);
+    add[3 * i] = a.x;
+    add[3 * i + 1] = a.y;
+    add[3 * i + 2] = a.z;
+  }
+  check(pmgpu_add_accelerations(ctx, add.data()), ctx, "pmgpu_add_accelerations");
 }
 
 bool PMMethod::escapedComputationalBox() {
@@ -230,6 +234,7 @@ std::string PMMethod::runLoop(StateRecorder& rec, int simLength, bool collectDia
   initGreensFunction();
   auto force = [&]() {
     check(pmgpu_pm_step(ctx), ctx, "pmgpu_pm_step");
+    applyHostField();
     if (p3m) check(pmgpu_p3m_short_range(ctx), ctx, "pmgpu_p3m_short_range");
   };
   force();

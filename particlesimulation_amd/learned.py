@@ -109,11 +109,13 @@ def _time_steps(pm, steps, p3m=False):
     return (time.perf_counter() - t0) / steps
 
 
-def evaluate_large(params, n_targets=65536, steps=5, seed=42):
+def evaluate_large(params, n_targets=65536, steps=5, seed=42, fp64_check=256):
     """The 4M-particle clustered workload on 256^3 (TSC, 4-point): e_rms of plain
     PM, learned PM and P3M (S2 optimal G, a = r_c = 3 cells) against the periodic
-    direct sum on a seeded subsample, and the throughput of each."""
-    from . import S2, S2_OPTIMAL
+    direct sum on a seeded subsample, and the throughput of each.  The GPU's
+    FP32 direct sum is checked against the FP64 host sum on `fp64_check` of
+    the targets, and every error is also stated against the FP64 sum there."""
+    from . import S2, S2_OPTIMAL, direct_sum_host
     w = ics.C5_PM_CLUMPS_256
     pos, vel, m = w.code_units(seed=seed)
     rng = np.random.default_rng(seed + 1)
@@ -121,8 +123,14 @@ def evaluate_large(params, n_targets=65536, steps=5, seed=42):
     t0 = time.perf_counter()
     ref = direct_sum(pos, m, pos[tgt], box=w.grid).astype(np.float64)
     t_ref = time.perf_counter() - t0
+    sub = np.arange(min(fp64_check, len(tgt)))
+    t0 = time.perf_counter()
+    ref64 = direct_sum_host(pos, m, pos[tgt[sub]], box=w.grid)
+    t_ref64 = time.perf_counter() - t0
     out = {"workload": w.name, "n": w.n, "grid": list(w.grid), "targets": int(len(tgt)), "direct_sum_s": t_ref,
-           "reference": "gpu fp32 periodic (Ewald) direct sum"}
+           "reference": "gpu fp32 periodic (Ewald) direct sum",
+           "fp64_check": {"targets": int(len(sub)), "seconds": t_ref64,
+                          "rms_rel_fp32_vs_fp64": float(np.linalg.norm(ref[sub] - ref64) / np.linalg.norm(ref64))}}
     for name, corr in (("plain_pm", None), ("learned_pm", params)):
         pm = PM(w.grid, interpolation=w.interpolation, finite_diff=w.finite_diff, greens_function=w.greens_function,
                 box=(0, 0, 0))
@@ -133,7 +141,9 @@ def evaluate_large(params, n_targets=65536, steps=5, seed=42):
         pm.pm_step()
         a = pm.get_particles()[2][tgt]
         s = _time_steps(pm, steps)
-        out[name] = {"e_rms": e_rms(a, ref), "ms_per_step": s * 1e3, "updates_per_s": w.n / s}
+        out[name] = {"e_rms": e_rms(a, ref), "e_rms_fp64_subset": e_rms(a[sub], ref64), "ms_per_step": s * 1e3,
+                     "updates_per_s": w.n / s}
+        pm.close()
     pm = PM(w.grid, interpolation=w.interpolation, finite_diff=w.finite_diff, greens_function=S2_OPTIMAL,
             particle_diameter=3.0, box=(0, 0, 0))
     pm.set_particles(pos, vel, m)
@@ -143,7 +153,9 @@ def evaluate_large(params, n_targets=65536, steps=5, seed=42):
     pm.p3m_short_range()
     a = pm.get_particles()[2][tgt]
     s = _time_steps(pm, steps, p3m=True)
-    out["plain_p3m"] = {"e_rms": e_rms(a, ref), "ms_per_step": s * 1e3, "updates_per_s": w.n / s}
+    out["plain_p3m"] = {"e_rms": e_rms(a, ref), "e_rms_fp64_subset": e_rms(a[sub], ref64), "ms_per_step": s * 1e3,
+                        "updates_per_s": w.n / s}
+    pm.close()
     return out
 
 

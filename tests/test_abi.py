@@ -33,7 +33,7 @@ def test_mlp_parameter_count(gpu_lib):
 
 
 @pytest.mark.parametrize("field,value,code", [
-    ("interpolation", 3, 1), ("finite_diff", 2, 1), ("greens_function", 4, 1), ("ext_kind", 7, 1),
+    ("interpolation", 4, 1), ("interpolation", -1, 1), ("finite_diff", 2, 1), ("greens_function", 4, 1), ("ext_kind", 7, 1),
 ])
 def test_create_rejects_invalid_enums(gpu_lib, field, value, code):
     P = gpu_lib

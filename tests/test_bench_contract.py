@@ -16,16 +16,25 @@ def test_more_than_one_gpu_is_refused_with_a_message():
     assert "one GPU" in r.stderr and r.stdout == ""
 
 
-def test_recorded_traffic_and_learned_comparison_parse():
+def test_recorded_traffic_parses_and_host_threads_follow_the_cpu_share(monkeypatch):
     import bench
     t, src = bench.measured_traffic("p3m_disk_1M_128")
     assert t is not None and 1e8 < t < 1e11 and src.startswith("profiles/")
     assert bench.measured_traffic("no_such_workload") == (None, None)
-    d = bench.measured_learned()
-    assert d is not None and d["workload"] == "pm_clumps_4M_256"
-    for k in ("plain_pm", "learned_pm", "plain_p3m"):
-        assert d[k]["e_rms"] > 0 and d[k]["updates_per_s"] > 0
-    json.dumps(d)  # serialisable into the bench line
+    monkeypatch.setenv("OMP_NUM_THREADS", "3")
+    n, counts = bench.host_threads()
+    assert n == min(3, counts["affinity"]) and counts["nproc"] >= counts["affinity"] >= 1
+    monkeypatch.delenv("OMP_NUM_THREADS")
+    assert bench.host_threads()[0] == counts["affinity"]
+
+
+def test_workload_configs_name_the_error_target_workload():
+    import bench
+    from particlesimulation_amd import ics
+    c = bench.workload_config(ics.C2_P3M_DISK_ERR, 42)
+    assert c["interpolation"] == "TSC_CENTRED" and c["finite_diff"] == "FOUR_POINT"
+    assert c["a_cells"] == c["cutoff_cells"] == 5.0 and c["n"] == 1048576 and c["grid"] == [128, 128, 128]
+    json.dumps(bench.workload_config(ics.C2_P3M_DISK, 42))
 
 
 def test_traffic_file_matches_its_own_counters():

@@ -46,7 +46,7 @@ def _rel(a, b):
 # --------------------------------------------------------------------------
 
 @pytest.mark.parametrize("grid", [(32, 32, 32), (64, 32, 128)])
-@pytest.mark.parametrize("scheme", [0, 1, 2])
+@pytest.mark.parametrize("scheme", [0, 1, 2, 3])
 def test_deposit_matches_oracle(gpu_lib, oracle, grid, scheme):
     pos, vel, m = _particles(4096, grid, 1 + scheme)
     pm = _make(gpu_lib, grid, scheme, 0, 0)
@@ -58,7 +58,7 @@ def test_deposit_matches_oracle(gpu_lib, oracle, grid, scheme):
     assert np.max(np.abs(rho - ref)) / np.max(np.abs(ref)) <= DENSITY_TOL
 
 
-@pytest.mark.parametrize("scheme", [1, 2])
+@pytest.mark.parametrize("scheme", [1, 2, 3])
 def test_deposit_wraps_periodically(gpu_lib, oracle, scheme):
     grid = (32, 32, 32)
     r = _rng(3)
@@ -122,7 +122,7 @@ def test_greens_matches_oracle(gpu_lib, oracle, gf, scheme, fds):
 # full PM step
 # --------------------------------------------------------------------------
 
-COMBOS = [(s, f, g) for s in (0, 1, 2) for f in (0, 1) for g in (0, 1, 2, 3)]
+COMBOS = [(s, f, g) for s in (0, 1, 2) for f in (0, 1) for g in (0, 1, 2, 3)] + [(3, 0, 2), (3, 1, 2), (3, 1, 0)]
 
 
 @pytest.mark.parametrize("scheme,fds,gf", COMBOS)
@@ -139,6 +139,22 @@ def test_pm_step_matches_oracle(gpu_lib, oracle, scheme, fds, gf):
     assert _rel(acc, ref) <= FORCE_TOL
     assert _rel(pm.potential(), phi) <= FORCE_TOL
     assert _rel(pm.get_grid(gpu_lib.GRID_FIELD), E) <= FORCE_TOL
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3])
+def test_centred_tsc_interpolation_variants_match_oracle(gpu_lib, oracle, variant):
+    # the textbook TSC through each interpolation kernel (tiled, field
+    # gather, potential gather) and a run of the fused kick/drift
+    grid = (64, 64, 64)
+    pos, vel, m = _particles(5000, grid, 61, clustered=True)
+    pm = _make(gpu_lib, grid, gpu_lib.TSC_CENTRED, 1, 2, a=3.0)
+    pm.set_tuning(interp_kernel=variant)
+    pm.set_particles(pos, vel, m)
+    pm.init_greens()
+    pm.pm_step()
+    acc = pm.get_particles()[2]
+    G = oracle.green_grid(2, 3, 1, 3.0, grid)
+    assert _rel(acc, oracle.pm_accel(pos, m, grid, 3, 1, G)) <= FORCE_TOL
 
 
 def test_external_field_matches_oracle(gpu_lib, oracle):
@@ -232,7 +248,9 @@ def test_p3m_common_mass_matches_oracle(gpu_lib, oracle, use_table, n_heavy, m0)
     # loop without a mass multiply and a per-i correction for the heavy ones;
     # 64 heavy particles is the most the correction list holds, 65 takes the
     # per-pair mass multiply, and a zero common mass leaves only the heavy
-    # particles' forces
+    # particles' forces.  The reference divides the pair forces by m_i
+    # (correctAccelerations), which is 0/0 for a massless particle, so that
+    # case is checked against an FP64 brute-force sum of m_j F(r) d_ij
     grid = (64, 64, 64)
     pos, vel, _ = _particles(6000, grid, 41, clustered=True)
     m = np.full(len(pos), np.float32(m0 / len(pos)))
@@ -246,8 +264,19 @@ def test_p3m_common_mass_matches_oracle(gpu_lib, oracle, use_table, n_heavy, m0)
     pm.p3m_short_range()
     acc = pm.get_particles()[2]
     H = 0.25
-    ref_sr, _ = oracle.p3m_short_range(pos, m, np.array(grid, np.float32) * H, 3 * H, H, 3.0, 3.0, 0.0, 1,
-                                       use_table=use_table)
+    if m0 > 0:
+        ref_sr, _ = oracle.p3m_short_range(pos, m, np.array(grid, np.float32) * H, 3 * H, H, 3.0, 3.0, 0.0, 1,
+                                           use_table=use_table)
+    else:
+        x, mj = pos.astype(np.float64), m.astype(np.float64)
+        ref_sr = np.zeros_like(x)
+        for i in range(len(x)):
+            d = x[i] - x
+            r2 = np.einsum("ij,ij->i", d, d)
+            sel = (r2 > 0.0) & (r2 < 9.0) & (mj > 0)
+            F = oracle.sr_pair_factor(1, 3.0, 0.0, r2[sel])
+            ref_sr[i] = ((mj[sel] * F)[:, None] * d[sel]).sum(0)
+        assert np.count_nonzero(np.any(ref_sr != 0.0, axis=1)) > 100
     assert _rel(acc - a_pm, ref_sr) <= 10 * FORCE_TOL
     G = oracle.green_grid(2, 2, 0, 3.0, grid)
     assert _rel(acc, oracle.pm_accel(pos, m, grid, 2, 0, G) + ref_sr) <= FORCE_TOL

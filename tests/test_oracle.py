@@ -26,7 +26,7 @@ def _rng(seed=0):
 # deposit (source/pmMethod.cpp:200-277)
 # --------------------------------------------------------------------------
 
-@pytest.mark.parametrize("scheme", [0, 1, 2])
+@pytest.mark.parametrize("scheme", [0, 1, 2, 3])
 def test_deposit_conserves_mass(oracle, scheme):
     r = _rng(1)
     pos = r.uniform(3, 13, (500, 3))
@@ -35,7 +35,7 @@ def test_deposit_conserves_mass(oracle, scheme):
     assert abs(rho.sum() - m.astype(np.float32).astype(np.float64).sum()) < 1e-9 * m.sum()
 
 
-@pytest.mark.parametrize("scheme", [1, 2])
+@pytest.mark.parametrize("scheme", [1, 2, 3])
 def test_deposit_first_moment_is_position(oracle, scheme):
     # sum_w w*(anchor+offset) = x for CIC and (floor-anchored) TSC
     r = _rng(2)
@@ -60,6 +60,38 @@ def test_tsc_weights_closed_form(oracle, delta):
     assert np.isclose((w * (np.array([-1, 0, 1]) - d) ** 2).sum(), 0.25, atol=1e-12)
     if delta == 0.25:
         assert np.allclose(w, [0.03125, 0.6875, 0.28125], atol=1e-12)
+
+
+@pytest.mark.parametrize("delta", [0.0, 0.25, 0.49, 0.5, 0.75, 0.999])
+def test_centred_tsc_weights_closed_form(oracle, delta):
+    # textbook TSC (an extension, scheme 3): the quadratics about the nearest
+    # grid point, d = x - round(x) in [-1/2, 1/2]: weights never negative,
+    # sum 1, mean at the particle, variance d^2 + 1/4 - d^2 = 1/4 about it
+    pos = np.array([[5 + delta, 5.0, 5.0]], np.float32)
+    x = float(np.float32(5 + delta))
+    c = int(np.floor(x + 0.5))
+    d = x - c
+    rho = oracle.deposit(pos, np.ones(1, np.float32), (16, 16, 16), 3)
+    w = rho[5, 5, c - 1:c + 2] / (0.75 ** 2)
+    assert rho[5, 5].sum() / 0.75 ** 2 == pytest.approx(1.0, abs=1e-12)  # nothing outside c-1..c+1
+    assert np.all(w >= 0.0)
+    assert np.allclose(w, [0.5 * (0.5 - d) ** 2, 0.75 - d * d, 0.5 * (0.5 + d) ** 2], atol=1e-12)
+    assert np.isclose((w * (np.arange(c - 1, c + 2) - x)).sum(), 0.0, atol=1e-12)
+    assert np.isclose((w * (np.arange(c - 1, c + 2) - x) ** 2).sum(), 0.25, atol=1e-12)
+
+
+def test_centred_tsc_equals_floor_tsc_below_half(oracle):
+    # the two anchors agree wherever floor(x) == round(x): fractional parts
+    # below 1/2 on every axis give identical densities, above 1/2 they differ
+    r = _rng(9)
+    base = r.integers(4, 12, (300, 3)).astype(np.float64)
+    lo = (base + r.uniform(0.0, 0.45, (300, 3))).astype(np.float32)
+    hi = (base + r.uniform(0.55, 0.95, (300, 3))).astype(np.float32)
+    m = r.uniform(0.5, 1.5, 300).astype(np.float32)
+    assert np.array_equal(oracle.deposit(lo, m, (16, 16, 16), 3), oracle.deposit(lo, m, (16, 16, 16), 2))
+    a, b = oracle.deposit(hi, m, (16, 16, 16), 3), oracle.deposit(hi, m, (16, 16, 16), 2)
+    assert np.abs(a - b).max() > 0.1 * np.abs(a).max()
+    assert b.min() < 0.0 <= a.min()  # floor-anchored centre weights go negative above sqrt(3)/2
 
 
 def test_cic_weights_closed_form(oracle):
@@ -221,7 +253,8 @@ def _point_mass_samples(oracle, scheme, fds, G, n, a, rng, reps, rmin=1.0):
 # (scheme, fds, bias, rms, band): NGP's force scatters over the source's cell
 # by tens of per cent, so its averages carry looser bounds
 _OPT_CASES = [(1, 0, 0.015, 0.06, 0.03), (2, 0, 0.015, 0.06, 0.03), (1, 1, 0.015, 0.06, 0.03),
-              (2, 1, 0.015, 0.06, 0.03), (0, 0, 0.09, 0.22, 0.03), (0, 1, 0.09, 0.22, 0.03)]
+              (2, 1, 0.015, 0.06, 0.03), (0, 0, 0.09, 0.22, 0.03), (0, 1, 0.09, 0.22, 0.03),
+              (3, 0, 0.015, 0.06, 0.03), (3, 1, 0.015, 0.06, 0.03)]
 
 
 @pytest.mark.parametrize("scheme,fds,bias,rms,band", _OPT_CASES)
@@ -249,6 +282,22 @@ def test_optimal_green_reproduces_reference_force(oracle, scheme, fds, bias, rms
 # a G that assumes another gradient stencil or another assignment window
 _MISMATCH = [((2, 1), (2, 0)), ((2, 0), (2, 1)), ((1, 1), (1, 0)), ((0, 0), (1, 0)), ((0, 1), (1, 1)),
              ((1, 0), (0, 0)), ((1, 0), (2, 0)), ((2, 1), (1, 1))]
+
+
+def test_centred_tsc_follows_the_reference_force_closer(oracle):
+    # the optimal G is built for the TSC window sinc^3 about the nearest grid
+    # point; the textbook (centred) TSC has exactly that window, the floor
+    # anchor does not, so on the same samples the centred scheme's PM force
+    # is closer to the S2 reference force (for both difference stencils)
+    n, a = 32, 4.0
+    for fds in (0, 1):
+        errs = []
+        for scheme in (2, 3):
+            G = oracle.green_grid(2, scheme, fds, a, (n, n, n))
+            _, acc, d, dist = _point_mass_samples(oracle, scheme, fds, G, n, a, _rng(21), 12, rmin=0.5)
+            R = np.array([oracle.reference_force(1, float(x), a) for x in dist]) - dist / (3 * n ** 3)
+            errs.append(np.sqrt(((acc + R[:, None] * d / dist[:, None]) ** 2).sum() / (R ** 2).sum()))
+        assert errs[1] < 0.8 * errs[0], (fds, errs)
 
 
 @pytest.mark.parametrize("used,built", _MISMATCH)
