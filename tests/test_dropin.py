@@ -252,13 +252,15 @@ def test_galaxy_timing_demo_with_lambda_field_matches_oracle(gpu_lib, oracle, tm
     assert np.linalg.norm(fin[0] - xs) / np.linalg.norm(xs) < 1e-4
     assert np.linalg.norm(fin[1] - vs) / np.linalg.norm(vs) < 1e-3
 
-    # expected momentum: initial (zero integer-step velocities) plus DT times
-    # the total external force at each recorded frame
+    # expected momentum: the initial momentum (integer-step velocities start
+    # as the velocities, source/particle.cpp:3-4) plus DT times the total
+    # external force at each recorded frame
     n_, nframes, rec = _read_frames(out / "positions.dat")
     assert (n_, nframes) == (n, steps + 1)
     F = np.array([(mass[:, None] * _sph_rad_decr(rec[t].astype(np.float64), center, rb, mb, G)).sum(0)
                   for t in range(steps + 1)])
-    want = DT * np.cumsum(F, axis=0)
+    p0 = (mass.astype(np.float32).astype(np.float64)[:, None] * vel.astype(np.float32)).sum(0)
+    want = p0 + DT * np.cumsum(F, axis=0)
     got = np.loadtxt(out / "expected_momentum.txt").reshape(-1, 3)
     assert got.shape == want.shape
     assert np.max(np.abs(got - want)) <= 1e-4 * np.abs(want).max()
