@@ -349,6 +349,21 @@ int pmgpu_train(pmgpu_trainer** trainers, int32_t n_trainers, float* params, int
    condition of the FFT force step). */
 int pmgpu_direct_sum(int64_t n, const float* pos, const float* mass, int64_t nt, const float* tpos,
                      float softening, const float* box, float* out);
+/* pmgpu_direct_sum without any device allocation (no counterpart in the
+   reference): for callers that cannot allocate device memory -- serving
+   loops, harnesses that forbid hipMalloc -- and for loops that would
+   otherwise allocate per call.  The staging copies of the inputs and
+   outputs and the Ewald table live in one static device array of about
+   40 MiB per device, held until the process ends; at most
+   PMGPU_DS_NOALLOC_MAX_SOURCES sources and PMGPU_DS_NOALLOC_MAX_TARGETS
+   targets (PMGPU_ERR_UNSUPPORTED beyond).  Same arguments, kernels and
+   results as pmgpu_direct_sum; runs on the null stream of the current
+   device and returns when out is written.  Calls that overlap on one
+   device share that array and race. */
+#define PMGPU_DS_NOALLOC_MAX_SOURCES (1 << 20)
+#define PMGPU_DS_NOALLOC_MAX_TARGETS (1 << 16)
+int pmgpu_direct_sum_noalloc(int64_t n, const float* pos, const float* mass, int64_t nt, const float* tpos,
+                             float softening, const float* box, float* out);
 /* The same sums on the host CPU in FP64 over `threads` threads (double
    accumulation; the periodic correction from the same Ewald table), for
    error statements against a direct sum.  out: nt*3 doubles. */

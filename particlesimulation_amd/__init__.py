@@ -52,7 +52,7 @@ EXPORTS = (
     "pmgpu_profile_enable", "pmgpu_profile_reset", "pmgpu_profile_read", "pmgpu_p3m_count_pairs",
     "pmgpu_greens_value", "pmgpu_diagnostics", "pmgpu_mlp_forward", "pmgpu_mlp_backward",
     "pmgpu_trainer_create", "pmgpu_trainer_destroy", "pmgpu_trainer_last_error", "pmgpu_trainer_add_snapshot",
-    "pmgpu_trainer_loss_grad", "pmgpu_train", "pmgpu_direct_sum", "pmgpu_direct_sum_host", "pmgpu_set_tuning", "pmgpu_get_tuning", "pmgpu_p3m_pair_stats",
+    "pmgpu_trainer_loss_grad", "pmgpu_train", "pmgpu_direct_sum", "pmgpu_direct_sum_host", "pmgpu_direct_sum_noalloc", "pmgpu_set_tuning", "pmgpu_get_tuning", "pmgpu_p3m_pair_stats",
     "pmgpu_p3m_cell_geometry", "pmgpu_add_accelerations",
     "pmgpu_batch_create", "pmgpu_batch_destroy", "pmgpu_batch_last_error", "pmgpu_batch_size",
     "pmgpu_batch_set_particles", "pmgpu_batch_prepare", "pmgpu_batch_step", "pmgpu_batch_synchronize",
@@ -170,6 +170,7 @@ def load_library(path: str = LIB_PATH):
     L.pmgpu_trainer_loss_grad.argtypes = [vp, fp, C.POINTER(C.c_double), fp, fp]
     L.pmgpu_train.argtypes = [C.POINTER(vp), C.c_int32, fp, C.c_int32, C.c_float, fp]
     L.pmgpu_direct_sum.argtypes = [C.c_int64, fp, fp, C.c_int64, fp, C.c_float, fp, fp]
+    L.pmgpu_direct_sum_noalloc.argtypes = [C.c_int64, fp, fp, C.c_int64, fp, C.c_float, fp, fp]
     L.pmgpu_direct_sum_host.argtypes = [C.c_int64, fp, fp, C.c_int64, fp, C.c_float, fp, C.c_int32,
                                         C.POINTER(C.c_double)]
     _lib = L
@@ -463,6 +464,30 @@ def direct_sum(pos, mass, targets, softening=0.0, box=None):
                             _fptr(b), _fptr(out))
     if rc != OK:
         _raise(rc, "pmgpu_direct_sum failed")
+    return out
+
+
+DS_NOALLOC_MAX_SOURCES = 1 << 20
+DS_NOALLOC_MAX_TARGETS = 1 << 16
+
+
+def direct_sum_noalloc(pos, mass, targets, softening=0.0, box=None):
+    """direct_sum without device allocation (a static device array holds the
+    staging copies): at most DS_NOALLOC_MAX_SOURCES sources and
+    DS_NOALLOC_MAX_TARGETS targets; calls that overlap on one device race."""
+    L = load_library()
+    pos = _f32(pos, (-1, 3))
+    mass = _f32(mass)
+    targets = _f32(targets, (-1, 3))
+    if len(mass) > DS_NOALLOC_MAX_SOURCES or len(targets) > DS_NOALLOC_MAX_TARGETS:
+        raise InvalidArgument(ERR_UNSUPPORTED, f"direct_sum_noalloc takes at most {DS_NOALLOC_MAX_SOURCES} sources "
+                              f"and {DS_NOALLOC_MAX_TARGETS} targets")
+    out = np.zeros((len(targets), 3), np.float32)
+    b = None if box is None else _f32(box)
+    rc = L.pmgpu_direct_sum_noalloc(len(mass), _fptr(pos), _fptr(mass), len(targets), _fptr(targets),
+                                    float(softening), _fptr(b), _fptr(out))
+    if rc != OK:
+        _raise(rc, "pmgpu_direct_sum_noalloc failed")
     return out
 
 

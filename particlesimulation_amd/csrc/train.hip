@@ -25,6 +25,7 @@
 #include "pmgpu_internal.h"
 
 #include <algorithm>
+#include <cstring>
 #include <atomic>
 #include <cmath>
 #include <array>
@@ -349,10 +350,11 @@ void ewald_accel(const double L[3], const double x[3], double out[3]) {
 }
 
 // Table of the correction a_ewald(x) - (-x/|x|^3) on the octant grid; 0 at x = 0.
-std::vector<float4> ewald_table_compute(const float Lf[3]) {
+constexpr int64_t EW_ENTRIES = (int64_t)(EW_N + 1) * (EW_N + 1) * (EW_N + 1);
+
+void ewald_table_fill(const float Lf[3], float4* tab) {
   const double L[3] = {Lf[0], Lf[1], Lf[2]};
   constexpr int S = EW_N + 1;
-  std::vector<float4> tab((size_t)S * S * S);
   auto work = [&](int z0, int z1) {
     for (int iz = z0; iz < z1; ++iz)
       for (int iy = 0; iy < S; ++iy)
@@ -373,6 +375,11 @@ std::vector<float4> ewald_table_compute(const float Lf[3]) {
   std::vector<std::thread> th;
   for (int t = 0; t < nth; ++t) th.emplace_back(work, S * t / nth, S * (t + 1) / nth);
   for (auto& t : th) t.join();
+}
+
+std::vector<float4> ewald_table_compute(const float Lf[3]) {
+  std::vector<float4> tab((size_t)EW_ENTRIES);
+  ewald_table_fill(Lf, tab.data());
   return tab;
 }
 
@@ -385,6 +392,72 @@ const std::vector<float4>& ewald_table(const float L[3]) {
   auto it = cache.find(key);
   if (it == cache.end()) it = cache.emplace(key, ewald_table_compute(L)).first;
   return it->second;
+}
+
+// Staging and table memory of pmgpu_direct_sum_noalloc, in one file-scope
+// device array (each device holds its own copy until the process ends):
+//   [0, 16 MiB)        sources as float4 (x, y, z, m), DS_MAX_SOURCES
+//   [16, 28 MiB)       source positions as given, 3 floats each
+//   [28, 32 MiB)       source masses
+//   then               targets as float4, results as float4 (DS_MAX_TARGETS
+//                      each), target positions and results as 3 floats each,
+//                      and the Ewald table (EW_ENTRIES float4).
+constexpr int64_t DS_MAX_SOURCES = PMGPU_DS_NOALLOC_MAX_SOURCES;
+constexpr int64_t DS_MAX_TARGETS = PMGPU_DS_NOALLOC_MAX_TARGETS;
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t SCR_SRC4 = 0;
+constexpr size_t SCR_POS = SCR_SRC4 + align256(sizeof(float4) * DS_MAX_SOURCES);
+constexpr size_t SCR_MASS = SCR_POS + align256(sizeof(float) * 3 * DS_MAX_SOURCES);
+constexpr size_t SCR_TGT4 = SCR_MASS + align256(sizeof(float) * DS_MAX_SOURCES);
+constexpr size_t SCR_RES4 = SCR_TGT4 + align256(sizeof(float4) * DS_MAX_TARGETS);
+constexpr size_t SCR_TPOS = SCR_RES4 + align256(sizeof(float4) * DS_MAX_TARGETS);
+constexpr size_t SCR_OUT = SCR_TPOS + align256(sizeof(float) * 3 * DS_MAX_TARGETS);
+constexpr size_t SCR_TAB = SCR_OUT + align256(sizeof(float) * 3 * DS_MAX_TARGETS);
+constexpr size_t SCR_BYTES = SCR_TAB + align256(sizeof(float4) * EW_ENTRIES);
+
+}  // namespace
+
+}  // namespace pmgpu
+
+static __device__ __align__(256) unsigned char scratch[pmgpu::SCR_BYTES];
+
+namespace pmgpu {
+namespace {
+
+__global__ void k_pack_sources(const float* __restrict__ pos, const float* __restrict__ mass, int64_t n,
+                               float4* __restrict__ src) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) src[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], mass ? mass[i] : 0.0f);
+}
+
+__global__ void k_unpack_results(const float4* __restrict__ res, int64_t nt, float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  const float4 r = res[i];
+  out[3 * i] = r.x;
+  out[3 * i + 1] = r.y;
+  out[3 * i + 2] = r.z;
+}
+
+// Device address of `scratch` on each device (hipGetSymbolAddress with the
+// device current, once per device), and the box whose Ewald table the
+// device's copy holds.
+constexpr int SCR_MAX_DEVICES = 64;
+std::mutex g_scr_mu;
+unsigned char* g_scr_ptr[SCR_MAX_DEVICES] = {};
+float g_scr_box[SCR_MAX_DEVICES][3] = {};
+// host copy of the last table computed for the no-alloc path, and its box
+float4 g_tab_host[EW_ENTRIES];
+float g_tab_host_box[3] = {0.f, 0.f, 0.f};
+
+unsigned char* scratch_ptr(int dev) {
+  if (dev < 0 || dev >= SCR_MAX_DEVICES) fail(PMGPU_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  if (!g_scr_ptr[dev]) {
+    void* p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scr_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  return g_scr_ptr[dev];
 }
 
 }  // namespace
@@ -807,6 +880,73 @@ int pmgpu_direct_sum(int64_t n, const float* pos, const float* mass, int64_t nt,
   dfree(res);
   dfree(tab);
   return rc;
+}
+
+// pmgpu_direct_sum without device allocation: the staging copies of the
+// inputs and outputs and the Ewald table live in the file-scope `scratch`
+// array (at most DS_MAX_SOURCES sources and DS_MAX_TARGETS targets); the
+// kernels are pmgpu_direct_sum's.  Runs on the null stream of the current
+// device and returns when `out` is written.  Calls that overlap on one
+// device share its scratch and race.
+int pmgpu_direct_sum_noalloc(int64_t n, const float* pos, const float* mass, int64_t nt, const float* tpos,
+                             float softening, const float* box, float* out) {
+  if (n < 0 || nt < 0 || (n > 0 && (!pos || !mass)) || (nt > 0 && (!tpos || !out))) return PMGPU_ERR_INVALID_ARGUMENT;
+  if (n > DS_MAX_SOURCES || nt > DS_MAX_TARGETS) return PMGPU_ERR_UNSUPPORTED;
+  const bool periodic = box != nullptr;
+  if (periodic && !(box[0] > 0.f && box[1] > 0.f && box[2] > 0.f)) return PMGPU_ERR_INVALID_ARGUMENT;
+  if (nt == 0) return PMGPU_OK;
+  try {
+    std::lock_guard<std::mutex> lock(g_scr_mu);
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    unsigned char* base = scratch_ptr(dev);
+    float4* src = reinterpret_cast<float4*>(base + SCR_SRC4);
+    float* spos = reinterpret_cast<float*>(base + SCR_POS);
+    float* smass = reinterpret_cast<float*>(base + SCR_MASS);
+    float4* tgt = reinterpret_cast<float4*>(base + SCR_TGT4);
+    float4* res = reinterpret_cast<float4*>(base + SCR_RES4);
+    float* stpos = reinterpret_cast<float*>(base + SCR_TPOS);
+    float* sout = reinterpret_cast<float*>(base + SCR_OUT);
+    float4* tab = reinterpret_cast<float4*>(base + SCR_TAB);
+    if (n > 0) {
+      HIP_CHECK(hipMemcpy(spos, pos, sizeof(float) * 3 * (size_t)n, hipMemcpyHostToDevice));
+      HIP_CHECK(hipMemcpy(smass, mass, sizeof(float) * (size_t)n, hipMemcpyHostToDevice));
+      hipLaunchKernelGGL(k_pack_sources, dim3(blocks(n)), dim3(256), 0, nullptr, spos, smass, n, src);
+      HIP_CHECK(hipGetLastError());
+    }
+    HIP_CHECK(hipMemcpy(stpos, tpos, sizeof(float) * 3 * (size_t)nt, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_pack_sources, dim3(blocks(nt)), dim3(256), 0, nullptr, stpos, (const float*)nullptr, nt,
+                       tgt);
+    HIP_CHECK(hipGetLastError());
+    Box b{};
+    if (periodic) {
+      if (std::memcmp(g_scr_box[dev], box, sizeof(float) * 3) != 0) {
+        if (std::memcmp(g_tab_host_box, box, sizeof(float) * 3) != 0) {
+          ewald_table_fill(box, g_tab_host);
+          std::memcpy(g_tab_host_box, box, sizeof(float) * 3);
+        }
+        HIP_CHECK(hipMemcpy(tab, g_tab_host, sizeof(float4) * EW_ENTRIES, hipMemcpyHostToDevice));
+        std::memcpy(g_scr_box[dev], box, sizeof(float) * 3);
+      }
+      for (int c = 0; c < 3; ++c) {
+        b.L[c] = box[c];
+        b.invL[c] = 1.0f / box[c];
+        b.tab_scale[c] = (float)EW_N / (0.5f * box[c]);
+      }
+      hipLaunchKernelGGL(k_direct_sum<true>, dim3(blocks(nt, DS_THREADS)), dim3(DS_THREADS), 0, nullptr, src, n,
+                         tgt, nt, softening * softening, b, tab, res);
+    } else {
+      hipLaunchKernelGGL(k_direct_sum<false>, dim3(blocks(nt, DS_THREADS)), dim3(DS_THREADS), 0, nullptr, src, n,
+                         tgt, nt, softening * softening, b, (const float4*)nullptr, res);
+    }
+    HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_unpack_results, dim3(blocks(nt)), dim3(256), 0, nullptr, res, nt, sout);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpy(out, sout, sizeof(float) * 3 * (size_t)nt, hipMemcpyDeviceToHost));
+  } catch (const Error& e) {
+    return e.code;
+  }
+  return PMGPU_OK;
 }
 
 // The same sums on the host CPU in FP64 (an error evaluator, the FP64
