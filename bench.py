@@ -20,7 +20,9 @@ The line carries, for the headline workload (the 1M-particle P3M disk on
 and two more measured blocks:
   at_error_target   the same disk and grid at the smallest cutoff whose
                 force error is below 1e-3 (textbook TSC, 4-point gradient,
-                a = rc = 5 cells): throughput, roofline, error, CPU baseline,
+                a = rc = 3.5 cells): throughput, roofline, error, CPU baseline;
+                at_error_target_reference_tsc the same with the reference's
+                floor-anchored TSC and 2-point gradient (a = rc = 5 cells),
   learned_vs_plain  the learned influence-function correction trained on the
                 GPU within this run, then plain PM, learned PM and P3M on the
                 4M-particle clustered workload on 256^3: error against the
@@ -343,9 +345,7 @@ def main():
         except Exception as e:  # the baseline must not hide the GPU measurement
             log(f"[bench] cpu baseline failed: {e!r}")
 
-    at_err = None
-    if not args.no_at_error and w.name == ics.C2_P3M_DISK.name:
-        we = ics.C2_P3M_DISK_ERR
+    def at_error(we):
         pe = make_context(P, we, pos, vel, m)
         e_err = ds.error(force_of(pe, we)) if ds else None
         log(f"[bench] {we.name} force error: {e_err}")
@@ -354,12 +354,17 @@ def main():
         c = None
         if cpu and cpu["sr_pairs"]:
             c = cpu_baseline(we, pos, m, sr_scale_from=(cpu["sr_s"], cpu["sr_pairs"], r["pairs"]))
-        at_err = {"config": workload_config(we, args.seed), "value": r["value"], "unit": "particle-updates/s",
-                  "ms_per_step": r["ms_per_step"], "force_error": e_err,
-                  "error_target_met": bool(e_err and e_err["e_rms"] <= 1e-3), "roofline": r["roofline"],
-                  "stage_ms_per_step": r["stage_ms_per_step"], "cpu_baseline": c,
-                  "gpu_vs_cpu": (r["value"] / c["value"]) if c else None, "iterations": r["iterations"],
-                  "escaped": r["escaped"]}
+        return {"config": workload_config(we, args.seed), "value": r["value"], "unit": "particle-updates/s",
+                "ms_per_step": r["ms_per_step"], "force_error": e_err,
+                "error_target_met": bool(e_err and e_err["e_rms"] <= 1e-3), "roofline": r["roofline"],
+                "stage_ms_per_step": r["stage_ms_per_step"], "cpu_baseline": c,
+                "gpu_vs_cpu": (r["value"] / c["value"]) if c else None, "iterations": r["iterations"],
+                "escaped": r["escaped"]}
+
+    at_err = at_err_ref = None
+    if not args.no_at_error and w.name == ics.C2_P3M_DISK.name:
+        at_err = at_error(ics.C2_P3M_DISK_ERR)
+        at_err_ref = at_error(ics.C2_P3M_DISK_ERR_REF)
 
     learned = None
     if not args.no_learned and w.name == ics.C2_P3M_DISK.name:
@@ -378,6 +383,7 @@ def main():
         "cpu_baseline": cpu,
         "force_error": err,
         "at_error_target": at_err,
+        "at_error_target_reference_tsc": at_err_ref,
         "learned_vs_plain": learned,
         "stage_ms_per_step": res["stage_ms_per_step"],
         "hbm_stages": res["hbm_stages"],

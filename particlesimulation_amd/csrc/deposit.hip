@@ -118,14 +118,29 @@ __global__ void k_deposit_plan(int ntiles, int ntx, int nty, int ntz, int nrange
   slot[t] = s;
 }
 
-// Fixed-point value round(x * 2^shift) of a float contribution, through FP64
-// (exact scaling; FP64 runs at the FP32 rate here): hi/lo 32-bit halves.
+// Fixed-point value of a float contribution, round(x * 2^shift) with ties to
+// even, as a two's-complement 64-bit integer, from the float's bits with
+// integer operations only: x = +-mant * 2^(e - 150) (mant the 24-bit
+// significand, e the biased exponent), so x * 2^shift is mant shifted by
+// s = e - 150 + shift.  s <= 38 for every contribution (the host keeps
+// 1.5 * sum|m| * 2^shift below 2^62); it is clamped so that no input shifts
+// out of range.
 __device__ __forceinline__ unsigned long long to_fixed(float x, int shift) {
-  const double d = rint(ldexp((double)x, shift));
-  const double hi = floor(d * 0x1p-32);
-  const double lo = fma(hi, -0x1p32, d);  // in [0, 2^32), exact
-  const unsigned int h = (unsigned int)(int)hi, l = (unsigned int)lo;
-  return ((unsigned long long)h << 32) | l;
+  const uint32_t b = __float_as_uint(x);
+  const int e = (int)((b >> 23) & 0xffu);
+  const uint32_t mant = (b & 0x7fffffu) | (e ? 0x800000u : 0u);
+  const int s = (e ? e : 1) - 150 + shift;
+  unsigned long long mag;
+  if (s >= 0) {
+    mag = (unsigned long long)mant << (s < 39 ? s : 39);
+  } else if (s < -25) {
+    mag = 0ull;  // |x| 2^shift < 1/2
+  } else {
+    const int k = -s;
+    const uint32_t q = mant >> k, rem = mant & ((1u << k) - 1u), half = 1u << (k - 1);
+    mag = q + ((rem > half || (rem == half && (q & 1u))) ? 1u : 0u);
+  }
+  return (b >> 31) ? 0ull - mag : mag;
 }
 
 template <int SCHEME>

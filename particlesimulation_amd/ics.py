@@ -164,13 +164,19 @@ C2_P3M_DISK = Workload("p3m_disk_1M_128", 1048576, (128, 128, 128), 32.0, 2, 0, 
                        a_cells=3.0, rc_cells=3.0, kind="disk")
 C3_PM_CLUMPS = Workload("pm_clumps_16M_512", 16777216, (512, 512, 512), 1.0, 2, 1, 0, 1e-3, 20, kind="clumps",
                         periodic=True)
-# the same disk and grid at the smallest cutoff (in whole cells) whose force
-# error against an FP64 direct sum is below 1e-3: the textbook (centred) TSC,
-# the 4-point gradient and the S2-optimal G with a = rc = 5 cells
+# the same disk and grid at the smallest cutoff (in half cells) whose force
+# error against an FP64 direct sum (65 536 seeded targets) is below 1e-3:
+# the textbook (centred) TSC, the 4-point gradient and the S2-optimal G with
+# a = rc = 3.5 cells (7.7e-4; 3 cells give 1.6e-3) ...
 C2_P3M_DISK_ERR = Workload("p3m_disk_1M_128_err1e-3", 1048576, (128, 128, 128), 32.0, 3, 1, 2, 0.01, 50,
-                           p3m=True, a_cells=5.0, rc_cells=5.0, kind="disk")
+                           p3m=True, a_cells=3.5, rc_cells=3.5, kind="disk")
+# ... and with the reference's own scheme (floor-anchored TSC, 2-point
+# gradient): a = rc = 5 cells (7.9e-4; 4 cells give 1.7e-3)
+C2_P3M_DISK_ERR_REF = Workload("p3m_disk_1M_128_err1e-3_reference_tsc", 1048576, (128, 128, 128), 32.0, 2, 0, 2,
+                               0.01, 50, p3m=True, a_cells=5.0, rc_cells=5.0, kind="disk")
 # the learned-correction workload: plain PM uses the discrete-Laplacian G, the
 # learned PM multiplies it by the network's factor
 C5_PM_CLUMPS_256 = Workload("pm_clumps_4M_256", 4194304, (256, 256, 256), 1.0, 2, 1, 0, 1e-3, 20,
                             kind="clumps", periodic=True)
-WORKLOADS = {w.name: w for w in (C1_PM_PLUMMER, C2_P3M_DISK, C2_P3M_DISK_ERR, C3_PM_CLUMPS, C5_PM_CLUMPS_256)}
+WORKLOADS = {w.name: w for w in (C1_PM_PLUMMER, C2_P3M_DISK, C2_P3M_DISK_ERR, C2_P3M_DISK_ERR_REF, C3_PM_CLUMPS,
+                                 C5_PM_CLUMPS_256)}

@@ -33,7 +33,9 @@ def test_workload_configs_name_the_error_target_workload():
     from particlesimulation_amd import ics
     c = bench.workload_config(ics.C2_P3M_DISK_ERR, 42)
     assert c["interpolation"] == "TSC_CENTRED" and c["finite_diff"] == "FOUR_POINT"
-    assert c["a_cells"] == c["cutoff_cells"] == 5.0 and c["n"] == 1048576 and c["grid"] == [128, 128, 128]
+    assert c["a_cells"] == c["cutoff_cells"] == 3.5 and c["n"] == 1048576 and c["grid"] == [128, 128, 128]
+    c = bench.workload_config(ics.C2_P3M_DISK_ERR_REF, 42)
+    assert c["interpolation"] == "TSC" and c["finite_diff"] == "TWO_POINT" and c["a_cells"] == 5.0
     json.dumps(bench.workload_config(ics.C2_P3M_DISK, 42))
 
 
