@@ -331,7 +331,10 @@ int pmgpu_create(const pmgpu_pm_params* params, pmgpu_ctx** out) {
     c->nz = params->grid[2];
     c->nxh = c->nx / 2 + 1;
     c->M = (int64_t)c->nx * c->ny * c->nz;
-    c->K = (int64_t)c->nxh * c->ny * c->nz;
+    // rows of the half spectrum start on 128-byte boundaries, so the y and z
+    // passes' column tiles (16 or 32 complex values) are whole cache lines
+    c->sp = (c->nxh + 15) / 16 * 16;
+    c->K = (int64_t)c->sp * c->ny * c->nz;
     c->ntx = c->nx / TILE_X;
     c->nty = c->ny / TILE_Y;
     c->ntz = c->nz / TILE_Z;
@@ -730,9 +733,11 @@ int pmgpu_get_grid(pmgpu_ctx* c, int32_t which, float* out) {
       HIP_CHECK(hipMemcpy(out, c->field, sizeof(float) * 3 * c->M, hipMemcpyDeviceToHost));
       break;
     case PMGPU_GRID_GREENS: {
-      HIP_CHECK(hipMemcpy(out, c->greens, sizeof(float) * c->K, hipMemcpyDeviceToHost));
+      std::vector<float> g((size_t)c->K);
+      HIP_CHECK(hipMemcpy(g.data(), c->greens, sizeof(float) * c->K, hipMemcpyDeviceToHost));
       const double m = (double)c->M;
-      for (int64_t i = 0; i < c->K; ++i) out[i] = (float)((double)out[i] * m);
+      for (int64_t r = 0; r < (int64_t)c->ny * c->nz; ++r)
+        for (int kx = 0; kx < c->nxh; ++kx) out[r * c->nxh + kx] = (float)((double)g[r * c->sp + kx] * m);
       break;
     }
     default:

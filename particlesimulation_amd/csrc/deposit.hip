@@ -119,28 +119,18 @@ __global__ void k_deposit_plan(int ntiles, int ntx, int nty, int ntz, int nrange
 }
 
 // Fixed-point value of a float contribution, round(x * 2^shift) with ties to
-// even, as a two's-complement 64-bit integer, from the float's bits with
-// integer operations only: x = +-mant * 2^(e - 150) (mant the 24-bit
-// significand, e the biased exponent), so x * 2^shift is mant shifted by
-// s = e - 150 + shift.  s <= 38 for every contribution (the host keeps
-// 1.5 * sum|m| * 2^shift below 2^62); it is clamped so that no input shifts
-// out of range.
-__device__ __forceinline__ unsigned long long to_fixed(float x, int shift) {
-  const uint32_t b = __float_as_uint(x);
-  const int e = (int)((b >> 23) & 0xffu);
-  const uint32_t mant = (b & 0x7fffffu) | (e ? 0x800000u : 0u);
-  const int s = (e ? e : 1) - 150 + shift;
-  unsigned long long mag;
-  if (s >= 0) {
-    mag = (unsigned long long)mant << (s < 39 ? s : 39);
-  } else if (s < -25) {
-    mag = 0ull;  // |x| 2^shift < 1/2
-  } else {
-    const int k = -s;
-    const uint32_t q = mant >> k, rem = mant & ((1u << k) - 1u), half = 1u << (k - 1);
-    mag = q + ((rem > half || (rem == half && (q & 1u))) ? 1u : 0u);
-  }
-  return (b >> 31) ? 0ull - mag : mag;
+// even, as a two's-complement 64-bit integer, in FP32 and integer operations
+// only (scale = 2^shift).  t = x * scale is exact, and |t| < 2^62 for every
+// contribution (the host keeps 1.5 * sum|m| * 2^shift below 2^62).  Then
+// hi = trunc(t / 2^31) is exact (at most 24 significant bits), the remainder
+// t - hi 2^31 is exact in FP32 (|rem| < 2^31, a multiple of t's ulp), and
+// hi 2^31 is even, so hi 2^31 + rint(rem) is t rounded to nearest even.
+__device__ __forceinline__ unsigned long long to_fixed(float x, float scale) {
+  const float t = x * scale;
+  const int hi = (int)(t * 0x1p-31f);
+  const float rem = fmaf((float)hi, -0x1p31f, t);
+  const int lo = (int)rintf(rem);
+  return (unsigned long long)(((long long)hi << 31) + (long long)lo);
 }
 
 template <int SCHEME>
@@ -157,6 +147,7 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit(
   __shared__ int s_tile, s_chunk, s_nchunk;
 
   const int total = chunk_start[ntiles];
+  const float scale = ldexpf(1.0f, shift);
   constexpr int LO = SCHEME == PMGPU_TSC ? -1 : 0;
   constexpr int CNT = SCHEME == PMGPU_TSC ? 3 : (SCHEME == PMGPU_CIC ? 2 : 1);
 
@@ -256,7 +247,7 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit(
           for (int k = 0; k < CNT; ++k) {
             if (!okz[k]) continue;
             const float t3 = t2 * wz[k];
-            atomicAdd(&acc[lx[i] + TILE_X * (ly[j] + TILE_Y * lz[k])], to_fixed(t3, shift));
+            atomicAdd(&acc[lx[i] + TILE_X * (ly[j] + TILE_Y * lz[k])], to_fixed(t3, scale));
           }
         }
       }

@@ -10,7 +10,9 @@
 //              inverse along z -- one kernel, the pencil never leaves LDS;
 //   y        : inverse;
 //   x  (C2R): Nx/2+1 complex -> Nx reals.
-// The spectrum is stored [z][y][kx] with pitch Nx/2+1 (kx fastest).  Every
+// The spectrum is stored [z][y][kx] (kx fastest) with row pitch SP: Nx/2+1
+// rounded up to a multiple of 16, so that every row, and with it every
+// column tile of the y and z passes, starts on a 128-byte boundary.  Every
 // 1-D transform is a Stockham autosort FFT of radix-8 / radix-4 stages with
 // L/8 threads per transform; the inverse is unnormalised and 1/(NxNyNz) is
 // folded into G (the reference's adapters divide after the inverse,
@@ -152,7 +154,7 @@ __device__ __forceinline__ void fft_lds(float2* buf, int cs, int ns, int col, in
 // workgroup of 256 threads.
 template <int L>
 __global__ __launch_bounds__(256) void k_fft_x_r2c(const float* __restrict__ rho,
-                                                   float2* __restrict__ spec,
+                                                   float2* __restrict__ spec, int sp,
                                                    const float2* __restrict__ tw,
                                                    const float2* __restrict__ tw_r2c,
                                                    const int32_t* __restrict__ escape,
@@ -166,7 +168,7 @@ __global__ __launch_bounds__(256) void k_fft_x_r2c(const float* __restrict__ rho
   __syncthreads();
   const int col = threadIdx.x / T, t = threadIdx.x % T;
   fft_lds<L, false>(buf, P, 1, col, t, true, tw);
-  float2* dst = spec + (int64_t)(row0 + col) * (L + 1);
+  float2* dst = spec + (int64_t)(row0 + col) * sp;
   for (int k = t; k <= L; k += T) {
     float2 zk = buf[col * P + (k & (L - 1))];
     float2 zc = conjf2(buf[col * P + ((L - k) & (L - 1))]);
@@ -178,7 +180,7 @@ __global__ __launch_bounds__(256) void k_fft_x_r2c(const float* __restrict__ rho
 
 // x pass, complex -> real (unnormalised).
 template <int L>
-__global__ __launch_bounds__(256) void k_fft_x_c2r(const float2* __restrict__ spec,
+__global__ __launch_bounds__(256) void k_fft_x_c2r(const float2* __restrict__ spec, int sp,
                                                    float* __restrict__ phi,
                                                    const float2* __restrict__ tw,
                                                    const float2* __restrict__ tw_r2c,
@@ -188,13 +190,13 @@ __global__ __launch_bounds__(256) void k_fft_x_c2r(const float2* __restrict__ sp
   constexpr int T = L / 8, ROWS = 256 / T, P = L + 2;
   __shared__ float2 buf[ROWS * P];
   const int row0 = blockIdx.x * ROWS;
-  const float2* src = spec + (int64_t)row0 * (L + 1);
+  const float2* src = spec + (int64_t)row0 * sp;
   constexpr int PER = (ROWS * (L + 1) + 255) / 256;
   float2 v[PER];
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const int e = threadIdx.x + q * 256;
-    if (e < ROWS * (L + 1)) v[q] = src[e];
+    if (e < ROWS * (L + 1)) v[q] = src[(e / (L + 1)) * sp + e % (L + 1)];
   }
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
@@ -244,7 +246,7 @@ __device__ __forceinline__ void col_fft(float2* buf, int ncols, const float2* tw
 
 // y pass (forward or inverse), in place on the spectrum.
 template <int L, int TW, bool INV>
-__global__ __launch_bounds__(TW * (L / 8)) void k_fft_y(float2* __restrict__ spec, int nxh,
+__global__ __launch_bounds__(TW * (L / 8)) void k_fft_y(float2* __restrict__ spec, int sp,
                                                                int ntile,
                                                                const float2* __restrict__ tw,
                                                                const int32_t* __restrict__ escape,
@@ -255,7 +257,7 @@ __global__ __launch_bounds__(TW * (L / 8)) void k_fft_y(float2* __restrict__ spe
   const int tile = blockIdx.x, z = blockIdx.y;
   const int c0 = tile * CT::TW;
   const int ncols = tile == ntile - 1 ? CT::TW + 1 : CT::TW;
-  float2* base = spec + (int64_t)z * L * nxh + c0;
+  float2* base = spec + (int64_t)z * L * sp + c0;
   // all of a thread's loads are issued before the first LDS store
   constexpr int PER = (L * CT::NC + CT::THREADS - 1) / CT::THREADS;
   float2 v[PER];
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(TW * (L / 8)) void k_fft_y(float2* __restrict__ spe
     const int e = threadIdx.x + q * CT::THREADS;
     if (e < L * ncols) {
       const int n = e / ncols, col = e % ncols;
-      v[q] = base[(int64_t)n * nxh + col];
+      v[q] = base[(int64_t)n * sp + col];
     }
   }
 #pragma unroll
@@ -276,7 +278,7 @@ __global__ __launch_bounds__(TW * (L / 8)) void k_fft_y(float2* __restrict__ spe
   col_fft<L, TW, INV>(buf, ncols, tw);
   for (int e = threadIdx.x; e < L * ncols; e += CT::THREADS) {
     const int n = e / ncols, col2 = e % ncols;
-    base[(int64_t)n * nxh + col2] = buf[n * CT::NC + col2];
+    base[(int64_t)n * sp + col2] = buf[n * CT::NC + col2];
   }
 }
 
@@ -288,7 +290,7 @@ constexpr int Z_GREEN = 0, Z_FWD = 1, Z_MUL_INV = 2;
 
 template <int L, int TW, int MODE>
 __global__ __launch_bounds__(TW * (L / 8)) void k_fft_z(
-    const float2* __restrict__ src, float2* __restrict__ dst, const float* __restrict__ greens, int nxh,
+    const float2* __restrict__ src, float2* __restrict__ dst, const float* __restrict__ greens, int sp,
     int ny, int ntile, const float2* __restrict__ tw, const int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
   using CT = ColTile<L, TW>;
@@ -296,8 +298,8 @@ __global__ __launch_bounds__(TW * (L / 8)) void k_fft_z(
   const int tile = blockIdx.x, y = blockIdx.y;
   const int c0 = tile * CT::TW;
   const int ncols = tile == ntile - 1 ? CT::TW + 1 : CT::TW;
-  const int64_t zstride = (int64_t)ny * nxh;
-  const int64_t off = (int64_t)y * nxh + c0;
+  const int64_t zstride = (int64_t)ny * sp;
+  const int64_t off = (int64_t)y * sp + c0;
   // all of a thread's loads (and its G values) are issued before the first
   // LDS store
   constexpr int PER = (L * CT::NC + CT::THREADS - 1) / CT::THREADS;
@@ -364,7 +366,7 @@ void launch_y_tw(pmgpu_ctx* c, float2* spec) {
   using CT = ColTile<L, TW>;
   const int ntile = (c->nx / 2) / CT::TW;
   hipLaunchKernelGGL((k_fft_y<L, TW, INV>), dim3(ntile, c->nz), dim3(CT::THREADS), 0, c->stream, spec,
-                     c->nxh, ntile, c->tw_y, c->escape, c->iter);
+                     c->sp, ntile, c->tw_y, c->escape, c->iter);
 }
 
 template <int L, bool INV>
@@ -380,7 +382,7 @@ void launch_z_tw(pmgpu_ctx* c, const float2* src, float2* dst, const float* g) {
   using CT = ColTile<L, TW>;
   const int ntile = (c->nx / 2) / CT::TW;
   hipLaunchKernelGGL((k_fft_z<L, TW, MODE>), dim3(ntile, c->ny), dim3(CT::THREADS), 0, c->stream, src, dst,
-                     g, c->nxh, c->ny, ntile, c->tw_z, c->escape, c->iter);
+                     g, c->sp, c->ny, ntile, c->tw_z, c->escape, c->iter);
 }
 
 template <int L, int MODE>
@@ -422,14 +424,14 @@ void z_pass(pmgpu_ctx* c, const float2* src, float2* dst, const float* g) {
 void x_r2c(pmgpu_ctx* c, const float* in, float2* spec) {
   const int L = c->nx / 2;
   const dim3 gx(c->ny * c->nz / rows_per_block(L));
-  PMGPU_LAUNCH_X(k_fft_x_r2c, L, gx, c->stream, in, spec, c->tw_x, c->tw_r2c, c->escape, c->iter);
+  PMGPU_LAUNCH_X(k_fft_x_r2c, L, gx, c->stream, in, spec, c->sp, c->tw_x, c->tw_r2c, c->escape, c->iter);
   HIP_CHECK(hipGetLastError());
 }
 
 void x_c2r(pmgpu_ctx* c, const float2* spec, float* out) {
   const int L = c->nx / 2;
   const dim3 gx(c->ny * c->nz / rows_per_block(L));
-  PMGPU_LAUNCH_X(k_fft_x_c2r, L, gx, c->stream, spec, out, c->tw_x, c->tw_r2c, c->escape, c->iter);
+  PMGPU_LAUNCH_X(k_fft_x_c2r, L, gx, c->stream, spec, c->sp, out, c->tw_x, c->tw_r2c, c->escape, c->iter);
   HIP_CHECK(hipGetLastError());
 }
 

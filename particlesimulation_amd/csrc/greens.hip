@@ -125,13 +125,17 @@ __host__ __device__ double green_at(int gf, int is, int fds, double a, int kx, i
                        nz);
 }
 
-__global__ void k_greens(int gf, int is, int fds, double a, int nx, int ny, int nz,
+__global__ void k_greens(int gf, int is, int fds, double a, int nx, int ny, int nz, int sp,
                          const float* __restrict__ corr, float* __restrict__ out) {
   const int nxh = nx / 2 + 1;
-  const int64_t K = (int64_t)nxh * ny * nz;
+  const int64_t K = (int64_t)sp * ny * nz;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= K) return;
-  const int kx = (int)(i % nxh), ky = (int)((i / nxh) % ny), kz = (int)(i / ((int64_t)nxh * ny));
+  const int kx = (int)(i % sp), ky = (int)((i / sp) % ny), kz = (int)(i / ((int64_t)sp * ny));
+  if (kx >= nxh) {  // row padding
+    out[i] = 0.0f;
+    return;
+  }
   const double g1 = green_at(gf, is, fds, a, kx, ky, kz, nx, ny, nz);
   const double g2 = green_at(gf, is, fds, a, (nx - kx) % nx, (ny - ky) % ny, (nz - kz) % nz, nx,
                              ny, nz);
@@ -147,7 +151,7 @@ void greens_compute(pmgpu_ctx* c, const float* corr_dev) {
   const unsigned g = (unsigned)((c->K + 127) / 128);
   hipLaunchKernelGGL(k_greens, dim3(g), dim3(128), 0, c->stream, c->prm.greens_function,
                      c->prm.interpolation, c->prm.finite_diff, (double)c->prm.particle_diameter,
-                     c->nx, c->ny, c->nz, corr_dev, c->greens);
+                     c->nx, c->ny, c->nz, c->sp, corr_dev, c->greens);
   HIP_CHECK(hipGetLastError());
   stage_end(c, tok);
 }

@@ -111,7 +111,7 @@ __device__ void load_weights(Shared& sh, const Packed* __restrict__ pk, int nmat
 // Input features of one mode.
 struct Inputs {
   const float* list;  // n x 4, or null: modes of the half spectrum
-  int nx, ny, nz, nxh;
+  int nx, ny, nz, pitch;  // pitch: entries per row of the half spectrum (>= nx/2 + 1)
   float order;
 };
 
@@ -127,9 +127,9 @@ __device__ __forceinline__ void mode_inputs(const Inputs& in, int64_t m, int64_t
     x[3] = in.list[4 * m + 3];
     return;
   }
-  const int kx = (int)(m % in.nxh);
-  const int y = (int)((m / in.nxh) % in.ny);
-  const int z = (int)(m / ((int64_t)in.nxh * in.ny));
+  const int kx = (int)(m % in.pitch);  // row padding (kx > nx/2) carries G = 0
+  const int y = (int)((m / in.pitch) % in.ny);
+  const int z = (int)(m / ((int64_t)in.pitch * in.ny));
   x[0] = (float)kx / (float)(in.nx / 2);
   x[1] = (float)min(y, in.ny - y) / (float)(in.ny / 2);
   x[2] = (float)min(z, in.nz - z) / (float)(in.nz / 2);
@@ -441,22 +441,22 @@ void mlp_pack(MlpDevice* d, const float* params_dev, hipStream_t s) {
   HIP_CHECK(hipGetLastError());
 }
 
-static Inputs grid_inputs(int nx, int ny, int nz, int scheme) {
+static Inputs grid_inputs(int nx, int ny, int nz, int pitch, int scheme) {
   Inputs in{};
   in.list = nullptr;
   in.nx = nx;
   in.ny = ny;
   in.nz = nz;
-  in.nxh = nx / 2 + 1;
+  in.pitch = pitch;
   in.order = (float)(scheme - PMGPU_CIC);  // NGP -1, CIC 0, TSC +1
   return in;
 }
 
-void mlp_forward_grid(MlpDevice* d, int nx, int ny, int nz, int scheme, const float* greens, float* out,
-                      hipStream_t s) {
-  const int64_t K = (int64_t)(nx / 2 + 1) * ny * nz;
+void mlp_forward_grid(MlpDevice* d, int nx, int ny, int nz, int pitch, int scheme, const float* greens,
+                      float* out, hipStream_t s) {
+  const int64_t K = (int64_t)pitch * ny * nz;
   hipLaunchKernelGGL(k_mlp_forward, dim3(mlp_blocks(K)), dim3(MLP_THREADS), 0, s, d->packed,
-                     grid_inputs(nx, ny, nz, scheme), K, greens, out);
+                     grid_inputs(nx, ny, nz, pitch, scheme), K, greens, out);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -477,10 +477,10 @@ static void backward(MlpDevice* d, Inputs in, int64_t n, const float* up, float 
   HIP_CHECK(hipGetLastError());
 }
 
-void mlp_backward_grid(MlpDevice* d, int nx, int ny, int nz, int scheme, const float* up, float scale,
+void mlp_backward_grid(MlpDevice* d, int nx, int ny, int nz, int pitch, int scheme, const float* up, float scale,
                        bool accumulate, float* grad, hipStream_t s) {
-  const int64_t K = (int64_t)(nx / 2 + 1) * ny * nz;
-  backward(d, grid_inputs(nx, ny, nz, scheme), K, up, scale, accumulate, grad, s);
+  const int64_t K = (int64_t)pitch * ny * nz;
+  backward(d, grid_inputs(nx, ny, nz, pitch, scheme), K, up, scale, accumulate, grad, s);
 }
 
 // The factor grid for pmgpu_init_greens: greens[k] *= (1 + y(k)).
@@ -492,7 +492,7 @@ void mlp_apply_to_greens(pmgpu_ctx* c) {
     HIP_CHECK(hipMemcpyAsync(p, c->mlp_weights.data(), sizeof(float) * NPARAM, hipMemcpyHostToDevice,
                              c->stream));
     mlp_pack(d, p, c->stream);
-    mlp_forward_grid(d, c->nx, c->ny, c->nz, c->prm.interpolation, c->greens, c->greens, c->stream);
+    mlp_forward_grid(d, c->nx, c->ny, c->nz, c->sp, c->prm.interpolation, c->greens, c->greens, c->stream);
     HIP_CHECK(hipStreamSynchronize(c->stream));
   } catch (...) {
     (void)hipFree(p);

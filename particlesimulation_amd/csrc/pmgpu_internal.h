@@ -83,7 +83,8 @@ struct pmgpu_ctx {
   pmgpu_pm_params prm{};  // prm.interpolation holds the assignment order (NGP, CIC or TSC)
   int tsc_centred = 0;     // PMGPU_TSC_CENTRED requested: TSC about the nearest grid point
   int nx = 0, ny = 0, nz = 0, nxh = 0;  // nxh = nx/2 + 1
-  int64_t M = 0, K = 0;                  // cells, half-spectrum modes
+  int sp = 0;                            // pitch of the half spectrum's rows: nxh rounded up to 16
+  int64_t M = 0, K = 0;                  // cells, half-spectrum entries (sp * ny * nz, pitch included)
   int ntx = 0, nty = 0, ntz = 0, ntiles = 0;
   hipStream_t stream = nullptr;
   std::string err;
@@ -235,11 +236,12 @@ struct MlpDevice;
 MlpDevice* mlp_device_create();
 void mlp_device_destroy(MlpDevice* d);
 void mlp_pack(MlpDevice* d, const float* params_dev, hipStream_t s);
-// out[k] = y(k) on the half spectrum, or greens[k] * (1 + y(k)) when greens is given
-void mlp_forward_grid(MlpDevice* d, int nx, int ny, int nz, int scheme, const float* greens, float* out,
-                      hipStream_t s);
+// out[k] = y(k) on the half spectrum (rows of `pitch` entries), or
+// greens[k] * (1 + y(k)) when greens is given
+void mlp_forward_grid(MlpDevice* d, int nx, int ny, int nz, int pitch, int scheme, const float* greens,
+                      float* out, hipStream_t s);
 // grad (+)= scale * d/dparams sum_k up[k] y(k)
-void mlp_backward_grid(MlpDevice* d, int nx, int ny, int nz, int scheme, const float* up, float scale,
+void mlp_backward_grid(MlpDevice* d, int nx, int ny, int nz, int pitch, int scheme, const float* up, float scale,
                        bool accumulate, float* grad, hipStream_t s);
 void mlp_apply_to_greens(pmgpu_ctx* c);
 void escape_check(pmgpu_ctx* c, int32_t* flag_dev);

@@ -170,11 +170,12 @@ __global__ void k_adj_div(const float* __restrict__ R, int nx, int ny, int nz, f
 }
 
 __global__ void k_grad_acc(const float2* __restrict__ rho_hat, const float2* __restrict__ lam_hat,
-                           const float* __restrict__ greens, int64_t K, int nxh, int nx,
+                           const float* __restrict__ greens, int64_t K, int sp, int nx,
                            float* __restrict__ gradc) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= K) return;
-  const int kx = (int)(k % nxh);
+  const int kx = (int)(k % sp);
+  if (kx > nx / 2) return;  // row padding: never written by the transforms
   const float mult = (kx == 0 || kx == nx / 2) ? 1.0f : 2.0f;
   const float2 a = rho_hat[k], b = lam_hat[k];
   gradc[k] += mult * greens[k] * (a.x * b.x + a.y * b.y);
@@ -529,7 +530,7 @@ void loss_and_gradc(pmgpu_trainer* t, double w) {
   pmgpu_ctx* c = t->ctx;
   hipStream_t s = c->stream;
   const int nx = c->nx, ny = c->ny, nz = c->nz;
-  mlp_forward_grid(t->mlp, nx, ny, nz, t->scheme, c->greens, t->geff, s);
+  mlp_forward_grid(t->mlp, nx, ny, nz, c->sp, t->scheme, c->greens, t->geff, s);
   hipLaunchKernelGGL(k_fill, dim3(blocks(c->K)), dim3(256), 0, s, t->gradc, c->K, 0.0f);
   ensure_loss(t);
   for (size_t k = 0; k < t->snaps.size(); ++k) {
@@ -557,7 +558,7 @@ void loss_and_gradc(pmgpu_trainer* t, double w) {
       hipLaunchKernelGGL(k_adj_clear, dim3(blocks(sn.ncell)), dim3(256), 0, s, sn.cell_id, sn.ncell, c->M, t->R);
     fft_r2c_forward(c, t->lam, c->spec);
     hipLaunchKernelGGL(k_grad_acc, dim3(blocks(c->K)), dim3(256), 0, s, sn.rho_hat, c->spec, c->greens, c->K,
-                       c->nxh, nx, t->gradc);
+                       c->sp, nx, t->gradc);
     hipLaunchKernelGGL(k_loss_sum, dim3(1), dim3(256), 0, s, sn.sq, sn.nt, w, t->loss_parts, (int)k);
     HIP_CHECK(hipGetLastError());
   }
@@ -768,10 +769,16 @@ int pmgpu_trainer_loss_grad(pmgpu_trainer* t, const float* params, double* loss,
   loss_and_gradc(t, 1.0 / norm);
   pmgpu_ctx* c = t->ctx;
   if (grad) {
-    mlp_backward_grid(t->mlp, c->nx, c->ny, c->nz, t->scheme, t->gradc, 1.0f, false, t->grad, c->stream);
+    mlp_backward_grid(t->mlp, c->nx, c->ny, c->nz, c->sp, t->scheme, t->gradc, 1.0f, false, t->grad, c->stream);
     HIP_CHECK(hipMemcpyAsync(grad, t->grad, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, c->stream));
   }
-  if (grad_c) HIP_CHECK(hipMemcpyAsync(grad_c, t->gradc, sizeof(float) * c->K, hipMemcpyDeviceToHost, c->stream));
+  if (grad_c) {  // [z][y][kx <= nx/2], without the row padding
+    std::vector<float> g((size_t)c->K);
+    HIP_CHECK(hipMemcpyAsync(g.data(), t->gradc, sizeof(float) * c->K, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (int64_t r = 0; r < (int64_t)c->ny * c->nz; ++r)
+      std::memcpy(grad_c + r * c->nxh, g.data() + r * c->sp, sizeof(float) * c->nxh);
+  }
   const double L = read_loss(t);
   if (loss) *loss = L;
   TRAINER_END(t)
@@ -804,7 +811,7 @@ int pmgpu_train(pmgpu_trainer** trs, int32_t ntr, float* params, int32_t steps, 
         mlp_pack(t->mlp, t->params, c->stream);
       }
       loss_and_gradc(t, 1.0 / norm);
-      mlp_backward_grid(t->mlp, c->nx, c->ny, c->nz, t->scheme, t->gradc, 1.0f, false, t->grad, c->stream);
+      mlp_backward_grid(t->mlp, c->nx, c->ny, c->nz, c->sp, t->scheme, t->gradc, 1.0f, false, t->grad, c->stream);
       L += read_loss(t);  // synchronises this trainer's stream
       if (i > 0) {
         hipLaunchKernelGGL(k_axpy_add, dim3(blocks(NPARAM)), dim3(256), 0, s0, t0->grad, t->grad, NPARAM);

@@ -21,7 +21,8 @@ import time
 
 import numpy as np
 
-from . import (CIC, DISCRETE_LAPLACIAN, S2, S2_OPTIMAL, TSC, TWO_POINT, Batch, ics, p3m_params, pm_params)
+from . import (CIC, DISCRETE_LAPLACIAN, PM, S2, S2_OPTIMAL, TSC, TWO_POINT, Batch, direct_sum, direct_sum_host, ics,
+               p3m_params, pm_params)
 
 
 def member(i, rng):
@@ -63,6 +64,39 @@ def build(ens, lanes=8, autotune=True):
     return b
 
 
+def member_error(e, n_targets=4096, fp64_check=256, seed=7):
+    """Force error of one member's configuration at its initial positions:
+    the force step of a context set up like the member's (the batch runs
+    the same kernels, bitwise for PM members) against the periodic direct
+    sum on the member's grid box (the FFT force's boundary condition; GPU,
+    FP32 with compensated sums), with the FP64 host sum on the first
+    `fp64_check` targets as a cross-check."""
+    g = e["grid"]
+    prm = e["params"]
+    pm = PM((g, g, g), interpolation=prm.interpolation, finite_diff=prm.finite_diff,
+            greens_function=prm.greens_function, particle_diameter=prm.particle_diameter, box=(0, 0, 0))
+    pm.set_particles(e["pos"], e["vel"], e["mass"])
+    pm.init_greens()
+    if e["p3m"]:
+        q = e["p3m_params"]
+        pm.p3m_configure(cutoff=q.cutoff, particle_diameter=q.particle_diameter, softening=q.softening,
+                         cloud_shape=q.cloud_shape, use_table=q.use_table, box=tuple(q.box))
+    pm.pm_step()
+    if e["p3m"]:
+        pm.p3m_short_range()
+    acc = pm.get_particles()[2].astype(np.float64)
+    pm.close()
+    rng = np.random.default_rng(seed)
+    tgt = np.sort(rng.choice(e["n"], size=min(n_targets, e["n"]), replace=False))
+    ref = direct_sum(e["pos"], e["mass"], e["pos"][tgt], box=(g, g, g)).astype(np.float64)
+    k = tgt[:fp64_check]
+    ref64 = direct_sum_host(e["pos"], e["mass"], e["pos"][k], box=(g, g, g))
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    return {"member": None, "n": e["n"], "grid": g, "kind": e["kind"], "scheme": ["NGP", "CIC", "TSC"][e["scheme"]],
+            "p3m": e["p3m"], "targets": int(len(tgt)), "e_rms": rel(acc[tgt], ref),
+            "fp32_vs_fp64_reference": rel(ref[:len(k)], ref64)}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--members", type=int, default=256)
@@ -70,6 +104,10 @@ def main(argv=None):
     ap.add_argument("--lanes", type=int, default=8)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--no-latency", action="store_true",
+                    help="skip the per-member latency (each member's own graph launched alone)")
+    ap.add_argument("--errors", type=int, default=0, metavar="K",
+                    help="also state the force error of K members (evenly spaced) against the direct sum")
     args = ap.parse_args(argv)
     t0 = time.perf_counter()
     ens = make_ensemble(args.members, args.seed)
@@ -81,19 +119,30 @@ def main(argv=None):
     b.synchronize()
     ms = b.time(args.steps)
     total_n = sum(e["n"] for e in ens)
-    lat = b.member_latency(3)
+    lat = None if args.no_latency else b.member_latency(3)
     out = {
         "metric": "ensemble particle-updates/s", "members": args.members, "steps": args.steps,
         "lanes": args.lanes, "autotune": not args.no_autotune,
         "particles": total_n, "p3m_members": int(sum(e["p3m"] for e in ens)),
         "ms_per_step": ms / args.steps, "value": total_n * args.steps / (ms * 1e-3),
-        "member_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                              "max": float(lat.max())},
-        "sequential_ms_per_step": float(lat.sum()),
-        "batch_speedup_vs_sequential": float(lat.sum() / (ms / args.steps)),
+        "member_latency_ms": None if lat is None else {
+            "p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)), "max": float(lat.max())},
+        "sequential_ms_per_step": None if lat is None else float(lat.sum()),
+        "batch_speedup_vs_sequential": None if lat is None else float(lat.sum() / (ms / args.steps)),
         "setup_s": {"initial_conditions": t_ic, "prepare": t_prep},
         "tuning_classes": len({b.tuning(i) for i in range(len(ens))}),
     }
+    if args.errors:
+        errs = []
+        for i in np.linspace(0, len(ens) - 1, args.errors).round().astype(int):
+            r = member_error(ens[i])
+            r["member"] = int(i)
+            errs.append(r)
+        e_all = np.array([r["e_rms"] for r in errs])
+        out["force_error"] = {"members": errs, "median_e_rms": float(np.median(e_all)),
+                              "max_e_rms": float(e_all.max()),
+                              "reference": "periodic direct sum on each member's grid box (gpu fp32, "
+                                           "checked against the fp64 host sum)"}
     print(json.dumps(out))
 
 
