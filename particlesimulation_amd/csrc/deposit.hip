@@ -151,7 +151,8 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit(
   constexpr int LO = SCHEME == PMGPU_TSC ? -1 : 0;
   constexpr int CNT = SCHEME == PMGPU_TSC ? 3 : (SCHEME == PMGPU_CIC ? 2 : 1);
 
-  for (int w = blockIdx.x; w < total; w += gridDim.x) {
+  const XcdRange wr = xcd_range(total);
+  for (int w = wr.first; w < wr.end; w += wr.stride) {
     if (threadIdx.x == 0) {
       const int t = owner[w];
       s_tile = t;

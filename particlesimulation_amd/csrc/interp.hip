@@ -98,7 +98,8 @@ __global__ __launch_bounds__(256) void k_interp(
   __shared__ float sE[3][Gm::EX * Gm::EY * Gm::EZ];
   __shared__ int s_tile, s_chunk;
   const int total = chunk_start[ntiles];
-  for (int w = blockIdx.x; w < total; w += gridDim.x) {
+  const XcdRange wr = xcd_range(total);
+  for (int w = wr.first; w < wr.end; w += wr.stride) {
     if (threadIdx.x == 0) {
       const int t = owner[w];
       s_tile = t;

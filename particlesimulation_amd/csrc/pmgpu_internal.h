@@ -265,6 +265,22 @@ __device__ __forceinline__ int next_item(int32_t* cnt, int total, int xcd) {
   return -1;
 }
 
+// Static XCD-aware assignment for a grid-stride loop over work items
+// [0, total) ordered so that neighbouring items share data: workgroup b
+// works through the segment of its XCD (b % NXCD), striding over that XCD's
+// workgroups, so neighbouring items meet in one L2.  With fewer workgroups
+// than XCDs, the plain grid stride.
+struct XcdRange {
+  int first, end, stride;
+};
+__device__ __forceinline__ XcdRange xcd_range(int total) {
+  if (gridDim.x < (unsigned)NXCD) return {(int)blockIdx.x, total, (int)gridDim.x};
+  const int x = (int)(blockIdx.x % NXCD);
+  const int b = (int)((int64_t)total * x / NXCD), e = (int)((int64_t)total * (x + 1) / NXCD);
+  const int nb = ((int)gridDim.x - x + NXCD - 1) / NXCD;  // workgroups on XCD x
+  return {b + (int)(blockIdx.x / NXCD), e, nb};
+}
+
 // Rank of this lane's key among all lanes of the grid that share it: one
 // integer atomic per distinct key per wavefront (lanes with equal keys are
 // ranked by lane order), so densely populated keys do not serialise 64
