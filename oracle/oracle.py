@@ -126,7 +126,7 @@ def fft3d(a, inverse=False):
     im = np.ascontiguousarray(a.imag).ravel().copy()
     rc = lib().orc_fft3d(nx, ny, nz, re, im, +1 if inverse else -1)
     if rc != 0:
-        raise ValueError("orc_fft3d: lengths must be powers of two")
+        raise ValueError("orc_fft3d: lengths must be positive")
     return (re + 1j * im).reshape(nz, ny, nx)
 
 

@@ -162,7 +162,7 @@ int orc_deposit(int64_t n, const float* pos, const float* mass, int nx, int ny, 
 }
 
 /* ------------------------------------------------------------------------ */
-/* FFT (complex, double, power-of-two lengths)                               */
+/* FFT (complex, double; radix 2 for powers of two, a direct DFT otherwise) */
 /* ------------------------------------------------------------------------ */
 
 /* Iterative radix-2 DIT on one line; sign = -1 forward, +1 backward
@@ -197,13 +197,37 @@ static void fft_line(double* re, double* im, int n, int sign) {
 
 static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
+/* Direct DFT of one line of any length (FFTW's definition, the same sign
+   convention): X[k] = sum_j x[j] exp(sign 2 pi i j k / n), with (j k) mod n
+   indexing an FP64 twiddle table.  tmp holds 2n doubles. */
+static void dft_line(double* re, double* im, int n, int sign, double* tmp) {
+  double* tr = tmp;
+  double* ti = tmp + n;
+  for (int k = 0; k < n; ++k) {
+    double sr = 0.0, si = 0.0;
+    int64_t idx = 0;
+    for (int j = 0; j < n; ++j) {
+      double ang = sign * 2.0 * kPi * (double)idx / (double)n;
+      double wr = cos(ang), wi = sin(ang);
+      sr += re[j] * wr - im[j] * wi;
+      si += re[j] * wi + im[j] * wr;
+      idx += k;
+      if (idx >= n) idx -= n;
+    }
+    tr[k] = sr;
+    ti[k] = si;
+  }
+  memcpy(re, tr, sizeof(double) * (size_t)n);
+  memcpy(im, ti, sizeof(double) * (size_t)n);
+}
+
 /*
  * Unnormalised 3-D complex DFT in place on separate re/im arrays laid out
  * x-fastest (the FFTW plan {Nz,Ny,Nx}, source/FFTWAdapter.cpp:15 with dims from
  * source/demos.cpp:126-127).  sign = -1 forward, +1 backward.
  */
 int orc_fft3d(int nx, int ny, int nz, double* re, double* im, int sign) {
-  if (!is_pow2(nx) || !is_pow2(ny) || !is_pow2(nz)) return -2;
+  if (nx <= 0 || ny <= 0 || nz <= 0) return -2;
   int dims[3] = {nx, ny, nz};
   int64_t strides[3] = {1, nx, (int64_t)nx * ny};
   for (int ax = 0; ax < 3; ++ax) {
@@ -217,6 +241,7 @@ int orc_fft3d(int nx, int ny, int nz, double* re, double* im, int sign) {
     {
       double* lr = (double*)malloc(sizeof(double) * (size_t)len);
       double* li = (double*)malloc(sizeof(double) * (size_t)len);
+      double* tmp = (double*)malloc(sizeof(double) * 2 * (size_t)len);
 #pragma omp for collapse(2) schedule(static)
       for (int b = 0; b < o2; ++b) {
         for (int a = 0; a < o1; ++a) {
@@ -225,7 +250,8 @@ int orc_fft3d(int nx, int ny, int nz, double* re, double* im, int sign) {
             lr[i] = re[base + i * st];
             li[i] = im[base + i * st];
           }
-          fft_line(lr, li, len, sign);
+          if (is_pow2(len)) fft_line(lr, li, len, sign);
+          else dft_line(lr, li, len, sign, tmp);
           for (int i = 0; i < len; ++i) {
             re[base + i * st] = lr[i];
             im[base + i * st] = li[i];
@@ -234,6 +260,7 @@ int orc_fft3d(int nx, int ny, int nz, double* re, double* im, int sign) {
       }
       free(lr);
       free(li);
+      free(tmp);
     }
   }
   return 0;

@@ -22,7 +22,7 @@ CSRC = os.path.join(HERE, "csrc")
 HOST = os.path.join(HERE, "host")
 BUILD = os.path.join(HERE, "_build")
 HIP_SOURCES = ["api.hip", "sort.hip", "deposit.hip", "fft.hip", "greens.hip", "interp.hip", "p3m.hip",
-               "mlp.hip", "diag.hip", "train.hip", "batch.hip"]
+               "mlp.hip", "diag.hip", "train.hip", "batch.hip", "dft.hip"]
 ARCH = os.environ.get("PMGPU_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 (v_pk_*_f32) is disabled for all device code.

@@ -83,13 +83,12 @@ static void profile_collect(pmgpu_ctx* c) {
   c->pending.clear();
 }
 
-static bool is_pow2_in(int v, int lo, int hi) { return v >= lo && v <= hi && (v & (v - 1)) == 0; }
 
 static void validate(const pmgpu_pm_params* p) {
   if (!p) fail(PMGPU_ERR_INVALID_ARGUMENT, "params is NULL");
   for (int i = 0; i < 3; ++i)
-    if (!is_pow2_in(p->grid[i], 32, 1024))
-      fail(PMGPU_ERR_UNSUPPORTED, "grid dimensions must be powers of two in [32, 1024]");
+    if (p->grid[i] < 8 || p->grid[i] > 1024)
+      fail(PMGPU_ERR_UNSUPPORTED, "grid dimensions must be in [8, 1024]");
   if (p->interpolation < 0 || p->interpolation > PMGPU_TSC_CENTRED)
     fail(PMGPU_ERR_INVALID_ARGUMENT, "Unkown interpolation scheme");
   if (p->finite_diff < 0 || p->finite_diff > 1)
@@ -335,9 +334,11 @@ int pmgpu_create(const pmgpu_pm_params* params, pmgpu_ctx** out) {
     // passes' column tiles (16 or 32 complex values) are whole cache lines
     c->sp = (c->nxh + 15) / 16 * 16;
     c->K = (int64_t)c->sp * c->ny * c->nz;
-    c->ntx = c->nx / TILE_X;
-    c->nty = c->ny / TILE_Y;
-    c->ntz = c->nz / TILE_Z;
+    c->general = general_grid(c->nx, c->ny, c->nz);
+    // the tiled deposit and interpolation need sides that the tiles divide
+    c->ntx = c->general ? 0 : c->nx / TILE_X;
+    c->nty = c->general ? 0 : c->ny / TILE_Y;
+    c->ntz = c->general ? 0 : c->nz / TILE_Z;
     c->ntiles = c->ntx * c->nty * c->ntz;
     int dev = 0;
     HIP_CHECK(hipGetDevice(&dev));
@@ -351,6 +352,10 @@ int pmgpu_create(const pmgpu_pm_params* params, pmgpu_ctx** out) {
     twiddles(&c->tw_y, c->ny, c->ny);
     twiddles(&c->tw_z, c->nz, c->nz);
     twiddles(&c->tw_r2c, c->nx / 2 + 1, c->nx);
+    if (c->general) {
+      twiddles(&c->tw_gx, c->nx, c->nx);
+      dalloc(&c->fx_grid, c->M);
+    }
     const int64_t nkeys = (int64_t)c->ntiles * NCLASS;
     dalloc(&c->cls_count, nkeys + 1);
     dalloc(&c->cls_start, nkeys + 1);
@@ -394,7 +399,7 @@ void pmgpu_destroy(pmgpu_ctx* c) {
   dfree(&c->dep_table); dfree(&c->counters); dfree(&c->fx_accum); dfree(&c->scan_tmp);
   dfree(&c->dep_owner); dfree(&c->int_owner); dfree(&c->sr_owner);
   dfree(&c->rho); dfree(&c->spec); dfree(&c->phi); dfree(&c->greens); dfree(&c->field); dfree(&c->efield);
-  dfree(&c->tw_x); dfree(&c->tw_y); dfree(&c->tw_z); dfree(&c->tw_r2c);
+  dfree(&c->tw_x); dfree(&c->tw_y); dfree(&c->tw_z); dfree(&c->tw_r2c); dfree(&c->tw_gx); dfree(&c->fx_grid);
   dfree(&c->sr_table); dfree(&c->sr_count); dfree(&c->sr_start); dfree(&c->sr_key);
   dfree(&c->sr_rank); dfree(&c->sr_pm); dfree(&c->sr_idx); dfree(&c->sr_nchunk);
   dfree(&c->sr_odd_ids); dfree(&c->sr_odd_pos);

@@ -324,6 +324,7 @@ std::vector<int4> deposit_range_table(int scheme) {
 }
 
 void deposit(pmgpu_ctx* c) {
+  if (c->general) return deposit_general(c);
   const int nranges = 1 + c->dep_table_len;
   int tok = stage_begin(c, "deposit_plan");
   zero_fill_i32(c, c->counters, 4);

@@ -438,6 +438,7 @@ void x_c2r(pmgpu_ctx* c, const float2* spec, float* out) {
 }  // namespace
 
 void fft_forward_green_inverse(pmgpu_ctx* c) {
+  if (c->general) return fft_general(c);
   int tok = stage_begin(c, "fft_x_r2c");
   x_r2c(c, c->rho, c->spec);
   stage_end(c, tok);

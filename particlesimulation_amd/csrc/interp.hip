@@ -525,7 +525,7 @@ void interpolate(pmgpu_ctx* c, int mode, float kick, int iteration) {
   }
   // auto: the gather variant when tiles are sparse (the tiled variant's
   // per-tile staging of the potential and field then dominates)
-  const bool gather = c->interp_kernel == 2 ||
+  const bool gather = c->general || c->interp_kernel == 2 ||
                       (c->interp_kernel == 0 && c->ntiles < INTERP_GATHER_TILES_PER_CU * c->num_cu);
   if (gather) {
     interpolate_gather(c, mode, kick, iteration);

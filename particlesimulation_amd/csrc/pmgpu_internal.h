@@ -84,6 +84,7 @@ struct pmgpu_ctx {
   int tsc_centred = 0;     // PMGPU_TSC_CENTRED requested: TSC about the nearest grid point
   int nx = 0, ny = 0, nz = 0, nxh = 0;  // nxh = nx/2 + 1
   int sp = 0;                            // pitch of the half spectrum's rows: nxh rounded up to 16
+  bool general = false;                  // a side is not a power of two in [32, 1024]: the untiled path
   int64_t M = 0, K = 0;                  // cells, half-spectrum entries (sp * ny * nz, pitch included)
   int ntx = 0, nty = 0, ntz = 0, ntiles = 0;
   hipStream_t stream = nullptr;
@@ -156,6 +157,8 @@ struct pmgpu_ctx {
   float2* tw_y = nullptr;
   float2* tw_z = nullptr;
   float2* tw_r2c = nullptr;  // exp(-2 pi i k / nx), k in [0, nx/2]
+  float2* tw_gx = nullptr;   // general grids: exp(-2 pi i m / nx), m in [0, nx)
+  unsigned long long* fx_grid = nullptr;  // general grids: fixed-point density, M
   bool greens_ready = false;
   std::vector<float> mlp_weights;  // learned correction (empty: analytic)
 
@@ -214,6 +217,10 @@ void tile_sort(pmgpu_ctx* c);
 size_t scan_scratch_elems(int64_t n);
 std::vector<int4> deposit_range_table(int scheme);
 void deposit(pmgpu_ctx* c);
+// grids that are not all powers of two in [32, 1024] (csrc/dft.hip)
+bool general_grid(int nx, int ny, int nz);
+void deposit_general(pmgpu_ctx* c);
+void fft_general(pmgpu_ctx* c);
 void fft_forward_green_inverse(pmgpu_ctx* c);
 // unnormalised forward R2C of a real grid into a half spectrum (in may be rho)
 void fft_r2c_forward(pmgpu_ctx* c, const float* in, float2* spec);

@@ -245,6 +245,7 @@ void expand_owner(pmgpu_ctx* c, const int32_t* starts, int64_t n, int32_t* owner
 }
 
 void tile_sort(pmgpu_ctx* c) {
+  if (c->general) return;  // the untiled path keeps the particles in place
   const int64_t nkeys = (int64_t)c->ntiles * NCLASS;
   int tok = stage_begin(c, "tile_sort");
   zero_fill_i32(c, c->cls_count, nkeys);

@@ -618,6 +618,10 @@ int pmgpu_trainer_create(const pmgpu_pm_params* params, pmgpu_trainer** out) {
   pmgpu_ctx* c = nullptr;
   int rc = pmgpu_create(params, &c);
   if (rc != PMGPU_OK) return rc;
+  if (c->general) {
+    pmgpu_destroy(c);
+    return PMGPU_ERR_UNSUPPORTED;  // the trainer's transforms use the power-of-two FFT passes
+  }
   pmgpu_trainer* t = new pmgpu_trainer();
   t->ctx = c;
   try {

@@ -51,7 +51,9 @@ def test_create_rejects_invalid_enums(gpu_lib, field, value, code):
 def test_create_rejects_unsupported_grid(gpu_lib):
     P = gpu_lib
     L = P.load_library()
-    for grid in ([48, 32, 32], [16, 32, 32], [32, 32, 2048]):
+    # any side in [8, 1024] is taken (powers of two from 32 on the tiled FFT
+    # path, the rest on the direct-DFT path); outside that range is refused
+    for grid in ([7, 32, 32], [32, 4, 32], [32, 32, 2048], [1025, 64, 64]):
         p = P.PMParams()
         p.grid[:] = grid
         ctx = C.c_void_p()

@@ -157,6 +157,51 @@ def test_centred_tsc_interpolation_variants_match_oracle(gpu_lib, oracle, varian
     assert _rel(acc, oracle.pm_accel(pos, m, grid, 3, 1, G)) <= FORCE_TOL
 
 
+@pytest.mark.parametrize("grid,scheme,fds,gf", [((60, 60, 30), 1, 0, 0), ((68, 68, 34), 2, 1, 2), ((36, 44, 52), 0, 0, 3),
+                                               ((45, 33, 21), 2, 0, 1), ((48, 64, 32), 3, 1, 2), ((16, 16, 16), 2, 0, 0)])
+def test_general_grid_matches_oracle(gpu_lib, oracle, grid, scheme, fds, gf):
+    # sides that are not powers of two in [32, 1024] (the reference's accuracy
+    # demo sweeps 2Ng x 2Ng x Ng, Ng = 30 .. 70, source/demos.cpp:381-408)
+    # take the untiled path: global fixed-point deposit, direct DFTs, gather
+    pos, vel, m = _particles(3000, grid, 70 + sum(grid), clustered=True, margin=3.0)
+    pm = _make(gpu_lib, grid, scheme, fds, gf, a=3.0)
+    pm.set_particles(pos, vel, m)
+    pm.init_greens()
+    pm.pm_step()
+    acc = pm.get_particles()[2]
+    G = oracle.green_grid(gf, scheme, fds, 3.0, grid)
+    ref, rho, phi, _ = oracle.pm_accel(pos, m, grid, scheme, fds, G, wrap=False, want_grids=True)
+    assert np.max(np.abs(pm.density() - rho)) / np.max(np.abs(rho)) <= DENSITY_TOL
+    assert _rel(pm.potential(), phi) <= FORCE_TOL
+    assert _rel(acc, ref) <= FORCE_TOL
+    # the density is bitwise reproducible and the run loop follows the oracle
+    rho0 = pm.density()
+    pm.pm_step()
+    assert np.array_equal(pm.density(), rho0)
+    pm.set_particles(pos, vel, m)
+    assert pm.run(4, initial=True) == 4
+    x = pm.get_particles()[0]
+    xr, _, _ = _oracle_run(oracle, pos, vel, m, grid, scheme, fds, G, 4)
+    assert _rel(x, xr) <= 1e-5
+
+
+def test_general_grid_p3m_matches_oracle(gpu_lib, oracle):
+    grid = (60, 60, 30)
+    H = 0.25
+    pos, vel, m = _particles(4000, grid, 77, clustered=True, margin=4.0)
+    pm = _make(gpu_lib, grid, 2, 0, 2, a=3.0)
+    pm.set_particles(pos, vel, m)
+    pm.init_greens()
+    pm.p3m_configure(cutoff=3.0, particle_diameter=3.0, cloud_shape=1, use_table=1)
+    pm.pm_step()
+    pm.p3m_short_range()
+    acc = pm.get_particles()[2]
+    G = oracle.green_grid(2, 2, 0, 3.0, grid)
+    ref_sr, _ = oracle.p3m_short_range(pos, m, np.array(grid, np.float32) * H, 3 * H, H, 3.0, 3.0, 0.0, 1,
+                                       use_table=1)
+    assert _rel(acc, oracle.pm_accel(pos, m, grid, 2, 0, G) + ref_sr) <= FORCE_TOL
+
+
 def test_external_field_matches_oracle(gpu_lib, oracle):
     grid = (32, 32, 32)
     H, DT = 0.5, 0.1

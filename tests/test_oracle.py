@@ -143,13 +143,27 @@ def test_fft_matches_kissfft_fixture(oracle):
 
 
 @pytest.mark.skipif("not __import__('oracle.oracle', fromlist=['x']).kissfft_available()")
-def test_fft_matches_live_kissfft(oracle):
-    a = (_rng(3).standard_normal((16, 8, 32)) + 1j * _rng(4).standard_normal((16, 8, 32))).astype(np.complex64)
+@pytest.mark.parametrize("shape", [(16, 8, 32), (30, 60, 60), (15, 34, 68)])
+def test_fft_matches_live_kissfft(oracle, shape):
+    # the reference's own mixed-radix FFT backend, built from its sources
+    a = (_rng(3).standard_normal(shape) + 1j * _rng(4).standard_normal(shape)).astype(np.complex64)
     k = oracle.kiss_fftnd(a)
     f = oracle.fft3d(a)
     assert np.max(np.abs(f - k)) < 1e-5 * np.max(np.abs(k))
     kb = oracle.kiss_fftnd(k, inverse=True)
     assert np.max(np.abs(kb - a)) < 1e-5
+
+
+@pytest.mark.parametrize("shape", [(6, 10, 15), (30, 60, 60), (7, 16, 9), (34, 68, 68)])
+def test_fft_any_length_matches_numpy(oracle, shape):
+    # lengths that are not powers of two take the direct DFT (the reference's
+    # pmAccuracy demo sweeps 2Ng x 2Ng x Ng grids, Ng = 30 .. 70, through FFTW,
+    # source/demos.cpp:381-408); numpy's pocketfft is an independent FFT
+    a = _rng(8).standard_normal(shape) + 1j * _rng(9).standard_normal(shape)
+    f = oracle.fft3d(a)
+    assert np.max(np.abs(f - np.fft.fftn(a))) <= 1e-10 * np.abs(f).max()
+    b = oracle.fft3d(a, inverse=True)
+    assert np.max(np.abs(b - np.fft.ifftn(a) * a.size)) <= 1e-10 * np.abs(b).max()
 
 
 def test_fft_matches_numpy(oracle):
