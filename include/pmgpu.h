@@ -164,6 +164,16 @@ int pmgpu_p3m_configure(pmgpu_ctx* ctx, const pmgpu_p3m_params* params);
    (source/p3mMethod.cpp:168-192, 50-57): adds the short-range correction
    to the acceleration of every particle. */
 int pmgpu_p3m_short_range(pmgpu_ctx* ctx);
+/* Record (on != 0) or stop recording the short-range part of each
+   particle's acceleration in every short-range pass, and read the last
+   pass's (code units, caller order, n x 3; zero for particles the pass did
+   not reach).  The reference keeps it as shortRangeForce plus the
+   shortRangeFromNeighbor slots of each Particle (include/particle.h:11-12,
+   filled at source/p3mMethod.cpp:269-272, summed in correctAccelerations,
+   :50-57); the recorded value is that sum divided by the mass.  Every pass
+   writes every particle's value. */
+int pmgpu_p3m_record_short_range(pmgpu_ctx* ctx, int32_t on);
+int pmgpu_p3m_get_short_range(pmgpu_ctx* ctx, float* acc);
 
 /* ---- leapfrog (source/leapfrog.cpp:5-24, dt in code units) ----------- */
 #define PMGPU_HALF_KICK 0     /* setHalfStepVelocities: v += dt/2 a      */

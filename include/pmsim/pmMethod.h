@@ -58,6 +58,9 @@ class PMMethod {
   void uploadIfModified();                // push host-side edits of the mirror
   void downloadMirror();                  // refresh the mirror from the device
   void markDeviceNewer() { mirrorStale = true; }
+  // P3MMethod asks for the short-range part of each force, which the mirror
+  // then shows as Particle::shortRangeForce
+  void recordShortRange();
   Vec3 externalFieldAt(Vec3 posOriginal) const { return externalField(posOriginal); }
 
  private:
@@ -83,4 +86,5 @@ class PMMethod {
   uint64_t mirrorSeen = 0;    // hash of the mirror when it was last synchronised
   bool greensReady = false;
   bool hostField = false;     // the external field is a host callable (not SphRadDecr / ZeroField)
+  bool srRecorded = false;    // the device records the short-range accelerations
 };

@@ -53,7 +53,8 @@ EXPORTS = (
     "pmgpu_greens_value", "pmgpu_diagnostics", "pmgpu_mlp_forward", "pmgpu_mlp_backward",
     "pmgpu_trainer_create", "pmgpu_trainer_destroy", "pmgpu_trainer_last_error", "pmgpu_trainer_add_snapshot",
     "pmgpu_trainer_loss_grad", "pmgpu_train", "pmgpu_direct_sum", "pmgpu_direct_sum_host", "pmgpu_direct_sum_noalloc", "pmgpu_set_tuning", "pmgpu_get_tuning", "pmgpu_p3m_pair_stats",
-    "pmgpu_p3m_cell_geometry", "pmgpu_add_accelerations",
+    "pmgpu_p3m_cell_geometry", "pmgpu_add_accelerations", "pmgpu_p3m_record_short_range",
+    "pmgpu_p3m_get_short_range",
     "pmgpu_batch_create", "pmgpu_batch_destroy", "pmgpu_batch_last_error", "pmgpu_batch_size",
     "pmgpu_batch_set_particles", "pmgpu_batch_prepare", "pmgpu_batch_step", "pmgpu_batch_synchronize",
     "pmgpu_batch_time", "pmgpu_batch_member_latency", "pmgpu_batch_get_particles", "pmgpu_batch_get_tuning",
@@ -118,6 +119,8 @@ def load_library(path: str = LIB_PATH):
     L.pmgpu_set_particles.argtypes = [vp, C.c_int64, fp, fp, fp]
     L.pmgpu_get_particles.argtypes = [vp, fp, fp, fp]
     L.pmgpu_add_accelerations.argtypes = [vp, fp]
+    L.pmgpu_p3m_record_short_range.argtypes = [vp, C.c_int32]
+    L.pmgpu_p3m_get_short_range.argtypes = [vp, fp]
     L.pmgpu_update_particles.argtypes = [vp, fp, fp, fp]
     L.pmgpu_num_particles.argtypes = [vp]
     L.pmgpu_num_particles.restype = C.c_int64
@@ -289,6 +292,15 @@ class PM:
 
     def p3m_short_range(self):
         self._check(self._lib.pmgpu_p3m_short_range(self._ctx))
+
+    def p3m_record_short_range(self, on=True):
+        self._check(self._lib.pmgpu_p3m_record_short_range(self._ctx, int(bool(on))))
+
+    def p3m_short_range_accelerations(self):
+        """The short-range part of the last short-range pass's accelerations (n, 3)."""
+        a = np.zeros((self.n, 3), np.float32)
+        self._check(self._lib.pmgpu_p3m_get_short_range(self._ctx, _fptr(a)))
+        return a
 
     def p3m_count_pairs(self, tested=False):
         v = C.c_int64(0)

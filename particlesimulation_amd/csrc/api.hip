@@ -132,6 +132,7 @@ static void alloc_particle_arrays(pmgpu_ctx* c, int64_t n) {
   dalloc(&c->rank, n);
   dfree(&c->vint);
   dfree(&c->ext_stage);
+  if (c->acc_sr) dalloc(&c->acc_sr, n);  // recording stays on for a new particle set
   c->fx_slots = 8 * n / DEPOSIT_CHUNK_MIN + 2;  // enough for any tunable chunk size
   dalloc(&c->fx_accum, c->fx_slots * TILE_CELLS);
   dalloc(&c->slot_tile, c->fx_slots);
@@ -393,7 +394,7 @@ void pmgpu_destroy(pmgpu_ctx* c) {
     dfree(&c->pm[b]);
     dfree(&c->vel[b]);
   }
-  dfree(&c->acc); dfree(&c->vint); dfree(&c->ext_stage); dfree(&c->key); dfree(&c->rank);
+  dfree(&c->acc); dfree(&c->vint); dfree(&c->ext_stage); dfree(&c->acc_sr); dfree(&c->key); dfree(&c->rank);
   dfree(&c->cls_count); dfree(&c->cls_start); dfree(&c->tile_nchunk); dfree(&c->tile_chunk_start);
   dfree(&c->tile_slot); dfree(&c->slot_tile); dfree(&c->int_nchunk); dfree(&c->int_chunk_start);
   dfree(&c->dep_table); dfree(&c->counters); dfree(&c->fx_accum); dfree(&c->scan_tmp);
@@ -632,6 +633,34 @@ int pmgpu_leapfrog(pmgpu_ctx* c, int32_t op, float dt) {
   if (op == PMGPU_INTEGER_STEP && !c->vint) dalloc(&c->vint, c->n);
   leapfrog_op(c, op, dt, INT_MIN);
   if (op == PMGPU_DRIFT) c->sorted_valid = false;
+  PMGPU_API_END(c)
+}
+
+int pmgpu_p3m_record_short_range(pmgpu_ctx* c, int32_t on) {
+  PMGPU_API_BEGIN(c)
+  if (on && !c->acc_sr) dalloc(&c->acc_sr, std::max<int64_t>(c->n, 1));
+  if (!on) dfree(&c->acc_sr);
+  PMGPU_API_END(c)
+}
+
+int pmgpu_p3m_get_short_range(pmgpu_ctx* c, float* acc) {
+  PMGPU_API_BEGIN(c)
+  if (!acc) fail(PMGPU_ERR_INVALID_ARGUMENT, "acc is NULL");
+  if (!c->acc_sr) fail(PMGPU_ERR_NOT_READY, "call pmgpu_p3m_record_short_range first");
+  const int64_t n = c->n;
+  if (n == 0) return PMGPU_OK;
+  std::vector<float4> hv(n), ha(n);
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  HIP_CHECK(hipMemcpy(hv.data(), c->vel[c->cur], sizeof(float4) * n, hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(ha.data(), c->acc_sr, sizeof(float4) * n, hipMemcpyDeviceToHost));
+  for (int64_t s = 0; s < n; ++s) {
+    int32_t id;
+    std::memcpy(&id, &hv[s].w, sizeof(id));
+    if (id < 0 || id >= n) fail(PMGPU_ERR_DEVICE, "corrupted particle index");
+    acc[3 * id] = ha[s].x;
+    acc[3 * id + 1] = ha[s].y;
+    acc[3 * id + 2] = ha[s].z;
+  }
   PMGPU_API_END(c)
 }
 

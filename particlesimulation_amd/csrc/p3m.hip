@@ -221,7 +221,7 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
     const float2* __restrict__ table_g, SrParams P, float4* __restrict__ pm,
     float4* __restrict__ vel, float4* __restrict__ acc, int mode, float kick, float3 box,
     float m0, int nodd, const int32_t* __restrict__ odd_pos,
-    int32_t* __restrict__ counters, int32_t* __restrict__ escape, int iteration) {
+    int32_t* __restrict__ counters, float4* __restrict__ acc_sr, int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
   // The partial sums of half of the waves at a time.
   constexpr int HALF = SR_WAVES / 2;
@@ -377,6 +377,7 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
         }
       }
       const int idx = sr_idx[i];
+      if (acc_sr) acc_sr[idx] = make_float4(s.x, s.y, s.z, 0.0f);  // recorded on request
       float4 a = acc[idx];
       a.x += s.x;
       a.y += s.y;
@@ -690,7 +691,7 @@ static void launch_sr_mass(pmgpu_ctx* c, int mode, float kick, int iteration) {
                      c->sr_start, c->sr_chunk_start, c->sr_owner, sr_units(c), c->sr_idx, c->sr_table,
                      sr_params(c), c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick,
                      make_float3(c->prm.box[0], c->prm.box[1], c->prm.box[2]), c->sr_m0,
-                     std::max(c->sr_nodd, 0), c->sr_odd_pos, c->counters, c->escape, iteration);
+                     std::max(c->sr_nodd, 0), c->sr_odd_pos, c->counters, c->acc_sr, c->escape, iteration);
   HIP_CHECK(hipGetLastError());
 }
 

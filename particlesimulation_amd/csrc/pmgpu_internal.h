@@ -97,6 +97,7 @@ struct pmgpu_ctx {
   float4* acc = nullptr;                // ax, ay, az, -
   float4* vint = nullptr;               // integer-step velocities (lazy)
   float* ext_stage = nullptr;           // host-evaluated accelerations, caller order, 3 n (lazy)
+  float4* acc_sr = nullptr;             // the short-range part of acc, when recorded (sorted order)
   int cur = 0;                          // which buffer holds the state
   int32_t* key = nullptr;
   int32_t* rank = nullptr;

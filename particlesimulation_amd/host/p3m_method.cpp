@@ -36,6 +36,7 @@ void P3MMethod::configure() {
   const int rc = pmgpu_p3m_configure(pmMethod.context(), &q);
   if (rc == PMGPU_ERR_INVALID_ARGUMENT) throw std::invalid_argument(pmgpu_last_error(pmMethod.context()));
   if (rc != PMGPU_OK) throw std::runtime_error(pmgpu_last_error(pmMethod.context()));
+  pmMethod.recordShortRange();
 }
 
 void P3MMethod::setShortRangeMode(int m) {

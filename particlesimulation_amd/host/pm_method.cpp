@@ -112,9 +112,18 @@ void PMMethod::uploadIfModified() {
   if (!mirrorStale && mirrorHash() != mirrorSeen) upload();
 }
 
+void PMMethod::recordShortRange() {
+  check(pmgpu_p3m_record_short_range(ctx, 1), ctx, "pmgpu_p3m_record_short_range");
+  srRecorded = true;
+}
+
 void PMMethod::downloadMirror() {
-  std::vector<float> pos(3 * (size_t)N), vel(3 * (size_t)N), acc(3 * (size_t)N);
+  std::vector<float> pos(3 * (size_t)N), vel(3 * (size_t)N), acc(3 * (size_t)N), sr;
   check(pmgpu_get_particles(ctx, pos.data(), vel.data(), acc.data()), ctx, "pmgpu_get_particles");
+  if (srRecorded) {
+    sr.resize(3 * (size_t)N);
+    check(pmgpu_p3m_get_short_range(ctx, sr.data()), ctx, "pmgpu_p3m_get_short_range");
+  }
   for (int i = 0; i < N; ++i) {
     Particle& q = particles[i];
     Vec3 x(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
@@ -123,6 +132,14 @@ void PMMethod::downloadMirror() {
     q.position = inCodeUnits ? x : positionToOriginalUnits(x, H);
     q.velocity = inCodeUnits ? v : velocityToOriginalUnits(v, H, DT);
     q.acceleration = inCodeUnits ? a : accelerationToOriginalUnits(a, H, DT);
+    if (srRecorded) {
+      // the reference's shortRangeForce + shortRangeFromNeighbor hold the
+      // short-range force in code units (source/p3mMethod.cpp:269-272); the
+      // GPU pass has no per-neighbour slots, so their sum is kept here
+      const float mc = inCodeUnits ? q.mass : massToCodeUnits(q.mass, H, DT, G);
+      q.shortRangeForce = mc * Vec3(sr[3 * i], sr[3 * i + 1], sr[3 * i + 2]);
+      q.shortRangeFromNeighbor.fill(Vec3::zero());
+    }
   }
   mirrorStale = false;
   mirrorSeen = mirrorHash();

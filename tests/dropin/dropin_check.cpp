@@ -6,8 +6,10 @@
 //   int32 n, steps, p3m, grid; float32 H, DT, G, diameter, cutoff;
 //   float32 pos[3n], vel[3n], mass[n]            (original units)
 // Outputs in <outdir>: the StateRecorder files and final.bin with the
-// particles' positions, velocities and accelerations (float32 [3n] each) as
-// getParticles() returns them after run().
+// particles' positions, velocities, accelerations and total short-range
+// forces (shortRangeForce plus the shortRangeFromNeighbor slots, as
+// correctAccelerations sums them) (float32 [3n] each) as getParticles()
+// returns them after run().
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -83,9 +85,13 @@ int main(int argc, char** argv) {
     }
     const auto& ps = pm.getParticles();
     std::ofstream out(std::string(argv[2]) + "/final.bin", std::ios::binary);
-    for (int which = 0; which < 3; ++which)
+    for (int which = 0; which < 4; ++which)
       for (const auto& p : ps) {
-        const Vec3& v = which == 0 ? p.position : which == 1 ? p.velocity : p.acceleration;
+        Vec3 v = which == 0 ? p.position : which == 1 ? p.velocity : p.acceleration;
+        if (which == 3) {
+          v = p.shortRangeForce;
+          for (const Vec3& s : p.shortRangeFromNeighbor) v += s;
+        }
         out.write(reinterpret_cast<const char*>(&v), sizeof(Vec3));
       }
     std::printf("\ndropin_check: %d particles, %d steps, p3m=%d, code units after run: %d\n", n, steps, p3m,

@@ -111,7 +111,19 @@ def test_dropin_run_matches_abi_loop(gpu_lib, oracle, tmp_path, p3m):
     assert rec.shape == (steps + 1, n, 3)
     for t in range(steps + 1):
         assert np.max(np.abs(rec[t] - frames[t])) <= 2e-6 * box, t
-    fin = np.fromfile(out / "final.bin", np.float32).reshape(3, n, 3)
+    fin = np.fromfile(out / "final.bin", np.float32).reshape(4, n, 3)
+    if p3m:
+        # the Particle records carry the last pass's short-range force (code
+        # units): m times the short-range part of the acceleration
+        pm.p3m_record_short_range(True)
+        pm.pm_step()
+        a_pm = pm.get_particles()[2].astype(np.float64)
+        pm.p3m_short_range()
+        a_sr = pm.p3m_short_range_accelerations().astype(np.float64)
+        assert np.linalg.norm(pm.get_particles()[2] - a_pm - a_sr) <= 1e-5 * np.linalg.norm(a_sr)
+        assert np.linalg.norm(fin[3] - mc[:, None] * a_sr) <= 1e-4 * np.linalg.norm(mc[:, None] * a_sr)
+    else:
+        assert not fin[3].any()
     # after run() the mirror is in code units, as the reference's particles are
     assert np.max(np.abs(fin[0] - x_end)) <= 1e-5 * g
     assert np.max(np.abs(fin[1] - v_end)) <= 1e-6 + 1e-4 * np.abs(v_end).max()
@@ -221,7 +233,7 @@ def test_galaxy_timing_demo_with_lambda_field_matches_oracle(gpu_lib, oracle, tm
                        timeout=300)
     print(r.stdout[-1500:])
     assert r.returncode == 0
-    fin = np.fromfile(out / "final.bin", np.float32).reshape(3, n, 3)
+    fin = np.fromfile(out / "final.bin", np.float32).reshape(-1, n, 3)
 
     # the oracle's run loop (source/pmMethod.cpp:71-122) in code units
     grid = (128, 128, 64)

@@ -236,9 +236,11 @@ def test_p3m_matches_oracle(gpu_lib, oracle, cs, use_table, eps, a):
     pm.set_particles(pos, vel, m)
     pm.init_greens()
     pm.p3m_configure(cutoff=rc, particle_diameter=a, softening=eps, cloud_shape=cs, use_table=use_table)
+    pm.p3m_record_short_range(True)
     pm.pm_step()
     pm.p3m_short_range()
     _, _, acc = pm.get_particles()
+    a_sr = pm.p3m_short_range_accelerations()
     G = oracle.green_grid(gf, 2, 0, a, grid)
     ref_pm = oracle.pm_accel(pos, m, grid, 2, 0, G)
     box_orig = np.array(grid, np.float32) * H
@@ -246,6 +248,7 @@ def test_p3m_matches_oracle(gpu_lib, oracle, cs, use_table, eps, a):
     ref = ref_pm + ref_sr
     assert _rel(acc, ref) <= FORCE_TOL
     assert _rel(acc - ref_pm, ref_sr) <= 10 * FORCE_TOL  # the short-range part alone
+    assert _rel(a_sr, ref_sr) <= 10 * FORCE_TOL  # as recorded for the drop-in's Particle records
     # r^2 may round differently (FMA) for pairs within an ulp of the cutoff
     assert abs(pm.p3m_count_pairs() - pairs) <= 2 + 1e-5 * pairs
 
