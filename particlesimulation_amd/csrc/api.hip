@@ -291,6 +291,46 @@ static void pm_force(pmgpu_ctx* c, int interp_mode, float kick) {
   interpolate(c, interp_mode, kick, c->iter);
 }
 
+// The P3M force of one run-loop iteration with the short-range cell sort on
+// the second stream, beside the deposit, transforms and interpolation: the
+// sort needs only the particles' order after the tile sort, and the pair
+// pass, which adds to the interpolated accelerations, waits for both.
+static void p3m_force_overlapped(pmgpu_ctx* c, int mode, float kick) {
+  if (!c->side) {
+    HIP_CHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+  }
+  const size_t need = std::max(scan_scratch_elems(c->ncell_sr), scan_scratch_elems(c->ncell_sr + 1));
+  if (need > c->scan_tmp_side_elems) {
+    dalloc(&c->scan_tmp_side, (int64_t)need);
+    c->scan_tmp_side_elems = need;
+  }
+  tile_sort(c);
+  HIP_CHECK(hipEventRecord(c->ev_fork, c->stream));
+  HIP_CHECK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+  {
+    // the sort's kernels go to the side stream with the side scratch
+    struct Swap {
+      pmgpu_ctx* c;
+      explicit Swap(pmgpu_ctx* c_) : c(c_) { flip(); }
+      ~Swap() { flip(); }
+      void flip() {
+        std::swap(c->stream, c->side);
+        std::swap(c->scan_tmp, c->scan_tmp_side);
+        std::swap(c->scan_tmp_elems, c->scan_tmp_side_elems);
+      }
+    } swap(c);
+    p3m_sort(c, c->iter);
+    HIP_CHECK(hipEventRecord(c->ev_join, c->stream));
+  }
+  deposit(c);
+  fft_forward_green_inverse(c);
+  interpolate(c, MODE_ACC, 0.0f, c->iter);
+  HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  p3m_pairs(c, MODE_ACC | mode, kick, c->iter);
+}
+
 }  // namespace pmgpu
 
 using namespace pmgpu;
@@ -410,6 +450,11 @@ void pmgpu_destroy(pmgpu_ctx* c) {
     (void)hipEventDestroy(p.b);
   }
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->side) (void)hipStreamSynchronize(c->side);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->side) (void)hipStreamDestroy(c->side);
+  dfree(&c->scan_tmp_side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -707,8 +752,7 @@ int pmgpu_run(pmgpu_ctx* c, int32_t iterations, int32_t initial, int32_t p3m,
   const int base = c->run_iter;
   auto force = [&](int mode, float kick) {
     if (p3m) {
-      pm_force(c, MODE_ACC, 0.0f);
-      p3m_short_range(c, MODE_ACC | mode, kick, c->iter);
+      p3m_force_overlapped(c, mode, kick);
     } else {
       pm_force(c, MODE_ACC | mode, kick);
     }

@@ -703,8 +703,14 @@ static void launch_sr(pmgpu_ctx* c, int mode, float kick, int iteration) {
   else launch_sr_mass<MODE, false>(c, mode, kick, iteration);
 }
 
+void p3m_sort(pmgpu_ctx* c, int iteration) { sr_sort(c, iteration); }
+
 void p3m_short_range(pmgpu_ctx* c, int mode, float kick, int iteration) {
   sr_sort(c, iteration);
+  p3m_pairs(c, mode, kick, iteration);
+}
+
+void p3m_pairs(pmgpu_ctx* c, int mode, float kick, int iteration) {
   int tok = stage_begin(c, "sr_pairs");
   switch (c->p3m.use_table) {
     case 0: launch_sr<0>(c, mode, kick, iteration); break;

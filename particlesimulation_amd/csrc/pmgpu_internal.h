@@ -147,6 +147,12 @@ struct pmgpu_ctx {
   int64_t fx_slots = 0;
   int32_t* scan_tmp = nullptr;
   size_t scan_tmp_elems = 0;
+  // a second stream for the short-range cell sort, which runs beside the
+  // PM force in pmgpu_run, with its own scan scratch
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int32_t* scan_tmp_side = nullptr;
+  size_t scan_tmp_side_elems = 0;
 
   // grids
   float* rho = nullptr;      // M
@@ -238,6 +244,9 @@ void sr_cells_setup(pmgpu_ctx* c);  // short-range cell grid for c->p3m and the 
 void sr_set_division(pmgpu_ctx* c, int div);  // isotropic division, cells rebuilt if configured
 void sr_choose_cells(pmgpu_ctx* c);           // the cell geometry with the fewest candidates
 void p3m_short_range(pmgpu_ctx* c, int mode, float kick, int iteration);
+// its two halves: the cell sort and the pair pass
+void p3m_sort(pmgpu_ctx* c, int iteration);
+void p3m_pairs(pmgpu_ctx* c, int mode, float kick, int iteration);
 void p3m_count_pairs(pmgpu_ctx* c, int64_t* accepted, int64_t* tested, int64_t* issued);
 // learned correction (mlp.hip)
 struct MlpDevice;
