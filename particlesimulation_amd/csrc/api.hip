@@ -503,6 +503,8 @@ int pmgpu_set_particles(pmgpu_ctx* c, int64_t n, const float* pos, const float* 
   c->sorted_valid = false;
   c->run_iter = 0;
   reset_escape(c);
+  // the accurate pair table's layout depends on the masses' scale
+  if (c->p3m_ready) p3m_setup(c);
   // the short-range cells depend on the particles: their budget on the
   // count, and (unless the division is set) their shape on the positions
   if (c->p3m_ready && n != old_n) {

@@ -74,11 +74,13 @@ __global__ void k_sr_keys(const float4* __restrict__ pm, int64_t n, float ihx, f
 // Also records the sorted positions of the particles off the common mass m0
 // (original ids ascending in odd_ids, nodd of them; nodd <= 0: none kept).
 // The particle arrays are in the tile sort's order: the original id of the
-// particle at i is the int in vel[i].w.
+// particle at i is the int in vel[i].w.  The sorted copy holds positions
+// times pscale and masses times mscale (powers of two: exact).
 __global__ void k_sr_scatter(const float4* __restrict__ pm, const float4* __restrict__ vel, int64_t n,
                              const int32_t* __restrict__ key,
                              const int32_t* __restrict__ rank, const int32_t* __restrict__ start,
-                             float4* __restrict__ sr_pm, int32_t* __restrict__ sr_idx, float m0, int nodd,
+                             float4* __restrict__ sr_pm, int32_t* __restrict__ sr_idx, float pscale,
+                             float mscale, float m0, int nodd,
                              const int32_t* __restrict__ odd_ids, int32_t* __restrict__ odd_pos,
                              const int32_t* __restrict__ escape, int iteration) {
   if (escape[0] <= iteration) return;
@@ -86,7 +88,7 @@ __global__ void k_sr_scatter(const float4* __restrict__ pm, const float4* __rest
   if (i >= n) return;
   int64_t d = (int64_t)start[key[i]] + rank[i];
   const float4 p = pm[i];
-  sr_pm[d] = p;
+  sr_pm[d] = make_float4(p.x * pscale, p.y * pscale, p.z * pscale, p.w * mscale);
   sr_idx[d] = (int32_t)i;
   if (nodd > 0 && __float_as_uint(p.w) != __float_as_uint(m0)) {
     const int32_t id = __float_as_int(vel[i].w);
@@ -159,13 +161,22 @@ struct SrParams {
 //           the r^-3 and r^-1 singularities at small r are resolved to a
 //           fixed relative accuracy), and F is linear in s within the bin:
 //           F = B + A s, one FMA on the clamped s.  s is clamped to the
-//           floats of bits [log_lo, log_hi], 32 binades: closer pairs take
-//           the value at the start of the first bin, and the last bin is
-//           zero (A = B = 0).  MODE 2 serves
-//           the unsoftened force with a <= rc, which vanishes for r >= a, so
-//           the zero bin can start at the bin boundary at or above a^2 with
-//           no scale; MODE 3 puts the cutoff s = 1 on a bin boundary with
-//           one multiply.  The LDS copy holds bin k at k mod SR_LOG_WRAP.
+//           floats of bits [log_lo, log_hi], 31 to 32 binades: closer pairs
+//           take the value at the start of the first bin, and the last bin
+//           is zero (A = B = 0).
+//   MODE 2 serves the unsoftened force with a <= rc, which vanishes for
+//           r >= a, so the zero bin can start at the bin boundary at or
+//           above a^2 with no scale.  Positions arrive scaled by 2^-q, which
+//           puts the first bin at the smallest normal float: the entry of s
+//           is (bits >> 16) - SR_LOG_ABS0, one shift, the constant folded
+//           into the LDS address (the caller passes the table's start minus
+//           SR_LOG_ABS0 entries).  The table is scaled by 2^-t (B) and
+//           2^(2q - t) (A) to keep A finite there; masses arrive scaled by
+//           2^(q + t), which undoes both: every product is the unscaled one
+//           times a power of two, the same floats up to that factor.
+//   MODE 3 puts the cutoff s = 1 on a bin boundary with one multiply (or
+//           runs over r^2 itself, scale 1); its LDS copy holds bin k at
+//           k mod SR_LOG_WRAP, a shift and a mask.
 template <int MODE>
 __device__ __forceinline__ float pair_factor(float r2, float mj, const float2* table,
                                              const SrParams& P, uint32_t log_hi_v) {
@@ -182,8 +193,13 @@ __device__ __forceinline__ float pair_factor(float r2, float mj, const float2* t
     const int t = (int)fminf(ksi, P.tmax_f);
     const float2 tv = table[t];
     return fmaf(__builtin_amdgcn_fractf(ksi), tv.y, tv.x) * mj;
+  } else if constexpr (MODE == 2) {
+    // clamp in the float domain (s >= 0, where float order is bit order)
+    const float sc = __builtin_amdgcn_fmed3f(r2, __uint_as_float(P.log_lo), __uint_as_float(log_hi_v));
+    const float2 tv = table[__float_as_uint(sc) >> SR_LOG_SHIFT];
+    return fmaf(tv.y, sc, tv.x) * mj;
   } else {
-    const float s = MODE == 3 ? r2 * P.log_scale : r2;
+    const float s = r2 * P.log_scale;
     // clamp in the float domain (s >= 0, where float order is bit order)
     const float sc = __builtin_amdgcn_fmed3f(s, __uint_as_float(P.log_lo), __uint_as_float(log_hi_v));
     const float2 tv = table[(__float_as_uint(sc) >> SR_LOG_SHIFT) & (SR_LOG_WRAP - 1)];
@@ -226,13 +242,23 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
   // The partial sums of half of the waves at a time.
   constexpr int HALF = SR_WAVES / 2;
   static_assert(SR_WAVES % 2 == 0, "the partial sums are added in halves");
-  constexpr int TS = MODE >= 2 ? SR_LOG_WRAP : (MODE == 1 ? SR_TABLE : 1);
-  __shared__ float2 table[TS];
-  __shared__ float3 part[HALF * 64];
-  __shared__ int s_w;
+  constexpr int TS = MODE >= 2 ? SR_TABLE_LOG : (MODE == 1 ? SR_TABLE : 1);
+  // One LDS block, the table last: the absolute layout (MODE 2) reads it
+  // through a pointer SR_LOG_ABS0 entries before its start, which stays
+  // inside the block, so the entry offset folds into the LDS address.
+  constexpr int PART_B = HALF * 64 * (int)sizeof(float3);
   // the j-ranges of the work item's (dz, dy) rows, filled in parallel
   static_assert(SR_ROWS_MAX <= SR_THREADS, "one thread per neighbourhood row");
-  __shared__ int2 rows[SR_ROWS_MAX];
+  constexpr int ROWS_B = SR_ROWS_MAX * (int)sizeof(int2);
+  constexpr int TAB_OFF = (PART_B + ROWS_B + (int)sizeof(int) + 15) / 16 * 16;
+  static_assert(TAB_OFF >= SR_LOG_ABS0 * (int)sizeof(float2), "the absolute table's base stays in the block");
+  __shared__ __align__(16) unsigned char lds[TAB_OFF + TS * sizeof(float2)];
+  float3* part = reinterpret_cast<float3*>(lds);
+  int2* rows = reinterpret_cast<int2*>(lds + PART_B);
+  int& s_w = *reinterpret_cast<int*>(lds + PART_B + ROWS_B);
+  float2* table = reinterpret_cast<float2*>(lds + TAB_OFF);
+  const float2* table_rd = reinterpret_cast<const float2*>(
+      lds + TAB_OFF - (MODE == 2 ? SR_LOG_ABS0 * (int)sizeof(float2) : 0));
   const int total = chunk_start[nunit];
   // the first work item before the table: workgroups that find none (small
   // particle sets leave most of the grid idle) exit without loading it
@@ -241,7 +267,9 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
   if (s_w < 0) return;
   if constexpr (MODE == 1)
     for (int i = threadIdx.x; i < SR_TABLE; i += SR_THREADS) table[i] = table_g[i];
-  if constexpr (MODE >= 2) {
+  if constexpr (MODE == 2)
+    for (int i = threadIdx.x; i < SR_TABLE_LOG; i += SR_THREADS) table[i] = table_g[i];
+  if constexpr (MODE == 3) {
     const uint32_t k0 = P.log_lo >> SR_LOG_SHIFT;
     for (int i = threadIdx.x; i < SR_TABLE_LOG; i += SR_THREADS) table[(k0 + i) % SR_LOG_WRAP] = table_g[i];
   }
@@ -319,7 +347,7 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
               const float4 pj = sr_pm[j0 + k];  // uniform address: scalar load
               const float dx = pi.x - pj.x, dy_ = pi.y - pj.y, dz_ = pi.z - pj.z;
               const float r2 = fmaf(dz_, dz_, fmaf(dy_, dy_, dx * dx));
-              const float F = pair_factor<MODE>(r2, UNIFORM ? 1.0f : pj.w, table, P, log_hi_v);
+              const float F = pair_factor<MODE>(r2, UNIFORM ? 1.0f : pj.w, table_rd, P, log_hi_v);
               ax = fmaf(F, dx, ax);
               ay = fmaf(F, dy_, ay);
               az = fmaf(F, dz_, az);
@@ -332,7 +360,7 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
               const float mj = j0 + k < jend ? (UNIFORM ? 1.0f : pj.w) : 0.0f;
               const float dx = pi.x - pj.x, dy_ = pi.y - pj.y, dz_ = pi.z - pj.z;
               const float r2 = fmaf(dz_, dz_, fmaf(dy_, dy_, dx * dx));
-              const float F = pair_factor<MODE>(r2, mj, table, P, log_hi_v);
+              const float F = pair_factor<MODE>(r2, mj, table_rd, P, log_hi_v);
               ax = fmaf(F, dx, ax);
               ay = fmaf(F, dy_, ay);
               az = fmaf(F, dz_, az);
@@ -370,7 +398,7 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
           const float4 pj = sr_pm[odd_pos[q]];
           const float dx = pi.x - pj.x, dy_ = pi.y - pj.y, dz_ = pi.z - pj.z;
           const float r2 = fmaf(dz_, dz_, fmaf(dy_, dy_, dx * dx));
-          const float F = pair_factor<MODE>(r2, pj.w - m0, table, P, log_hi_v);
+          const float F = pair_factor<MODE>(r2, pj.w - m0, table_rd, P, log_hi_v);
           s.x = fmaf(F, dx, s.x);
           s.y = fmaf(F, dy_, s.y);
           s.z = fmaf(F, dz_, s.z);
@@ -559,44 +587,26 @@ void sr_table_host(float rc, float a, float eps, int cs, std::vector<float>& tab
 
 // The accurate table of the direct formula's pair factor
 // F(s) = (-G/(r^2 + eps^2) + R(r)) / r (shortRangeForce,
-// p3mMethod.cpp:220-238) in the bit-binned layout pair_factor<2, 3> reads,
-// s = r^2 scale: entry i covers the floats s whose bits >> 16 equal
-// (lo >> 16) + i, [s0, s1), and holds the chord of F through the bin's ends
-// as (B, A), F = B + A s, lowered by half the sum of its largest deviations
-// above and below F (so the error is centred); the last entry is zero.  With no softening and
-// a <= rc (F = 0 for r >= a) the table runs over r^2 itself (scale 1, the
-// zero bin at the bin boundary at or above a^2); otherwise over r^2 / rc^2
-// (the zero bin at s = 1, the cutoff).  Built in FP64.
-static void sr_table_log_host(float rc, float a, float eps, int cs, std::vector<float>& table,
-                              float& scale, uint32_t& lo, uint32_t& hi, int& mode) {
-  auto to_bits = [](float f) {
-    uint32_t b;
-    std::memcpy(&b, &f, sizeof b);
-    return b;
-  };
+// p3mMethod.cpp:220-238) over s = r^2 scale, binned by float bits: bin i of
+// nbins covers the floats s whose bits >> 16 equal (lo >> 16) + i, [s0, s1),
+// and holds the chord of F through the bin's ends as (B, A), F = B + A s,
+// lowered by half the sum of its largest deviations above and below F (so
+// the error is centred); entry nbins is zero.  Built in FP64, rounded to FP32.
+static void sr_table_log_bins(float a, float eps, int cs, float scale, uint32_t lo, int nbins,
+                              std::vector<float>& table) {
   auto from_bits = [](uint32_t b) {
     float f;
     std::memcpy(&f, &b, sizeof f);
     return (double)f;
   };
   const uint32_t bin = 1u << SR_LOG_SHIFT;
-  if (eps == 0.0f && a <= rc && a >= 1.0e-6f) {
-    mode = 2;
-    scale = 1.0f;
-    hi = (to_bits(a * a) + bin - 1) / bin * bin;
-  } else {
-    mode = 3;
-    scale = 1.0f / (rc * rc);
-    hi = to_bits(1.0f);
-  }
-  lo = hi - (uint32_t)(SR_TABLE_LOG - 1) * bin;
   const double c = (double)scale, G = 1.0 / (4.0 * M_PI), e2 = (double)eps * eps;
   auto F = [&](double sp) {
     const double s = sp / c, r = std::sqrt(s);
     return (-G / (s + e2) + ref_force_d(cs, r, a)) / r;
   };
-  table.assign(2 * SR_TABLE_LOG, 0.0f);
-  for (int i = 0; i + 1 < SR_TABLE_LOG; ++i) {
+  table.assign(2 * (nbins + 1), 0.0f);
+  for (int i = 0; i < nbins; ++i) {
     const uint32_t b0 = lo + (uint32_t)i * bin;
     const double s0 = from_bits(b0), s1 = from_bits(b0 + bin);
     const double f0 = F(s0), f1 = F(s1);
@@ -612,19 +622,82 @@ static void sr_table_log_host(float rc, float a, float eps, int cs, std::vector<
   }
 }
 
+// The accurate table in the layout the pair loop reads.
+//  - Unsoftened with a <= rc (F = 0 for r >= a), over r^2 itself: the zero
+//    bin at the bin boundary at or above a^2 (hi), the first bin at the start
+//    of the binade 30 or 31 below hi's (lo, an odd exponent field), so with
+//    positions scaled by 2^-q the first bin starts at the smallest normal
+//    float, entry SR_LOG_ABS0 (MODE 2, absolute layout).  B is scaled by 2^-t
+//    and A by 2^(2q - t), t putting the largest |A| in [2^125, 2^126) (A
+//    grows as r^-5 towards lo), and masses by 2^(q + t), which undoes both;
+//    all scalings are powers of two.  Where the scaled masses could overflow
+//    (sum |m| 2^(q + t) >= 2^120) the same range stays unscaled and wrapped
+//    (MODE 3, scale 1).
+//  - Otherwise over r^2 / rc^2, the zero bin at s = 1 (the cutoff) and
+//    SR_TABLE_LOG - 1 bins below it, wrapped: bin k at k mod SR_LOG_WRAP
+//    (MODE 3).
+static void sr_table_log_host(pmgpu_ctx* c, std::vector<float>& table) {
+  const float rc = c->p3m.cutoff, a = c->p3m.particle_diameter, eps = c->p3m.softening;
+  const int cs = c->p3m.cloud_shape;
+  const uint32_t bin = 1u << SR_LOG_SHIFT;
+  c->sr_pos_scale = c->sr_mass_scale = 1.0f;
+  c->sr_log_mode = 3;
+  if (!(eps == 0.0f && a <= rc && a >= 1.0e-6f)) {
+    c->sr_log_scale = 1.0f / (rc * rc);
+    c->sr_log_hi = 0x3f800000u;  // 1.0f
+    c->sr_log_lo = c->sr_log_hi - (uint32_t)(SR_TABLE_LOG - 1) * bin;
+    sr_table_log_bins(a, eps, cs, c->sr_log_scale, c->sr_log_lo, SR_TABLE_LOG - 1, table);
+    return;
+  }
+  uint32_t hi;
+  const float a2 = a * a;
+  std::memcpy(&hi, &a2, sizeof hi);
+  hi = (hi + bin - 1) / bin * bin;
+  const int e_hi = (int)(hi >> 23);
+  const int e_lo = (e_hi - 31) % 2 != 0 ? e_hi - 31 : e_hi - 30;
+  const uint32_t lo = (uint32_t)e_lo << 23;
+  const int nbins = (int)((hi - lo) / bin);  // <= 31 * 128 + 127
+  const int q = (e_lo - 1) / 2;
+  c->sr_log_scale = 1.0f;
+  std::vector<float> bins;
+  sr_table_log_bins(a, eps, cs, 1.0f, lo, nbins, bins);
+  float amax = 0.0f;
+  for (int i = 0; i < nbins; ++i) amax = std::max(amax, std::fabs(bins[2 * i + 1]));
+  const int t = std::isfinite(amax) && amax > 0.0f ? std::ilogb(amax) + 2 * q - 125 : 0;
+  const bool scaled = e_lo >= 3 && nbins + 1 <= SR_TABLE_LOG && std::isfinite(amax) && amax > 0.0f &&
+                      q + t <= 126 && q + t >= -126 && std::ldexp(c->mass_abs_sum, q + t) < 0x1p120;
+  if (!scaled) {
+    // wrapped: the same bins from lo, the zero bin repeated up to SR_TABLE_LOG
+    c->sr_log_lo = lo;
+    c->sr_log_hi = hi;
+    table = bins;
+    table.resize(2 * SR_TABLE_LOG, 0.0f);
+    return;
+  }
+  table.assign(2 * SR_TABLE_LOG, 0.0f);
+  for (int i = 0; i < nbins; ++i) {
+    table[2 * i] = std::ldexp(bins[2 * i], -t);
+    table[2 * i + 1] = std::ldexp(bins[2 * i + 1], 2 * q - t);
+  }
+  c->sr_log_mode = 2;
+  c->sr_log_lo = lo - ((uint32_t)(2 * q) << 23);  // 2^-126
+  c->sr_log_hi = hi - ((uint32_t)(2 * q) << 23);
+  c->sr_pos_scale = std::ldexp(1.0f, -q);
+  c->sr_mass_scale = std::ldexp(1.0f, q + t);
+}
+
 void p3m_setup(pmgpu_ctx* c) {
   std::vector<float> tab;
   float d2 = 0.0f;
   if (c->p3m.use_table == 2) {
-    sr_table_log_host(c->p3m.cutoff, c->p3m.particle_diameter, c->p3m.softening,
-                      c->p3m.cloud_shape, tab, c->sr_log_scale, c->sr_log_lo, c->sr_log_hi,
-                      c->sr_log_mode);
+    sr_table_log_host(c, tab);
     d2 = 1.0f;
     c->sr_tmax = SR_TABLE_LOG - 1;
   } else {
     sr_table_host(c->p3m.cutoff, c->p3m.particle_diameter, c->p3m.softening, c->p3m.cloud_shape,
                   tab, d2);
     c->sr_tmax = SR_TABLE - 1;
+    c->sr_pos_scale = c->sr_mass_scale = 1.0f;
   }
   HIP_CHECK(hipMemcpy(c->sr_table, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
   c->sr_inv_d2 = (float)(1.0 / (double)d2);
@@ -643,13 +716,15 @@ static SrParams sr_params(pmgpu_ctx* c) {
   P.divz = c->sr_divz;
   P.divx = c->sr_divx_base * c->sr_xfine;
   P.rowspan = sr_rowspan(c);
-  P.ihx = 1.0f / c->hcx;
+  // the sorted copy's positions are scaled by sr_pos_scale (a power of two):
+  // its cell index and cutoff test scale with it, exactly
+  P.ihx = (1.0f / c->hcx) / c->sr_pos_scale;
   P.hx = c->hcx;
   P.hy = c->hcy;
   P.hz = c->hcz;
   P.rc2m = c->p3m.cutoff * c->p3m.cutoff * (1.0f + 1e-4f);
   P.inv_d2 = c->sr_inv_d2;
-  P.rc2 = c->p3m.cutoff * c->p3m.cutoff;
+  P.rc2 = c->p3m.cutoff * c->p3m.cutoff * (c->sr_pos_scale * c->sr_pos_scale);
   P.a = c->p3m.particle_diameter;
   P.eps2 = c->p3m.softening * c->p3m.softening;
   P.cs = c->p3m.cloud_shape;
@@ -671,7 +746,8 @@ static void sr_sort(pmgpu_ctx* c, int iteration) {
   exclusive_scan_i32(c, c->sr_count, c->sr_start, c->ncell_sr);
   hipLaunchKernelGGL(k_sr_scatter, dim3(g), dim3(256), 0, c->stream, c->pm[c->cur], c->vel[c->cur], c->n,
                      c->sr_key,
-                     c->sr_rank, c->sr_start, c->sr_pm, c->sr_idx, c->sr_m0, c->sr_nodd, c->sr_odd_ids,
+                     c->sr_rank, c->sr_start, c->sr_pm, c->sr_idx, c->sr_pos_scale, c->sr_mass_scale,
+                     c->sr_m0, c->sr_nodd, c->sr_odd_ids,
                      c->sr_odd_pos, c->escape, iteration);
   HIP_CHECK(hipGetLastError());
   const int nunit = sr_units(c), stride = sr_rowspan(c) ? c->mcx : 1;
@@ -690,7 +766,7 @@ static void launch_sr_mass(pmgpu_ctx* c, int mode, float kick, int iteration) {
   hipLaunchKernelGGL((k_sr<MODE, UNIFORM>), dim3(grid), dim3(SR_THREADS), 0, c->stream, c->sr_pm,
                      c->sr_start, c->sr_chunk_start, c->sr_owner, sr_units(c), c->sr_idx, c->sr_table,
                      sr_params(c), c->pm[c->cur], c->vel[c->cur], c->acc, mode, kick,
-                     make_float3(c->prm.box[0], c->prm.box[1], c->prm.box[2]), c->sr_m0,
+                     make_float3(c->prm.box[0], c->prm.box[1], c->prm.box[2]), c->sr_m0 * c->sr_mass_scale,
                      std::max(c->sr_nodd, 0), c->sr_odd_pos, c->counters, c->acc_sr, c->escape, iteration);
   HIP_CHECK(hipGetLastError());
 }

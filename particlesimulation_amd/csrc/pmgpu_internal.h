@@ -46,13 +46,19 @@ constexpr int SR_ODD_MAX = 64;  // particles off the common mass that the unifor
 constexpr int64_t SR_CELLS_MAX = 128ll * 1024 * 1024;  // short-range cells, x refinement included
 constexpr int SR_TABLE = 512;  // padded reference table (the reference uses 500 entries)
 constexpr int SR_TABLE_REF = 500;
-// The accurate pair table: SR_LOG_WRAP bins of s = r^2 (times a scale)
-// taken from the float bits of s, 2^(23 - SR_LOG_SHIFT) bins per binade, the
-// last of them zero.  In LDS, bin k sits at k mod SR_LOG_WRAP, so its byte
-// address is a shift and a mask of the bits.
+// The accurate pair table: at most SR_TABLE_LOG bins of s = r^2 (times a
+// scale) taken from the float bits of s, 2^(23 - SR_LOG_SHIFT) bins per
+// binade, the last of them zero.  Two layouts:
+//   absolute (over r^2, unsoftened, a <= rc): the short-range pass scales
+//     positions by 2^-q so that the first bin starts at the smallest normal
+//     float, whose bits >> SR_LOG_SHIFT are SR_LOG_ABS0; the bin of s sits at
+//     entry (bits(s) >> SR_LOG_SHIFT) - SR_LOG_ABS0, one shift of the bits
+//     with the constant folded into the LDS address;
+//   wrapped (otherwise): bin k sits at k mod SR_LOG_WRAP, a shift and a mask.
 constexpr int SR_LOG_SHIFT = 16;
 constexpr int SR_LOG_WRAP = 4096;
 constexpr int SR_TABLE_LOG = SR_LOG_WRAP;
+constexpr int SR_LOG_ABS0 = (1 << 23) >> SR_LOG_SHIFT;
 
 struct Stage {
   const char* name;
@@ -178,7 +184,10 @@ struct pmgpu_ctx {
   float sr_inv_d2 = 0.0f;
   float sr_log_scale = 1.0f;             // the accurate table's argument s = r^2 * scale
   uint32_t sr_log_lo = 0, sr_log_hi = 0;  // float bits of its first and last (zero) bin
-  int sr_log_mode = 3;                    // 2: over r^2 (unsoftened, a <= rc), 3: over r^2 / rc^2
+  int sr_log_mode = 3;                    // 2: absolute layout over r^2 (scaled), 3: wrapped
+  // the pass's sorted copy holds positions x sr_pos_scale and masses x
+  // sr_mass_scale (powers of two; 1 outside the absolute layout)
+  float sr_pos_scale = 1.0f, sr_mass_scale = 1.0f;
   int sr_tmax = 0;
   int32_t* sr_count = nullptr;          // ncell_sr + 1
   int32_t* sr_start = nullptr;          // ncell_sr + 1
