@@ -68,6 +68,13 @@ struct Geo {
   static constexpr int F = FDS == PMGPU_TWO_POINT ? 1 : 2;
   static constexpr int EX = TILE_X + CNT - 1, EY = TILE_Y + CNT - 1, EZ = TILE_Z + CNT - 1;
   static constexpr int PX = EX + 2 * F, PY = EY + 2 * F, PZ = EZ + 2 * F;
+  // The staged potential's x rows start at the 4-aligned cell ox - 4 (the
+  // region's first cell ox + SLO - F is 1 to 3 cells later: XOFF) and hold
+  // NCH aligned float4 chunks, so a row is NCH 16-byte loads; a chunk wraps
+  // as a unit because the grid's x side is a multiple of 4.
+  static constexpr int XOFF = SLO - F + 4;
+  static constexpr int NCH = (XOFF + PX + 3) / 4;
+  static constexpr int SPX = 4 * NCH;
 };
 
 template <int SCHEME>
@@ -85,8 +92,10 @@ __device__ __forceinline__ void interp_weights(float d, float w[3]) {
   }
 }
 
+// At most 128 VGPRs: four 256-thread workgroups per CU, as many as the
+// LDS block of the largest stencil allows.
 template <int SCHEME, int FDS>
-__global__ __launch_bounds__(256) void k_interp(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_interp(
     const float* __restrict__ phi, int nx, int ny, int nz, int ntx, int nty, int ntiles,
     const int32_t* __restrict__ cls_start, const int32_t* __restrict__ chunk_start,
     const int32_t* __restrict__ owner, float4* __restrict__ pm, float4* __restrict__ vel, float4* __restrict__ acc, int mode,
@@ -94,8 +103,16 @@ __global__ __launch_bounds__(256) void k_interp(
     int iteration) {
   if (escape[0] <= iteration) return;
   using Gm = Geo<SCHEME, FDS>;
-  __shared__ float sphi[Gm::PX * Gm::PY * Gm::PZ];
-  __shared__ float sE[3][Gm::EX * Gm::EY * Gm::EZ];
+  // The potential and the field share one LDS block: each thread forms its
+  // share of E in registers, and E overwrites the potential after a barrier,
+  // which halves the block and doubles the workgroups a CU holds.
+  constexpr int NP = Gm::SPX * Gm::PY * Gm::PZ, NE = Gm::EX * Gm::EY * Gm::EZ;
+  constexpr int KE = (NE + 255) / 256;  // field points per thread
+  __shared__ __align__(16) float lds[NP > 3 * NE ? NP : 3 * NE];
+  float* const sphi = lds;
+  float* const sE0 = lds;
+  float* const sE1 = lds + NE;
+  float* const sE2 = lds + 2 * NE;
   __shared__ int s_tile, s_chunk;
   const int total = chunk_start[ntiles];
   const XcdRange wr = xcd_range(total);
@@ -109,39 +126,55 @@ __global__ __launch_bounds__(256) void k_interp(
     const int t = s_tile;
     const int ox = (t % ntx) * TILE_X, oy = ((t / ntx) % nty) * TILE_Y,
               oz = (t / (ntx * nty)) * TILE_Z;
-    const int px0 = ox + Gm::SLO - Gm::F, py0 = oy + Gm::SLO - Gm::F, pz0 = oz + Gm::SLO - Gm::F;
-    constexpr int NP = Gm::PX * Gm::PY * Gm::PZ, UNR = 16;
-    for (int e0 = threadIdx.x; e0 < NP; e0 += 256 * UNR) {
-      float v[UNR];
+    const int py0 = oy + Gm::SLO - Gm::F, pz0 = oz + Gm::SLO - Gm::F;
+    // stage the potential in float4 chunks of x rows (NCH per row)
+    constexpr int NF = Gm::NCH * Gm::PY * Gm::PZ, UNR = 4;
+    for (int f0 = threadIdx.x; f0 < NF; f0 += 256 * UNR) {
+      float4 v[UNR];
 #pragma unroll
       for (int q = 0; q < UNR; ++q) {
-        const int e = e0 + 256 * q;
-        const int ix = e % Gm::PX, iy = (e / Gm::PX) % Gm::PY, iz = e / (Gm::PX * Gm::PY);
-        const uint32_t gx = wrapc(px0 + ix, nx), gy = wrapc(py0 + iy, ny), gz = wrapc(pz0 + iz, nz);
-        v[q] = e < NP ? phi[gx + (uint32_t)nx * (gy + (uint32_t)ny * gz)] : 0.0f;
+        const int f = f0 + 256 * q;
+        const int cx = f % Gm::NCH, row = f / Gm::NCH;
+        const int iy = row % Gm::PY, iz = row / Gm::PY;
+        const uint32_t gx = wrapc(ox - 4 + 4 * cx, nx), gy = wrapc(py0 + iy, ny), gz = wrapc(pz0 + iz, nz);
+        v[q] = f < NF ? *reinterpret_cast<const float4*>(phi + gx + (uint32_t)nx * (gy + (uint32_t)ny * gz))
+                      : make_float4(0.f, 0.f, 0.f, 0.f);
       }
 #pragma unroll
-      for (int q = 0; q < UNR; ++q)
-        if (e0 + 256 * q < NP) sphi[e0 + 256 * q] = v[q];
+      for (int q = 0; q < UNR; ++q) {
+        const int f = f0 + 256 * q;
+        if (f < NF) *reinterpret_cast<float4*>(sphi + 4 * f) = v[q];
+      }
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < Gm::EX * Gm::EY * Gm::EZ; e += 256) {
-      const int ex = e % Gm::EX, ey = (e / Gm::EX) % Gm::EY, ez = e / (Gm::EX * Gm::EY);
-      const int c = (ex + Gm::F) + Gm::PX * ((ey + Gm::F) + Gm::PY * (ez + Gm::F));
-      const int sx = 1, sy = Gm::PX, sz = Gm::PX * Gm::PY;
-      float Ex, Ey, Ez;
-      if constexpr (FDS == PMGPU_TWO_POINT) {
-        Ex = -0.5f * (sphi[c + sx] - sphi[c - sx]);
-        Ey = -0.5f * (sphi[c + sy] - sphi[c - sy]);
-        Ez = -0.5f * (sphi[c + sz] - sphi[c - sz]);
-      } else {
-        Ex = (-1.0f / 12) * (-sphi[c + 2 * sx] + 8 * sphi[c + sx] - 8 * sphi[c - sx] + sphi[c - 2 * sx]);
-        Ey = (-1.0f / 12) * (-sphi[c + 2 * sy] + 8 * sphi[c + sy] - 8 * sphi[c - sy] + sphi[c - 2 * sy]);
-        Ez = (-1.0f / 12) * (-sphi[c + 2 * sz] + 8 * sphi[c + sz] - 8 * sphi[c - sz] + sphi[c - 2 * sz]);
+    float rEx[KE], rEy[KE], rEz[KE];
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int e = (int)threadIdx.x + 256 * k;
+      if (e < NE) {
+        const int ex = e % Gm::EX, ey = (e / Gm::EX) % Gm::EY, ez = e / (Gm::EX * Gm::EY);
+        const int c = (ex + Gm::F + Gm::XOFF) + Gm::SPX * ((ey + Gm::F) + Gm::PY * (ez + Gm::F));
+        const int sx = 1, sy = Gm::SPX, sz = Gm::SPX * Gm::PY;
+        if constexpr (FDS == PMGPU_TWO_POINT) {
+          rEx[k] = -0.5f * (sphi[c + sx] - sphi[c - sx]);
+          rEy[k] = -0.5f * (sphi[c + sy] - sphi[c - sy]);
+          rEz[k] = -0.5f * (sphi[c + sz] - sphi[c - sz]);
+        } else {
+          rEx[k] = (-1.0f / 12) * (-sphi[c + 2 * sx] + 8 * sphi[c + sx] - 8 * sphi[c - sx] + sphi[c - 2 * sx]);
+          rEy[k] = (-1.0f / 12) * (-sphi[c + 2 * sy] + 8 * sphi[c + sy] - 8 * sphi[c - sy] + sphi[c - 2 * sy]);
+          rEz[k] = (-1.0f / 12) * (-sphi[c + 2 * sz] + 8 * sphi[c + sz] - 8 * sphi[c - sz] + sphi[c - 2 * sz]);
+        }
       }
-      sE[0][e] = Ex;
-      sE[1][e] = Ey;
-      sE[2][e] = Ez;
+    }
+    __syncthreads();  // every read of the potential is done: E may overwrite it
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int e = (int)threadIdx.x + 256 * k;
+      if (e < NE) {
+        sE0[e] = rEx[k];
+        sE1[e] = rEy[k];
+        sE2[e] = rEz[k];
+      }
     }
     __syncthreads();
     const int pbeg = cls_start[t * NCLASS] + s_chunk * int_chunk;
@@ -154,8 +187,9 @@ __global__ __launch_bounds__(256) void k_interp(
       interp_weights<SCHEME>(p.y - (float)ay, wy);
       interp_weights<SCHEME>(p.z - (float)az, wz);
       const int lx = wrapc(ax, nx) - ox, ly = wrapc(ay, ny) - oy, lz = wrapc(az, nz) - oz;
+      const float3 g = ext_field(ext, p.x, p.y, p.z);
       float sx = 0.f, sy = 0.f, sz = 0.f;
-#pragma unroll
+#pragma unroll 1
       for (int a = 0; a < Gm::CNT; ++a) {
         const float t1 = wx[a];
 #pragma unroll
@@ -166,13 +200,12 @@ __global__ __launch_bounds__(256) void k_interp(
             float t3 = t2 * wz[c];
             if constexpr (SCHEME == PMGPU_TSC) t3 *= 0.125f;
             const int e = (lx + a) + Gm::EX * ((ly + b) + Gm::EY * (lz + c));
-            sx += t3 * sE[0][e];
-            sy += t3 * sE[1][e];
-            sz += t3 * sE[2][e];
+            sx += t3 * sE0[e];
+            sy += t3 * sE1[e];
+            sz += t3 * sE2[e];
           }
         }
       }
-      const float3 g = ext_field(ext, p.x, p.y, p.z);
       sx += g.x;
       sy += g.y;
       sz += g.z;
