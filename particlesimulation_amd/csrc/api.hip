@@ -439,7 +439,8 @@ void pmgpu_destroy(pmgpu_ctx* c) {
   dfree(&c->tile_slot); dfree(&c->slot_tile); dfree(&c->int_nchunk); dfree(&c->int_chunk_start);
   dfree(&c->dep_table); dfree(&c->counters); dfree(&c->fx_accum); dfree(&c->scan_tmp);
   dfree(&c->dep_owner); dfree(&c->int_owner); dfree(&c->sr_owner);
-  dfree(&c->rho); dfree(&c->spec); dfree(&c->phi); dfree(&c->greens); dfree(&c->field); dfree(&c->efield);
+  dfree(&c->rho); dfree(&c->spec); dfree(&c->phi); dfree(&c->greens); dfree(&c->greens_z); dfree(&c->field);
+  dfree(&c->efield);
   dfree(&c->tw_x); dfree(&c->tw_y); dfree(&c->tw_z); dfree(&c->tw_r2c); dfree(&c->tw_gx); dfree(&c->fx_grid);
   dfree(&c->sr_table); dfree(&c->sr_count); dfree(&c->sr_start); dfree(&c->sr_key);
   dfree(&c->sr_rank); dfree(&c->sr_pm); dfree(&c->sr_idx); dfree(&c->sr_nchunk);
@@ -591,6 +592,10 @@ int pmgpu_init_greens(pmgpu_ctx* c) {
   PMGPU_API_BEGIN(c)
   greens_compute(c, nullptr);
   if (!c->mlp_weights.empty()) mlp_apply_to_greens(c);  // G_sym * (1 + y(k))
+  if (!c->general) {
+    if (!c->greens_z) dalloc(&c->greens_z, c->K);  // (Nx/2 + 1) Ny Nz <= K values
+    greens_block_for_z(c);
+  }
   HIP_CHECK(hipStreamSynchronize(c->stream));
   c->greens_ready = true;
   PMGPU_API_END(c)

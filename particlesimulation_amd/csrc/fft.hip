@@ -288,14 +288,35 @@ __global__ __launch_bounds__(TW * (L / 8)) void k_fft_y(float2* __restrict__ spe
 // so a cached density spectrum can be reused with several G).
 constexpr int Z_GREEN = 0, Z_FWD = 1, Z_MUL_INV = 2;
 
+//
+// A one-dimensional grid of ntile * ny blocks.  Blocks are dealt to the 8
+// XCDs round-robin (block id mod 8); with pair set, the two tiles (2q,
+// 2q + 1) at one y get ids 8 apart, so they run on one XCD, one after the
+// other, and the 128-byte lines of G that their rows of TW = 16 floats share
+// are fetched into that XCD's L2 once (the spectrum's rows are whole lines).
+//
+// G is read in the spectrum's layout (gnxh = 0) or in the tile order of
+// greens_z (gnxh = Nx/2 + 1): the tile at (y, c0) holds its L x ncols values
+// contiguously from (y gnxh + c0) L, element e = n ncols + col, so a
+// thread's G loads are consecutive with its spectrum loads' e.
 template <int L, int TW, int MODE>
 __global__ __launch_bounds__(TW * (L / 8)) void k_fft_z(
     const float2* __restrict__ src, float2* __restrict__ dst, const float* __restrict__ greens, int sp,
-    int ny, int ntile, const float2* __restrict__ tw, const int32_t* __restrict__ escape, int iteration) {
+    int ny, int ntile, int pair, int gnxh, const float2* __restrict__ tw, const int32_t* __restrict__ escape,
+    int iteration) {
   if (escape[0] <= iteration) return;
   using CT = ColTile<L, TW>;
   __shared__ float2 buf[L * CT::NC];
-  const int tile = blockIdx.x, y = blockIdx.y;
+  int tile, y;
+  if (pair) {
+    const int b = blockIdx.x, r = b >> 3, half = ntile >> 1;
+    const int p = (r >> 1) * 8 + (b & 7);
+    y = p / half;
+    tile = 2 * (p - y * half) + (r & 1);
+  } else {
+    tile = blockIdx.x % ntile;
+    y = blockIdx.x / ntile;
+  }
   const int c0 = tile * CT::TW;
   const int ncols = tile == ntile - 1 ? CT::TW + 1 : CT::TW;
   const int64_t zstride = (int64_t)ny * sp;
@@ -311,7 +332,8 @@ __global__ __launch_bounds__(TW * (L / 8)) void k_fft_z(
     if (e < L * ncols) {
       const int n = e / ncols, col = e % ncols;
       v[q] = src[off + n * zstride + col];
-      if constexpr (MODE != Z_FWD) g[q] = greens[off + n * zstride + col];
+      if constexpr (MODE != Z_FWD)
+        g[q] = gnxh ? greens[((int64_t)y * gnxh + c0) * L + e] : greens[off + n * zstride + col];
     }
   }
 #pragma unroll
@@ -378,19 +400,43 @@ void launch_y(pmgpu_ctx* c, float2* spec) {
 }
 
 template <int L, int TW, int MODE>
-void launch_z_tw(pmgpu_ctx* c, const float2* src, float2* dst, const float* g) {
+void launch_z_tw(pmgpu_ctx* c, const float2* src, float2* dst, const float* g, int gnxh) {
   using CT = ColTile<L, TW>;
   const int ntile = (c->nx / 2) / CT::TW;
-  hipLaunchKernelGGL((k_fft_z<L, TW, MODE>), dim3(ntile, c->ny), dim3(CT::THREADS), 0, c->stream, src, dst,
-                     g, c->sp, c->ny, ntile, c->tw_z, c->escape, c->iter);
+  // the pairing is a bijection of the block ids when 8 divides the pairs
+  const int pair = ntile % 2 == 0 && (ntile / 2 * c->ny) % 8 == 0;
+  hipLaunchKernelGGL((k_fft_z<L, TW, MODE>), dim3(ntile * c->ny), dim3(CT::THREADS), 0, c->stream, src, dst,
+                     g, c->sp, c->ny, ntile, pair, gnxh, c->tw_z, c->escape, c->iter);
+}
+
+// the z pass's column-tile width for nz = L
+template <int L>
+int z_tile_width(const pmgpu_ctx* c) {
+  if constexpr (L <= 256) {
+    if (c->nx / 2 >= 32) return 32;
+  }
+  return tile_width(L);
 }
 
 template <int L, int MODE>
-void launch_z(pmgpu_ctx* c, const float2* src, float2* dst, const float* g) {
+void launch_z(pmgpu_ctx* c, const float2* src, float2* dst, const float* g, int gnxh) {
   if constexpr (L <= 256) {
-    if (c->nx / 2 >= 32) return launch_z_tw<L, 32, MODE>(c, src, dst, g);
+    if (z_tile_width<L>(c) == 32) return launch_z_tw<L, 32, MODE>(c, src, dst, g, gnxh);
   }
-  launch_z_tw<L, tile_width(L), MODE>(c, src, dst, g);
+  launch_z_tw<L, tile_width(L), MODE>(c, src, dst, g, gnxh);
+}
+
+// greens (spectrum layout, pitch sp) -> out in the z pass's tile order
+__global__ void k_greens_block(const float* __restrict__ greens, int sp, int nx, int ny, int nz, int tw,
+                               float* __restrict__ out) {
+  const int nxh = nx / 2 + 1, ntile = (nx / 2) / tw;
+  const int64_t n = (int64_t)nxh * ny * nz;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int kx = (int)(i % nxh), ky = (int)((i / nxh) % ny), kz = (int)(i / ((int64_t)nxh * ny));
+  const int tile = min(kx / tw, ntile - 1), c0 = tile * tw, col = kx - c0;
+  const int ncols = tile == ntile - 1 ? tw + 1 : tw;
+  out[((int64_t)ky * nxh + c0) * nz + (int64_t)kz * ncols + col] = greens[((int64_t)kz * ny + ky) * sp + kx];
 }
 
 template <bool INV>
@@ -408,17 +454,30 @@ void y_pass(pmgpu_ctx* c, float2* spec) {
 }
 
 template <int MODE>
-void z_pass(pmgpu_ctx* c, const float2* src, float2* dst, const float* g) {
+void z_pass(pmgpu_ctx* c, const float2* src, float2* dst, const float* g, int gnxh = 0) {
   switch (c->nz) {
-    case 32: launch_z<32, MODE>(c, src, dst, g); break;
-    case 64: launch_z<64, MODE>(c, src, dst, g); break;
-    case 128: launch_z<128, MODE>(c, src, dst, g); break;
-    case 256: launch_z<256, MODE>(c, src, dst, g); break;
-    case 512: launch_z<512, MODE>(c, src, dst, g); break;
-    case 1024: launch_z<1024, MODE>(c, src, dst, g); break;
+    case 32: launch_z<32, MODE>(c, src, dst, g, gnxh); break;
+    case 64: launch_z<64, MODE>(c, src, dst, g, gnxh); break;
+    case 128: launch_z<128, MODE>(c, src, dst, g, gnxh); break;
+    case 256: launch_z<256, MODE>(c, src, dst, g, gnxh); break;
+    case 512: launch_z<512, MODE>(c, src, dst, g, gnxh); break;
+    case 1024: launch_z<1024, MODE>(c, src, dst, g, gnxh); break;
     default: fail(PMGPU_ERR_INVALID_ARGUMENT, "unsupported nz");
   }
   HIP_CHECK(hipGetLastError());
+}
+
+int z_width(const pmgpu_ctx* c) {
+  switch (c->nz) {
+    case 32: return z_tile_width<32>(c);
+    case 64: return z_tile_width<64>(c);
+    case 128: return z_tile_width<128>(c);
+    case 256: return z_tile_width<256>(c);
+    case 512: return z_tile_width<512>(c);
+    case 1024: return z_tile_width<1024>(c);
+    default: fail(PMGPU_ERR_INVALID_ARGUMENT, "unsupported nz");
+  }
+  return 0;
 }
 
 void x_r2c(pmgpu_ctx* c, const float* in, float2* spec) {
@@ -446,7 +505,8 @@ void fft_forward_green_inverse(pmgpu_ctx* c) {
   y_pass<false>(c, c->spec);
   stage_end(c, tok);
   tok = stage_begin(c, "fft_z_green");
-  z_pass<Z_GREEN>(c, c->spec, c->spec, c->greens);
+  if (c->greens_z_valid) z_pass<Z_GREEN>(c, c->spec, c->spec, c->greens_z, c->nx / 2 + 1);
+  else z_pass<Z_GREEN>(c, c->spec, c->spec, c->greens);
   stage_end(c, tok);
   tok = stage_begin(c, "fft_y_inv");
   y_pass<true>(c, c->spec);
@@ -454,6 +514,14 @@ void fft_forward_green_inverse(pmgpu_ctx* c) {
   tok = stage_begin(c, "fft_x_c2r");
   x_c2r(c, c->spec, c->phi);
   stage_end(c, tok);
+}
+
+void greens_block_for_z(pmgpu_ctx* c) {
+  const int64_t n = (int64_t)(c->nx / 2 + 1) * c->ny * c->nz;
+  hipLaunchKernelGGL(k_greens_block, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->greens, c->sp,
+                     c->nx, c->ny, c->nz, z_width(c), c->greens_z);
+  HIP_CHECK(hipGetLastError());
+  c->greens_z_valid = true;
 }
 
 void fft_r2c_forward(pmgpu_ctx* c, const float* in, float2* spec) {

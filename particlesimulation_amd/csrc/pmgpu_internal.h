@@ -165,6 +165,11 @@ struct pmgpu_ctx {
   float2* spec = nullptr;    // K
   float* phi = nullptr;      // M
   float* greens = nullptr;   // K (symmetrised, /M)
+  // G in the z pass's tile order (per y and column tile, the tile's L x ncols
+  // values contiguous), built by pmgpu_init_greens; any rewrite of greens
+  // clears greens_z_valid
+  float* greens_z = nullptr;
+  bool greens_z_valid = false;
   float* field = nullptr;    // 3M, only on request
   float2* tw_x = nullptr;    // twiddles for nx/2
   float2* tw_y = nullptr;
@@ -238,6 +243,8 @@ bool general_grid(int nx, int ny, int nz);
 void deposit_general(pmgpu_ctx* c);
 void fft_general(pmgpu_ctx* c);
 void fft_forward_green_inverse(pmgpu_ctx* c);
+// greens -> greens_z (power-of-two grids)
+void greens_block_for_z(pmgpu_ctx* c);
 // unnormalised forward R2C of a real grid into a half spectrum (in may be rho)
 void fft_r2c_forward(pmgpu_ctx* c, const float* in, float2* spec);
 // out = unnormalised C2R of (src * g); src is kept, work holds the spectrum
