@@ -23,7 +23,7 @@ namespace pmgpu {
 namespace {
 
 constexpr int DEP_THREADS = 256;
-constexpr int MAX_RANGES = 99;
+constexpr int MAX_RANGES = 101;  // 3 own ranges + 98 neighbour class ranges (TSC)
 
 // Slots of the device counter block (ctx->counters).
 constexpr int CNT_SLOTS = 0;
@@ -59,19 +59,25 @@ __device__ __forceinline__ void axis_weights(int scheme, float d, float w[3]) {
   }
 }
 
-// Number of ranges and their virtual lengths for tile t.  Entry 0 is the own
-// tile (all 27 classes); entries 1.. come from the host-built table.
+// Ranges of tile t's virtual particle list and their lengths.  Entries 0-2
+// are the tile's own particles: 0 the interior class (13: the anchor at
+// least one cell from every face, so every stencil cell of any scheme lies in
+// the tile), 1 classes 0-12, 2 classes 14-26; entries 3.. come from the
+// host-built neighbour table.
+constexpr int OWN_RANGES = 3, INTERIOR_CLASS = 13;
 __device__ __forceinline__ void tile_range(int r, int t, int ntx, int nty, int ntz,
                                            const int4* __restrict__ table,
                                            const int32_t* __restrict__ cls_start, int32_t* start,
                                            int32_t* len) {
-  if (r == 0) {
-    int32_t s = cls_start[t * NCLASS];
+  if (r < OWN_RANGES) {
+    const int lo = r == 0 ? INTERIOR_CLASS : (r == 1 ? 0 : INTERIOR_CLASS + 1);
+    const int hi = r == 0 ? INTERIOR_CLASS + 1 : (r == 1 ? INTERIOR_CLASS : NCLASS);
+    int32_t s = cls_start[t * NCLASS + lo];
     *start = s;
-    *len = cls_start[t * NCLASS + NCLASS] - s;
+    *len = cls_start[t * NCLASS + hi] - s;
     return;
   }
-  int4 e = table[r - 1];
+  int4 e = table[r - OWN_RANGES];
   int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
   int ux = wrapc(tx + e.x, ntx), uy = wrapc(ty + e.y, nty), uz = wrapc(tz + e.z, ntz);
   int k = (ux + ntx * (uy + nty * uz)) * NCLASS + e.w;
@@ -95,16 +101,15 @@ __global__ void k_deposit_plan(int ntiles, int ntx, int nty, int ntz, int nrange
   const int t = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
   if (t >= ntiles) return;
   long long total = 0;
-  int32_t own = 0;
   for (int r = lane; r < nranges; r += 64) {
     int32_t s, l;
     tile_range(r, t, ntx, nty, ntz, table, cls_start, &s, &l);
-    if (r == 0) own = l;
     total += l;
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) total += __shfl_down(total, o, 64);
   if (lane != 0) return;
+  const int32_t own = cls_start[t * NCLASS + NCLASS] - cls_start[t * NCLASS];  // the tile's own particles
   int nch = (int)((total + dep_chunk - 1) / dep_chunk);
   if (nch < 1) nch = 1;
   dep_nchunk[t] = nch;
@@ -125,8 +130,8 @@ __global__ void k_deposit_plan(int ntiles, int ntx, int nty, int ntz, int nrange
 // hi = trunc(t / 2^31) is exact (at most 24 significant bits), the remainder
 // t - hi 2^31 is exact in FP32 (|rem| < 2^31, a multiple of t's ulp), and
 // hi 2^31 is even, so hi 2^31 + rint(rem) is t rounded to nearest even.
-__device__ __forceinline__ unsigned long long to_fixed(float x, float scale) {
-  const float t = x * scale;
+// (The deposit passes t already scaled: it folds the scale into the mass.)
+__device__ __forceinline__ unsigned long long to_fixed_scaled(float t) {
   const int hi = (int)(t * 0x1p-31f);
   const float rem = fmaf((float)hi, -0x1p31f, t);
   const int lo = (int)rintf(rem);
@@ -222,7 +227,26 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit(
       axis_weights(SCHEME, p.x - (float)ax, wx);
       axis_weights(SCHEME, p.y - (float)ay, wy);
       axis_weights(SCHEME, p.z - (float)az, wz);
-      const float m = SCHEME == PMGPU_TSC ? p.w / 8.0f : p.w;
+      // the mass times 2^shift: every product below is the unscaled one times
+      // a power of two, so its rounding is the same
+      const float m = (SCHEME == PMGPU_TSC ? p.w / 8.0f : p.w) * scale;
+      if (v < rpre[1]) {
+        // interior class: no stencil cell leaves the tile, so no wrap per cell
+        // and no mask; the cells sit at fixed offsets from the first
+        const int base = (wrapc(ax, nx) - ox + LO) + TILE_X * ((wrapc(ay, ny) - oy + LO) + TILE_Y * (wrapc(az, nz) - oz + LO));
+#pragma unroll
+        for (int i = 0; i < CNT; ++i) {
+          const float t1 = m * wx[i];
+#pragma unroll
+          for (int j = 0; j < CNT; ++j) {
+            const float t2 = t1 * wy[j];
+#pragma unroll
+            for (int k = 0; k < CNT; ++k)
+              atomicAdd(&acc[base + i + TILE_X * (j + TILE_Y * k)], to_fixed_scaled(t2 * wz[k]));
+          }
+        }
+        continue;
+      }
       // local coordinates of the stencil cells relative to the tile, after
       // periodic wrap, and whether each lies in the tile
       int lx[3], ly[3], lz[3];
@@ -248,7 +272,7 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit(
           for (int k = 0; k < CNT; ++k) {
             if (!okz[k]) continue;
             const float t3 = t2 * wz[k];
-            atomicAdd(&acc[lx[i] + TILE_X * (ly[j] + TILE_Y * lz[k])], to_fixed(t3, scale));
+            atomicAdd(&acc[lx[i] + TILE_X * (ly[j] + TILE_Y * lz[k])], to_fixed_scaled(t3));
           }
         }
       }
@@ -325,7 +349,7 @@ std::vector<int4> deposit_range_table(int scheme) {
 
 void deposit(pmgpu_ctx* c) {
   if (c->general) return deposit_general(c);
-  const int nranges = 1 + c->dep_table_len;
+  const int nranges = OWN_RANGES + c->dep_table_len;
   int tok = stage_begin(c, "deposit_plan");
   zero_fill_i32(c, c->counters, 4);
   hipLaunchKernelGGL(k_deposit_plan, dim3((unsigned)(((int64_t)c->ntiles * 64 + 255) / 256)), dim3(256), 0, c->stream,
