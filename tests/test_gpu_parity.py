@@ -253,6 +253,41 @@ def test_p3m_matches_oracle(gpu_lib, oracle, cs, use_table, eps, a):
     assert abs(pm.p3m_count_pairs() - pairs) <= 2 + 1e-5 * pairs
 
 
+@pytest.mark.parametrize("common_mass", [False, True])
+def test_p3m_pair_table_layouts_agree_up_to_the_mass_scale(gpu_lib, oracle, common_mass):
+    # The accurate table over r^2 is read in its absolute layout (positions
+    # scaled by 2^-q, masses by 2^(q+t)) unless the scaled masses could
+    # overflow; then the same bins are read wrapped and unscaled.  Masses
+    # times 2^60 take the second layout.  Every product of the two differs by
+    # a power of two, so the accelerations scale by 2^60 up to the order of
+    # the sums (the cell sort's order within a cell varies from run to run)
+    # and, with the common-mass loop, the gradual underflow of negligible
+    # terms (its unit-mass sums run at about 2^-94).
+    grid = (64, 64, 64)
+    pos, vel, m = _particles(6000, grid, 33, clustered=True)
+    if common_mass:
+        m = np.full_like(m, 1.0 / len(m))
+    out = []
+    for scale in (1.0, 2.0 ** 60):
+        pm = _make(gpu_lib, grid, 2, 0, 2, a=3.0)
+        pm.set_particles(pos, vel, (m * np.float32(scale)).astype(np.float32))
+        pm.init_greens()
+        pm.p3m_configure(cutoff=3.0, particle_diameter=3.0, cloud_shape=1, use_table=2)
+        pm.p3m_record_short_range(True)
+        pm.pm_step()
+        pm.p3m_short_range()
+        out.append((pm.get_particles()[2].astype(np.float64) / scale, pm.p3m_short_range_accelerations().astype(np.float64) / scale))
+        pm.close()
+    (a1, s1), (a2, s2) = out
+    assert _rel(s2, s1) <= 1e-6
+    assert _rel(a2, a1) <= 1e-6
+    # and both against the oracle
+    H = 0.25
+    ref_sr, _ = oracle.p3m_short_range(pos, m, np.array(grid, np.float32) * H, 3.0 * H, H, 3.0, 3.0, 0.0, 1,
+                                       use_table=2)
+    assert _rel(s1, ref_sr) <= 10 * FORCE_TOL
+
+
 @pytest.mark.parametrize("div,chunking", [(1, 2), (3, 2), (4, 1), (5, 2), (6, 1)])
 def test_p3m_cell_division_keeps_the_pair_set(gpu_lib, oracle, div, chunking):
     # finer short-range cells with trimmed rows visit fewer candidates but must
