@@ -179,6 +179,18 @@ def measure(pm, w, steps, warmup, pos, vel, m):
         log(f"[bench] invalid run of {w.name}: {done} of {steps} iterations done, escaped={escaped}")
         sys.exit(3)
     pairs1, tested1 = pm.p3m_count_pairs(tested=True) if w.p3m else (0, 0)
+    # the same number of iterations again with the stage events off: the run
+    # loop then replays two iterations at a time from a captured graph, and
+    # the events' own launches are gone (reported beside `value`, which keeps
+    # the timed region the stage events measured)
+    pm.synchronize()
+    t2 = time.perf_counter()
+    done2 = pm.run(steps, initial=False, p3m=w.p3m, sync=True)
+    t3 = time.perf_counter()
+    unprofiled = None
+    if not pm.escaped() and done2 == steps:
+        unprofiled = {"value": w.n * steps / (t3 - t2), "ms_per_step": (t3 - t2) / steps * 1e3}
+    log(f"[bench] {w.name} without stage events: {unprofiled}")
     pairs = 0.5 * (pairs0 + pairs1)
     tested = 0.5 * (tested0 + tested1)
     elapsed = t1 - t0
@@ -213,7 +225,7 @@ def measure(pm, w, steps, warmup, pos, vel, m):
             hbm[k] = {"ms": s * 1e3, "GB/s": b / s / 1e9, "frac": b / s / 1e9 / PEAK_HBM_GBS}
     return {"value": value, "ms_per_step": elapsed / steps * 1e3, "steps": steps, "iterations": done,
             "escaped": escaped, "roofline": roofline, "stage_ms_per_step": per_step, "hbm_stages": hbm,
-            "pairs": pairs}
+            "pairs": pairs, "unprofiled": unprofiled}
 
 
 def cpu_baseline(w, pos, m, sr_scale_from=None):
@@ -387,6 +399,7 @@ def main():
         "learned_vs_plain": learned,
         "stage_ms_per_step": res["stage_ms_per_step"],
         "hbm_stages": res["hbm_stages"],
+        "without_stage_events": res["unprofiled"],
         "gpu_vs_cpu": (res["value"] / cpu["value"]) if cpu else None,
         "iterations": res["iterations"],
         "escaped": res["escaped"],

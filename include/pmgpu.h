@@ -192,7 +192,9 @@ int pmgpu_get_integer_step_velocities(pmgpu_ctx* ctx, float* vel);
  * Kick and the next drift are fused on the device.  When a particle leaves
  * the box the remaining work is skipped on the device (the reference breaks
  * out of its loop); *iterations_done (may be NULL, then no sync) receives
- * the count of completed iterations.
+ * the count of completed iterations.  With profiling off, iterations after
+ * the first are replayed two at a time from a HIP graph captured on the
+ * context's stream (kept until a call changes the step's arguments).
  */
 int pmgpu_run(pmgpu_ctx* ctx, int32_t iterations, int32_t initial, int32_t p3m,
               int32_t* iterations_done);

@@ -279,6 +279,19 @@ static void reset_escape(pmgpu_ctx* c) {
   HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
+// Retires the captured run-loop graphs (their kernel arguments may change).
+static void graphs_stale(pmgpu_ctx* c) { ++c->graph_gen; }
+
+static void graphs_free(pmgpu_ctx* c) {
+  for (auto& row : c->run_exec)
+    for (auto& e : row)
+      if (e) {
+        (void)hipGraphExecDestroy(e);
+        e = nullptr;
+      }
+  c->run_exec_gen[0] = c->run_exec_gen[1] = -1;
+}
+
 static void require_particles(pmgpu_ctx* c) {
   if (c->n <= 0) fail(PMGPU_ERR_NOT_READY, "no particles: call pmgpu_set_particles first");
 }
@@ -352,6 +365,53 @@ static thread_local std::string g_create_error;
   }                               \
   return PMGPU_OK;
 
+// escape[1]: the iteration of kernels replayed from a graph (ITER_DEVICE)
+__global__ void k_iter_set(int32_t* __restrict__ escape, int v) { escape[1] = v; }
+__global__ void k_iter_advance(int32_t* __restrict__ escape) { escape[1] += 1; }
+
+// The graph of two run-loop iterations that both drift into the next, each
+// followed by the advance of escape[1], for the particle buffers as they are
+// now (c->cur); built for both buffer parities at once (a capture enqueues
+// nothing, and two iterations leave the parity as it was) and kept until
+// graph_gen moves.  Returns null when the capture or instantiation fails:
+// the caller then enqueues the iterations directly.
+template <typename Force>
+static hipGraphExec_t run_graph(pmgpu_ctx* c, bool p3m, Force& force) {
+  const int f = p3m ? 1 : 0;
+  if (c->run_exec_gen[f] == c->graph_gen && c->run_exec[f][c->cur]) return c->run_exec[f][c->cur];
+  for (auto& e : c->run_exec[f])
+    if (e) {
+      (void)hipGraphExecDestroy(e);
+      e = nullptr;
+    }
+  c->run_exec_gen[f] = c->graph_gen;
+  const int cur0 = c->cur, iter0 = c->iter;
+  for (int parity = 0; parity < 2; ++parity) {
+    c->cur = parity;
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed) != hipSuccess) break;
+    bool ok = true;
+    try {
+      c->iter = ITER_DEVICE;
+      for (int k = 0; k < 2; ++k) {
+        force(MODE_KICK | MODE_DRIFT, 1.0f);
+        hipLaunchKernelGGL(k_iter_advance, dim3(1), dim3(1), 0, c->stream, c->escape);
+      }
+    } catch (...) {
+      ok = false;
+    }
+    const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+    if (ok && ec == hipSuccess && graph)
+      ok = hipGraphInstantiate(&c->run_exec[f][parity], graph, nullptr, nullptr, 0) == hipSuccess;
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    if (!ok) c->run_exec[f][parity] = nullptr;
+  }
+  c->cur = cur0;
+  c->iter = iter0;
+  return c->run_exec[f][c->cur];
+}
+
 extern "C" {
 
 int pmgpu_create(const pmgpu_pm_params* params, pmgpu_ctx** out) {
@@ -406,7 +466,7 @@ int pmgpu_create(const pmgpu_pm_params* params, pmgpu_ctx** out) {
     dalloc(&c->int_nchunk, c->ntiles + 1);
     dalloc(&c->int_chunk_start, c->ntiles + 1);
     dalloc(&c->counters, 32);  // [16, 24): per-XCD work counters of the short-range pass
-    dalloc(&c->escape, 1);
+    dalloc(&c->escape, 2);
     ensure_scan_scratch(c, nkeys);
     std::vector<int4> tab = deposit_range_table(c->prm.interpolation);
     c->dep_table_len = (int)tab.size();
@@ -430,6 +490,7 @@ int pmgpu_create(const pmgpu_pm_params* params, pmgpu_ctx** out) {
 void pmgpu_destroy(pmgpu_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  graphs_free(c);
   for (int b = 0; b < 2; ++b) {
     dfree(&c->pm[b]);
     dfree(&c->vel[b]);
@@ -475,6 +536,7 @@ int64_t pmgpu_num_particles(const pmgpu_ctx* c) { return c ? c->n : -1; }
 int pmgpu_set_particles(pmgpu_ctx* c, int64_t n, const float* pos, const float* vel,
                         const float* mass) {
   PMGPU_API_BEGIN(c)
+  graphs_stale(c);
   if (n < 0 || (n > 0 && (!pos || !mass)))
     fail(PMGPU_ERR_INVALID_ARGUMENT, "pos and mass are required");
   if (n > (int64_t)INT_MAX / 2) fail(PMGPU_ERR_UNSUPPORTED, "too many particles");
@@ -557,6 +619,7 @@ int pmgpu_get_particles(pmgpu_ctx* c, float* pos, float* vel, float* acc) {
 
 int pmgpu_update_particles(pmgpu_ctx* c, const float* pos, const float* vel, const float* acc) {
   PMGPU_API_BEGIN(c)
+  graphs_stale(c);
   const int64_t n = c->n;
   if (n == 0) return PMGPU_OK;
   std::vector<float4> hp(n), hv(n), ha(n);
@@ -590,6 +653,7 @@ int pmgpu_update_particles(pmgpu_ctx* c, const float* pos, const float* vel, con
 
 int pmgpu_init_greens(pmgpu_ctx* c) {
   PMGPU_API_BEGIN(c)
+  graphs_stale(c);
   greens_compute(c, nullptr);
   if (!c->mlp_weights.empty()) mlp_apply_to_greens(c);  // G_sym * (1 + y(k))
   if (!c->general) {
@@ -612,6 +676,7 @@ int pmgpu_pm_step(pmgpu_ctx* c) {
 
 int pmgpu_p3m_configure(pmgpu_ctx* c, const pmgpu_p3m_params* p) {
   PMGPU_API_BEGIN(c)
+  graphs_stale(c);
   if (!p) fail(PMGPU_ERR_INVALID_ARGUMENT, "params is NULL");
   if (!(p->cutoff > 0.0f)) fail(PMGPU_ERR_INVALID_ARGUMENT, "cutoff must be > 0");
   if (p->cloud_shape != PMGPU_S1 && p->cloud_shape != PMGPU_S2)
@@ -690,6 +755,7 @@ int pmgpu_leapfrog(pmgpu_ctx* c, int32_t op, float dt) {
 
 int pmgpu_p3m_record_short_range(pmgpu_ctx* c, int32_t on) {
   PMGPU_API_BEGIN(c)
+  graphs_stale(c);
   if (on && !c->acc_sr) dalloc(&c->acc_sr, std::max<int64_t>(c->n, 1));
   if (!on) dfree(&c->acc_sr);
   PMGPU_API_END(c)
@@ -718,6 +784,7 @@ int pmgpu_p3m_get_short_range(pmgpu_ctx* c, float* acc) {
 
 int pmgpu_add_accelerations(pmgpu_ctx* c, const float* acc) {
   PMGPU_API_BEGIN(c)
+  graphs_stale(c);
   require_particles(c);
   if (!acc) fail(PMGPU_ERR_INVALID_ARGUMENT, "acc is NULL");
   if (!c->ext_stage) dalloc(&c->ext_stage, 3 * c->n);
@@ -770,7 +837,24 @@ int pmgpu_run(pmgpu_ctx* c, int32_t iterations, int32_t initial, int32_t p3m,
   } else if (iterations > 0) {
     leapfrog_op(c, PMGPU_DRIFT, 1.0f, base);
   }
-  for (int it = 0; it < iterations; ++it) {
+  int it = 0;
+  // Iterations that drift into a following one, two at a time, from a
+  // captured graph (the per-step launches of a small problem cost more than
+  // its kernels); the first is enqueued directly, which also makes any lazy
+  // allocation before a capture.  Not while profiling (its events are host
+  // state).
+  if (!c->profiling && iterations >= 2) {
+    c->iter = base;
+    force(MODE_KICK | MODE_DRIFT, 1.0f);
+    it = 1;
+    hipGraphExec_t exec = run_graph(c, p3m != 0, force);
+    if (exec && it + 2 < iterations) {
+      hipLaunchKernelGGL(k_iter_set, dim3(1), dim3(1), 0, c->stream, c->escape, base + it);
+      HIP_CHECK(hipGetLastError());
+      for (; it + 2 < iterations; it += 2) HIP_CHECK(hipGraphLaunch(exec, c->stream));
+    }
+  }
+  for (; it < iterations; ++it) {
     c->iter = base + it;
     force(MODE_KICK | (it + 1 < iterations ? MODE_DRIFT : 0), 1.0f);
   }
@@ -833,6 +917,7 @@ int pmgpu_get_grid(pmgpu_ctx* c, int32_t which, float* out) {
 
 int pmgpu_set_tuning(pmgpu_ctx* c, const int32_t* knobs) {
   PMGPU_API_BEGIN(c)
+  graphs_stale(c);
   if (!knobs) fail(PMGPU_ERR_INVALID_ARGUMENT, "knobs is NULL");
   const int dep = knobs[0], itp = knobs[1], bpc = knobs[2], div = knobs[3], ik = knobs[4], sc = knobs[5];
   if (dep < DEPOSIT_CHUNK_MIN || dep > DEPOSIT_CHUNK_MAX)
@@ -869,6 +954,7 @@ int pmgpu_get_tuning(pmgpu_ctx* c, int32_t* knobs) {
 
 int pmgpu_profile_enable(pmgpu_ctx* c, int32_t on) {
   PMGPU_API_BEGIN(c)
+  graphs_stale(c);
   c->profiling = on != 0;
   PMGPU_API_END(c)
 }

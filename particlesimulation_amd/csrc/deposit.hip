@@ -96,6 +96,7 @@ __global__ void k_deposit_plan(int ntiles, int ntx, int nty, int ntz, int nrange
                                int32_t* __restrict__ slot, int32_t* __restrict__ slot_tile,
                                int32_t* __restrict__ counters, int64_t max_slots, int dep_chunk,
                                int int_chunk, const int32_t* __restrict__ escape, int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   // one wave per tile, its lanes over the ranges
   const int t = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
@@ -146,6 +147,7 @@ __global__ __launch_bounds__(DEP_THREADS) void k_deposit(
     const int32_t* __restrict__ owner, const int32_t* __restrict__ slot, long long* __restrict__ fx_accum,
     int shift, float* __restrict__ rho, int dep_chunk, int centred, const int32_t* __restrict__ escape,
     int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   __shared__ unsigned long long acc[TILE_CELLS];
   __shared__ int32_t rstart[MAX_RANGES], rpre[MAX_RANGES + 1];
@@ -302,6 +304,7 @@ __global__ void k_deposit_finalize(int nx, int ny, int ntx, int nty, long long* 
                                    const int32_t* __restrict__ counters, int64_t max_slots,
                                    int shift, float* __restrict__ rho,
                                    const int32_t* __restrict__ escape, int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   const int nslots = (int)min((int64_t)counters[CNT_SLOTS], max_slots);
   for (int s = blockIdx.x; s < nslots; s += gridDim.x) {

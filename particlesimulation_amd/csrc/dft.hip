@@ -59,6 +59,7 @@ __device__ __forceinline__ unsigned long long fixed_g(float x, float scale) {
 
 __global__ void k_fill_u64(unsigned long long* __restrict__ p, int64_t n, const int32_t* __restrict__ escape,
                            int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
@@ -69,6 +70,7 @@ __global__ __launch_bounds__(256) void k_deposit_global(const float4* __restrict
                                                         int nz, int centred, int shift,
                                                         unsigned long long* __restrict__ fx,
                                                         const int32_t* __restrict__ escape, int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -103,6 +105,7 @@ __global__ __launch_bounds__(256) void k_deposit_global(const float4* __restrict
 
 __global__ void k_fixed_to_rho(const unsigned long long* __restrict__ fx, int64_t m, int shift,
                                float* __restrict__ rho, const int32_t* __restrict__ escape, int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m) rho[i] = ldexpf((float)(long long)fx[i], -shift);
@@ -113,6 +116,7 @@ __global__ void k_fixed_to_rho(const unsigned long long* __restrict__ fx, int64_
 __global__ __launch_bounds__(DFT_THREADS) void k_dft_x_r2c(const float* __restrict__ rho, float2* __restrict__ spec,
                                                            int nx, int sp, const float2* __restrict__ tw,
                                                            const int32_t* __restrict__ escape, int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   __shared__ float row[1024];
   __shared__ float2 w[1024];
@@ -142,6 +146,7 @@ __global__ __launch_bounds__(DFT_THREADS) void k_dft_x_r2c(const float* __restri
 __global__ __launch_bounds__(DFT_THREADS) void k_dft_x_c2r(const float2* __restrict__ spec, float* __restrict__ phi,
                                                            int nx, int sp, const float2* __restrict__ tw,
                                                            const int32_t* __restrict__ escape, int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   __shared__ float2 X[513];
   __shared__ float2 w[1024];
@@ -178,6 +183,7 @@ __global__ __launch_bounds__(DFT_THREADS) void k_dft_lines(float2* __restrict__ 
                                                            const float* __restrict__ g,
                                                            const float2* __restrict__ tw,
                                                            const int32_t* __restrict__ escape, int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   __shared__ float2 a[1024];
   __shared__ float2 b[1024];

@@ -159,6 +159,7 @@ __global__ __launch_bounds__(256) void k_fft_x_r2c(const float* __restrict__ rho
                                                    const float2* __restrict__ tw_r2c,
                                                    const int32_t* __restrict__ escape,
                                                    int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   constexpr int T = L / 8, ROWS = 256 / T, P = L + 2;
   __shared__ float2 buf[ROWS * P];
@@ -186,6 +187,7 @@ __global__ __launch_bounds__(256) void k_fft_x_c2r(const float2* __restrict__ sp
                                                    const float2* __restrict__ tw_r2c,
                                                    const int32_t* __restrict__ escape,
                                                    int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   constexpr int T = L / 8, ROWS = 256 / T, P = L + 2;
   __shared__ float2 buf[ROWS * P];
@@ -251,6 +253,7 @@ __global__ __launch_bounds__(TW * (L / 8)) void k_fft_y(float2* __restrict__ spe
                                                                const float2* __restrict__ tw,
                                                                const int32_t* __restrict__ escape,
                                                                int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   using CT = ColTile<L, TW>;
   __shared__ float2 buf[L * CT::NC];
@@ -304,6 +307,7 @@ __global__ __launch_bounds__(TW * (L / 8)) void k_fft_z(
     const float2* __restrict__ src, float2* __restrict__ dst, const float* __restrict__ greens, int sp,
     int ny, int ntile, int pair, int gnxh, const float2* __restrict__ tw, const int32_t* __restrict__ escape,
     int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   using CT = ColTile<L, TW>;
   __shared__ float2 buf[L * CT::NC];

@@ -148,6 +148,7 @@ __global__ void k_greens(int gf, int is, int fds, double a, int nx, int ny, int 
 
 void greens_compute(pmgpu_ctx* c, const float* corr_dev) {
   c->greens_z_valid = false;  // the z pass's copy is rebuilt by pmgpu_init_greens
+  ++c->graph_gen;            // captured run-loop graphs may read that copy
   int tok = stage_begin(c, "greens_fp64");
   const unsigned g = (unsigned)((c->K + 127) / 128);
   hipLaunchKernelGGL(k_greens, dim3(g), dim3(128), 0, c->stream, c->prm.greens_function,

@@ -101,6 +101,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int32_t* __restrict__ owner, float4* __restrict__ pm, float4* __restrict__ vel, float4* __restrict__ acc, int mode,
     float kick, float3 box, ExtField ext, int int_chunk, int centred, int32_t* __restrict__ escape,
     int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   using Gm = Geo<SCHEME, FDS>;
   // The potential and the field share one LDS block: each thread forms its
@@ -251,6 +252,7 @@ __global__ void k_field(const float* __restrict__ phi, int nx, int ny, int nz, i
 template <int FDS>
 __global__ void k_field4(const float* __restrict__ phi, int nx, int ny, int nz, float4* __restrict__ out,
                          const int32_t* __restrict__ escape, int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t M = (int64_t)nx * ny * nz;
@@ -283,6 +285,7 @@ __global__ __launch_bounds__(256) void k_interp_gather(
     const float4* __restrict__ E, int nx, int ny, int nz, int64_t n, float4* __restrict__ pm,
     float4* __restrict__ vel, float4* __restrict__ acc, int mode, float kick, float3 box, ExtField ext,
     int centred, int32_t* __restrict__ escape, int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -350,6 +353,7 @@ __global__ __launch_bounds__(256) void k_interp_phi(
     const float* __restrict__ phi, int nx, int ny, int nz, int64_t n, float4* __restrict__ pm,
     float4* __restrict__ vel, float4* __restrict__ acc, int mode, float kick, float3 box, ExtField ext,
     int centred, int32_t* __restrict__ escape, int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -440,6 +444,7 @@ __global__ void k_leapfrog(float4* __restrict__ pm, float4* __restrict__ vel,
                            const float4* __restrict__ acc, float4* __restrict__ vint, int64_t n,
                            int op, float dt, float3 box, int32_t* __restrict__ escape,
                            int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;

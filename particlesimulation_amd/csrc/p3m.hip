@@ -59,6 +59,7 @@ __global__ void k_sr_keys(const float4* __restrict__ pm, int64_t n, float ihx, f
                           int mx, int my, int mz, int32_t* __restrict__ key,
                           int32_t* __restrict__ rank, int32_t* __restrict__ count,
                           const int32_t* __restrict__ escape, int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = i < n;
@@ -83,6 +84,7 @@ __global__ void k_sr_scatter(const float4* __restrict__ pm, const float4* __rest
                              float mscale, float m0, int nodd,
                              const int32_t* __restrict__ odd_ids, int32_t* __restrict__ odd_pos,
                              const int32_t* __restrict__ escape, int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -110,6 +112,7 @@ __global__ void k_sr_scatter(const float4* __restrict__ pm, const float4* __rest
 // order, so sparse cells share a chunk instead of idling lanes).
 __global__ void k_sr_plan(const int32_t* __restrict__ start, int stride, int nunit, int32_t* __restrict__ nchunk,
                           const int32_t* __restrict__ escape, int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nunit) return;
@@ -238,6 +241,7 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
     float4* __restrict__ vel, float4* __restrict__ acc, int mode, float kick, float3 box,
     float m0, int nodd, const int32_t* __restrict__ odd_pos,
     int32_t* __restrict__ counters, float4* __restrict__ acc_sr, int32_t* __restrict__ escape, int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   // The partial sums of half of the waves at a time.
   constexpr int HALF = SR_WAVES / 2;

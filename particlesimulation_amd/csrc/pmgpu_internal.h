@@ -12,6 +12,18 @@
 namespace pmgpu {
 
 // ---------------------------------------------------------------------------
+// Run-loop iterations.  Every kernel of a force step takes its iteration and
+// returns at once when escape[0] (the first iteration whose drift left the
+// box) is not after it, so a whole run is enqueued without host round trips
+// and stops where the reference stops.  Kernels replayed from a captured
+// graph take ITER_DEVICE instead and read their iteration from escape[1],
+// which the graph advances after each iteration it holds.
+constexpr int ITER_DEVICE = -2147483647;  // INT_MIN + 1 (INT_MIN: untagged, never gated)
+__device__ __forceinline__ int gate_iteration(const int32_t* escape, int iteration) {
+  return iteration == ITER_DEVICE ? escape[1] : iteration;
+}
+
+// ---------------------------------------------------------------------------
 // Mesh tiling used by the deposit and the interpolation.  A tile is
 // TILE_X x TILE_Y x TILE_Z cells; particles are sorted by (tile, boundary
 // class), the class recording per axis whether the anchor cell sits on the
@@ -207,9 +219,16 @@ struct pmgpu_ctx {
   double* diag_scratch = nullptr;       // FP64 diagnostics partial sums
 
   // run loop state
-  int32_t* escape = nullptr;   // [0] = first iteration with an escaped particle, or INT_MAX
+  int32_t* escape = nullptr;   // [0] = first iteration with an escaped particle, or INT_MAX;
+                               // [1] = the iteration of graph-replayed kernels (ITER_DEVICE)
   int iter = 0;                // tag of the kernels being enqueued (INT_MIN: standalone call)
   int run_iter = 0;            // next run-loop iteration number
+  // two run-loop iterations captured as one graph, per P3M flag and per
+  // particle-buffer parity at the graph's start (pmgpu_run); any call that
+  // can change a kernel argument bumps graph_gen, which retires them
+  hipGraphExec_t run_exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  int64_t run_exec_gen[2] = {-1, -1};
+  int64_t graph_gen = 0;
 
   // profiling
   bool profiling = false;

@@ -148,6 +148,7 @@ __global__ void k_tile_keys(const float4* __restrict__ pm, int64_t n, int nx, in
                             int ntx, int nty, int scheme, int centred, int32_t* __restrict__ key,
                             int32_t* __restrict__ rank, int32_t* __restrict__ count,
                             const int32_t* __restrict__ escape, int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = i < n;
@@ -175,6 +176,7 @@ __global__ void k_tile_scatter(const float4* __restrict__ pm_in, const float4* _
   if (i >= n) return;
   // After an escape the keys were not computed: copy in place so that the
   // buffer swap on the host keeps the state.
+  iteration = gate_iteration(escape, iteration);
   int64_t d = escape[0] <= iteration ? i : (int64_t)start[key[i]] + rank[i];
   pm_out[d] = pm_in[i];
   vel_out[d] = vel_in[i];
@@ -231,6 +233,7 @@ namespace {
 // owner[w] = t for every work item w in [starts[t], starts[t+1])
 __global__ void k_expand_owner(const int32_t* __restrict__ starts, int64_t n, int32_t* __restrict__ owner,
                                const int32_t* __restrict__ escape, int iteration) {
+  iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;

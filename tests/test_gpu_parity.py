@@ -475,6 +475,40 @@ def test_run_matches_oracle_leapfrog(gpu_lib, oracle, p3m):
     assert _rel(a, ra) < 1e-2
 
 
+@pytest.mark.parametrize("p3m", [False, True])
+def test_graph_replayed_run_matches_the_direct_loop(gpu_lib, p3m):
+    """With the stage events off the run loop replays two iterations at a time
+    from a captured graph whose kernels read the iteration number from device
+    memory; with them on it launches every kernel itself.  Two calls in a row
+    (7 iterations, then 5 more) start the graph on both particle buffers and
+    run an odd tail outside it."""
+    grid = (64, 64, 64)
+    pos, vel, m = _particles(4000, grid, 51, margin=8.0, clustered=True)
+    m = m * 20
+    out = []
+    for profiled in (True, False):
+        pm = _make(gpu_lib, grid, 2, 0, 2, a=3.0)
+        pm.set_particles(pos, vel, m)
+        pm.init_greens()
+        if p3m:
+            pm.p3m_configure(cutoff=3.0, particle_diameter=3.0, cloud_shape=1)
+        pm.profile(profiled)
+        assert pm.run(7, initial=True, p3m=p3m) == 7
+        assert pm.run(5, initial=False, p3m=p3m) == 5
+        assert not pm.escaped()
+        out.append(pm.get_particles())
+        pm.close()
+    (x0, v0, a0), (x1, v1, a1) = out
+    if p3m:
+        # the short-range sums run over cells filled by atomics, so their
+        # order (and the last bits) differ from run to run
+        assert _rel(x1, x0) < 1e-6 and _rel(v1, v0) < 1e-4 and _rel(a1, a0) < 1e-3
+    else:
+        # fixed-point deposit, fixed FFT and per-particle interpolation: the
+        # two loops compute the same bits
+        assert np.array_equal(x1, x0) and np.array_equal(v1, v0) and np.array_equal(a1, a0)
+
+
 def test_leapfrog_ops_match_reference_formulas(gpu_lib):
     grid = (32, 32, 32)
     pos, vel, m = _particles(1000, grid, 60)
@@ -496,21 +530,24 @@ def test_leapfrog_ops_match_reference_formulas(gpu_lib):
     assert np.allclose(x2, pos + 0.5 * v2, atol=1e-6)
 
 
-def test_escape_stops_the_run(gpu_lib):
+# the run loop replays iterations 1-2, 3-4, ... from one captured graph: the
+# escape falls in the first or the second iteration of one replay
+@pytest.mark.parametrize("x0,escape_at", [(30.5, 3), (30.1, 4)])
+def test_escape_stops_the_run(gpu_lib, x0, escape_at):
     grid = (32, 32, 32)
     pos, vel, m = _particles(500, grid, 70, margin=8.0)
     vel[:] = 0
-    pos[7] = [30.5, 16.0, 16.0]
-    vel[7] = [0.4, 0.0, 0.0]  # crosses x = 32 during the third drift
+    pos[7] = [x0, 16.0, 16.0]
+    vel[7] = [0.4, 0.0, 0.0]  # crosses x = 32 during the drift of iteration escape_at
     m[:] = 0.0
     pm = _make(gpu_lib, grid, 1, 0, 0, box=grid)
     pm.set_particles(pos, vel, m)
     pm.init_greens()
     assert not pm.escaped()
     done = pm.run(10, initial=True)
-    assert done == 3  # iterations 0, 1, 2 ... the drift of iteration 3 leaves the box
+    assert done == escape_at  # iterations before it are whole; its drift leaves the box
     x, v, _ = pm.get_particles()
-    assert np.isclose(x[7, 0], 30.5 + 4 * 0.4, atol=1e-5)
+    assert np.isclose(x[7, 0], x0 + (escape_at + 1) * 0.4, atol=1e-5)
     assert pm.escaped()
 
 
