@@ -268,8 +268,9 @@ int pmgpu_batch_get_tuning(pmgpu_batch* batch, int32_t sim, int32_t* knobs);
 /* ---- launch tuning ------------------------------------------------------ */
 /* No counterpart in the reference.  knobs[0] particles per deposit work item
    (1024..16384, default 1024), knobs[1] particles per interpolation work item
-   (256..16384, default 2048), knobs[2] workgroups per CU of the persistent
-   deposit and interpolation kernels (1..32, default 16), knobs[3] the
+   (256..16384, default 2048), knobs[2] workgroups per CU of the
+   interpolation kernel (1..32, default 32; the deposit launches four times
+   as many), knobs[3] the
    short-range cell division: cells of y and z side >= cutoff / knobs[3]
    (x cells up to 4 times finer while the grid keeps within 128 cells per
    particle) and a

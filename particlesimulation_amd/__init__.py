@@ -375,7 +375,7 @@ class PM:
         self._check(self._lib.pmgpu_set_greens_correction(self._ctx, _fptr(w)))
 
     # -- launch tuning ----------------------------------------------------
-    def set_tuning(self, deposit_chunk=1024, interp_chunk=2048, blocks_per_cu=16, sr_div=2, interp_kernel=0,
+    def set_tuning(self, deposit_chunk=1024, interp_chunk=2048, blocks_per_cu=32, sr_div=2, interp_kernel=0,
                    sr_chunking=0):
         k = (C.c_int32 * 6)(int(deposit_chunk), int(interp_chunk), int(blocks_per_cu), int(sr_div),
                             int(interp_kernel), int(sr_chunking))

@@ -40,7 +40,7 @@ constexpr int NCLASS = 27;
 // and limits of the tunable values (pmgpu_set_tuning)
 constexpr int DEPOSIT_CHUNK = 1024, DEPOSIT_CHUNK_MIN = 1024, DEPOSIT_CHUNK_MAX = 16384;
 constexpr int INTERP_CHUNK = 2048, INTERP_CHUNK_MIN = 256, INTERP_CHUNK_MAX = 16384;
-constexpr int BLOCKS_PER_CU = 16, BLOCKS_PER_CU_MAX = 32;
+constexpr int BLOCKS_PER_CU = 32, BLOCKS_PER_CU_MAX = 32;
 // automatic interpolation variant: the gather variant for grids of fewer
 // tiles than this many per CU (too few work items to fill the device), the
 // tiled variant otherwise
