@@ -182,7 +182,11 @@ def measure(pm, w, steps, warmup, pos, vel, m):
     # the same number of iterations again with the stage events off: the run
     # loop then replays two iterations at a time from a captured graph, and
     # the events' own launches are gone (reported beside `value`, which keeps
-    # the timed region the stage events measured)
+    # the timed region the stage events measured); four untimed iterations
+    # first capture the graph that the timed run replays (turning the events
+    # off retired the graphs; an even count ends on the buffer the capture
+    # did not start from, which is the one the timed run's graph starts from)
+    pm.run(4, initial=False, p3m=w.p3m, sync=True)
     pm.synchronize()
     t2 = time.perf_counter()
     done2 = pm.run(steps, initial=False, p3m=w.p3m, sync=True)

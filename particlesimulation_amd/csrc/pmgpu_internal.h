@@ -333,17 +333,11 @@ __device__ __forceinline__ XcdRange xcd_range(int total) {
   return {b + (int)(blockIdx.x / NXCD), e, nb};
 }
 
-// Rank of this lane's key among all lanes of the grid that share it: one
-// integer atomic per distinct key per wavefront (lanes with equal keys are
-// ranked by lane order), so densely populated keys do not serialise 64
-// atomics on one address.
-__device__ __forceinline__ int32_t wave_rank(int32_t* count, int32_t key, bool active) {
-  const int lane = threadIdx.x & 63;
-  // group the lanes by key (one ballot per distinct key), then one atomic
-  // instruction for all groups at once: the leaders' atomics overlap
-  // instead of waiting for each other's return
+// The lanes of this wavefront whose key equals this lane's (0 for an
+// inactive lane): one ballot per distinct key.
+__device__ __forceinline__ unsigned long long wave_group(int32_t key, bool active) {
   unsigned long long pending = __ballot(active);
-  unsigned long long mine = 0;  // the lanes that share this lane's key
+  unsigned long long mine = 0;
   while (pending) {
     const int leader = __ffsll((unsigned long long)pending) - 1;
     const int32_t lk = __builtin_amdgcn_readlane(key, leader);
@@ -351,6 +345,19 @@ __device__ __forceinline__ int32_t wave_rank(int32_t* count, int32_t key, bool a
     if (active && key == lk) mine = same;
     pending &= ~same;
   }
+  return mine;
+}
+
+// Rank of this lane's key among all lanes of the grid that share it: one
+// integer atomic per distinct key per wavefront (lanes with equal keys are
+// ranked by lane order), so densely populated keys do not serialise 64
+// atomics on one address.
+__device__ __forceinline__ int32_t wave_rank(int32_t* count, int32_t key, bool active) {
+  const int lane = threadIdx.x & 63;
+  // group the lanes by key, then one atomic instruction for all groups at
+  // once: the leaders' atomics overlap instead of waiting for each other's
+  // return
+  const unsigned long long mine = wave_group(key, active);
   const int leader = active ? __ffsll((unsigned long long)mine) - 1 : lane;
   int32_t base = 0;
   if (active && lane == leader) base = atomicAdd(&count[key], (int32_t)__popcll(mine));

@@ -144,26 +144,58 @@ __device__ __forceinline__ int face_class(int local, int extent) {
   return local == 0 ? 0 : (local == extent - 1 ? 2 : 1);
 }
 
-__global__ void k_tile_keys(const float4* __restrict__ pm, int64_t n, int nx, int ny, int nz,
-                            int ntx, int nty, int scheme, int centred, int32_t* __restrict__ key,
-                            int32_t* __restrict__ rank, int32_t* __restrict__ count,
-                            const int32_t* __restrict__ escape, int iteration) {
+// K particles per thread (a workgroup takes 256 K consecutive ones, its
+// threads striding by 256): their loads are in flight together, and so are
+// their rank atomics, instead of one load and one returning atomic after the
+// other per thread (16.7M particles: 0.322 ms for the whole sort at K = 4
+// against 0.347 at K = 1).  Small particle sets keep K = 1, which fills more
+// of the GPU.
+template <int K>
+__global__ __launch_bounds__(256) void k_tile_keys(const float4* __restrict__ pm, int64_t n, int nx, int ny,
+                                                   int nz, int ntx, int nty, int scheme, int centred,
+                                                   int32_t* __restrict__ key, int32_t* __restrict__ rank,
+                                                   int32_t* __restrict__ count, const int32_t* __restrict__ escape,
+                                                   int iteration) {
   iteration = gate_iteration(escape, iteration);
   if (escape[0] <= iteration) return;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = i < n;
-  float4 p = active ? pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-  int ax = wrap_cell(anchor_of(p.x, scheme, centred), nx);
-  int ay = wrap_cell(anchor_of(p.y, scheme, centred), ny);
-  int az = wrap_cell(anchor_of(p.z, scheme, centred), nz);
-  int tx = ax / TILE_X, ty = ay / TILE_Y, tz = az / TILE_Z;
-  int cls = face_class(ax - tx * TILE_X, TILE_X) + 3 * face_class(ay - ty * TILE_Y, TILE_Y) +
-            9 * face_class(az - tz * TILE_Z, TILE_Z);
-  int k = (tx + ntx * (ty + nty * tz)) * NCLASS + cls;
-  const int32_t r = wave_rank(count, k, active);
-  if (active) {
-    key[i] = k;
-    rank[i] = r;
+  const int64_t i0 = (int64_t)blockIdx.x * (256 * K) + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  float4 p[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    const int64_t i = i0 + 256 * q;
+    p[q] = i < n ? pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  int32_t k[K];
+  unsigned long long mine[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    const bool active = i0 + 256 * q < n;
+    int ax = wrap_cell(anchor_of(p[q].x, scheme, centred), nx);
+    int ay = wrap_cell(anchor_of(p[q].y, scheme, centred), ny);
+    int az = wrap_cell(anchor_of(p[q].z, scheme, centred), nz);
+    int tx = ax / TILE_X, ty = ay / TILE_Y, tz = az / TILE_Z;
+    int cls = face_class(ax - tx * TILE_X, TILE_X) + 3 * face_class(ay - ty * TILE_Y, TILE_Y) +
+              9 * face_class(az - tz * TILE_Z, TILE_Z);
+    k[q] = (tx + ntx * (ty + nty * tz)) * NCLASS + cls;
+    mine[q] = wave_group(k[q], active);
+  }
+  // the group leaders' atomics of all K particles, then their results
+  int32_t base[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    base[q] = 0;
+    if (mine[q] && lane == __ffsll(mine[q]) - 1) base[q] = atomicAdd(&count[k[q]], (int32_t)__popcll(mine[q]));
+  }
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    const int64_t i = i0 + 256 * q;
+    const int leader = mine[q] ? __ffsll(mine[q]) - 1 : lane;
+    const int32_t b = __shfl(base[q], leader);
+    if (i < n) {
+      key[i] = k[q];
+      rank[i] = b + (int32_t)__popcll(mine[q] & ((1ull << lane) - 1ull));
+    }
   }
 }
 
@@ -253,9 +285,16 @@ void tile_sort(pmgpu_ctx* c) {
   int tok = stage_begin(c, "tile_sort");
   zero_fill_i32(c, c->cls_count, nkeys);
   unsigned g = (unsigned)((c->n + 255) / 256);
-  hipLaunchKernelGGL(k_tile_keys, dim3(g), dim3(256), 0, c->stream, c->pm[c->cur], c->n, c->nx,
-                     c->ny, c->nz, c->ntx, c->nty, c->prm.interpolation, c->tsc_centred, c->key, c->rank,
-                     c->cls_count, c->escape, c->iter);
+  // four keys per thread once that still gives every CU four workgroups
+  const bool four = c->n >= (int64_t)1024 * c->num_cu * 4;
+  const unsigned gk = four ? (unsigned)((c->n + 1023) / 1024) : g;
+  auto keys = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(gk), dim3(256), 0, c->stream, c->pm[c->cur], c->n, c->nx, c->ny, c->nz, c->ntx,
+                       c->nty, c->prm.interpolation, c->tsc_centred, c->key, c->rank, c->cls_count, c->escape,
+                       c->iter);
+  };
+  if (four) keys(k_tile_keys<4>);
+  else keys(k_tile_keys<1>);
   HIP_CHECK(hipGetLastError());
   exclusive_scan_i32(c, c->cls_count, c->cls_start, nkeys);
   int nxt = c->cur ^ 1;
