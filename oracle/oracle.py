@@ -1,7 +1,7 @@
 """ctypes front end of the CPU oracle (oracle/pm_oracle.c).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
-checker, and by bench.py for the cpu_baseline leg.  The product library
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to check
+the GPU result, and by bench.py for the cpu_baseline leg.  The product library
 (particlesimulation_amd) never imports this module.
 
 All arrays are numpy; positions are (n, 3) float32 in code units (grid cells),

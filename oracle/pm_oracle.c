@@ -1,8 +1,8 @@
 /*
  * pm_oracle.c -- CPU restatement of the reference PM / P3M force step.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library in
- * particlesimulation_amd/csrc: only tests/, __graft_entry__.smoke() and the
+ * TEST INFRASTRUCTURE ONLY.  The tests check the HIP library in
+ * particlesimulation_amd/csrc against this file: only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load it.  The product path never calls it.
  *
  * Every function names the reference file:line it restates (paths relative to
