@@ -531,20 +531,25 @@ def test_leapfrog_ops_match_reference_formulas(gpu_lib):
 
 
 # the run loop replays iterations 1-2, 3-4, ... from one captured graph: the
-# escape falls in the first or the second iteration of one replay
+# escape falls in the first or the second iteration of one replay; in a P3M
+# run the short-range kernel does the kick and the drift
+@pytest.mark.parametrize("p3m", [False, True])
 @pytest.mark.parametrize("x0,escape_at", [(30.5, 3), (30.1, 4)])
-def test_escape_stops_the_run(gpu_lib, x0, escape_at):
+def test_escape_stops_the_run(gpu_lib, x0, escape_at, p3m):
     grid = (32, 32, 32)
     pos, vel, m = _particles(500, grid, 70, margin=8.0)
     vel[:] = 0
     pos[7] = [x0, 16.0, 16.0]
     vel[7] = [0.4, 0.0, 0.0]  # crosses x = 32 during the drift of iteration escape_at
-    m[:] = 0.0
+    # no forces to speak of: the trajectory is the free drift
+    m[:] = 1e-12 if p3m else 0.0
     pm = _make(gpu_lib, grid, 1, 0, 0, box=grid)
     pm.set_particles(pos, vel, m)
     pm.init_greens()
+    if p3m:
+        pm.p3m_configure(cutoff=3.0, particle_diameter=3.0, cloud_shape=1)
     assert not pm.escaped()
-    done = pm.run(10, initial=True)
+    done = pm.run(10, initial=True, p3m=p3m)
     assert done == escape_at  # iterations before it are whole; its drift leaves the box
     x, v, _ = pm.get_particles()
     assert np.isclose(x[7, 0], x0 + (escape_at + 1) * 0.4, atol=1e-5)
