@@ -286,9 +286,11 @@ __global__ __launch_bounds__(DEP_THREADS) __attribute__((amdgpu_waves_per_eu(8))
     __syncthreads();
     if (s_nchunk == 1) {
       for (int i = threadIdx.x; i < TILE_CELLS; i += DEP_THREADS) {
-        int lx = i % TILE_X, ly = (i / TILE_X) % TILE_Y, lz = i / (TILE_X * TILE_Y);
-        float v = ldexpf((float)(long long)acc[i], -shift);
-        rho[(int64_t)(ox + lx) + (int64_t)nx * ((oy + ly) + (int64_t)ny * (oz + lz))] = v;
+        const int lx = i % TILE_X, ly = (i / TILE_X) % TILE_Y, lz = i / (TILE_X * TILE_Y);
+        // grids of sides up to 1024 have fewer than 2^31 cells, and their
+        // sides and plane sizes fit 24 bits: full-rate 24-bit multiplies
+        const int cell = (ox + lx) + (int)__umul24(nx, (oy + ly) + (int)__umul24(ny, oz + lz));
+        rho[cell] = ldexpf((float)(long long)acc[i], -shift);
       }
     } else {
       const int s = slot[t];
