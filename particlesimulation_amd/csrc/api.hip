@@ -691,7 +691,7 @@ int pmgpu_p3m_configure(pmgpu_ctx* c, const pmgpu_p3m_params* p) {
   c->sr_divy = c->sr_divz = c->sr_divx_base = c->sr_div;
   c->sr_xfine_max = 4;
   sr_cells_setup(c);
-  dalloc(&c->sr_table, SR_TABLE_LOG);  // the larger of the two tables
+  dalloc(&c->sr_table, SR_TABLE_EXT);  // the largest of the tables
   dalloc(&c->pair_count, 3);
   c->p3m_ready = true;
   if (c->n > 0) {

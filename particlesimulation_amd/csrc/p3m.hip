@@ -81,7 +81,7 @@ __global__ void k_sr_scatter(const float4* __restrict__ pm, const float4* __rest
                              const int32_t* __restrict__ key,
                              const int32_t* __restrict__ rank, const int32_t* __restrict__ start,
                              float4* __restrict__ sr_pm, int32_t* __restrict__ sr_idx, float pscale,
-                             float mscale, float m0, int nodd,
+                             float mscale, int clamp, float3 clo, float3 chi, float m0, int nodd,
                              const int32_t* __restrict__ odd_ids, int32_t* __restrict__ odd_pos,
                              const int32_t* __restrict__ escape, int iteration) {
   iteration = gate_iteration(escape, iteration);
@@ -89,7 +89,13 @@ __global__ void k_sr_scatter(const float4* __restrict__ pm, const float4* __rest
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int64_t d = (int64_t)start[key[i]] + rank[i];
-  const float4 p = pm[i];
+  float4 p = pm[i];
+  if (clamp) {
+    // for the whole table: positions in [clo, chi], which bounds every r^2
+    p.x = fminf(fmaxf(p.x, clo.x), chi.x);
+    p.y = fminf(fmaxf(p.y, clo.y), chi.y);
+    p.z = fminf(fmaxf(p.z, clo.z), chi.z);
+  }
   sr_pm[d] = make_float4(p.x * pscale, p.y * pscale, p.z * pscale, p.w * mscale);
   sr_idx[d] = (int32_t)i;
   if (nodd > 0 && __float_as_uint(p.w) != __float_as_uint(m0)) {
@@ -201,6 +207,10 @@ __device__ __forceinline__ float pair_factor(float r2, float mj, const float2* t
     const float sc = __builtin_amdgcn_fmed3f(r2, __uint_as_float(P.log_lo), __uint_as_float(log_hi_v));
     const float2 tv = table[__float_as_uint(sc) >> SR_LOG_SHIFT];
     return fmaf(tv.y, sc, tv.x) * mj;
+  } else if constexpr (MODE == 4) {
+    // the whole table: every r^2 the clamped positions allow has its entry
+    const float2 tv = table[__float_as_uint(r2) >> SR_LOG_SHIFT];
+    return fmaf(tv.y, r2, tv.x) * mj;
   } else {
     const float s = r2 * P.log_scale;
     // clamp in the float domain (s >= 0, where float order is bit order)
@@ -246,7 +256,7 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
   // The partial sums of half of the waves at a time.
   constexpr int HALF = SR_WAVES / 2;
   static_assert(SR_WAVES % 2 == 0, "the partial sums are added in halves");
-  constexpr int TS = MODE >= 2 ? SR_TABLE_LOG : (MODE == 1 ? SR_TABLE : 1);
+  constexpr int TS = MODE == 4 ? SR_TABLE_EXT : (MODE >= 2 ? SR_TABLE_LOG : (MODE == 1 ? SR_TABLE : 1));
   // One LDS block, the table last: the absolute layout (MODE 2) reads it
   // through a pointer SR_LOG_ABS0 entries before its start, which stays
   // inside the block, so the entry offset folds into the LDS address.
@@ -273,6 +283,8 @@ __global__ __launch_bounds__(SR_THREADS) void k_sr(
     for (int i = threadIdx.x; i < SR_TABLE; i += SR_THREADS) table[i] = table_g[i];
   if constexpr (MODE == 2)
     for (int i = threadIdx.x; i < SR_TABLE_LOG; i += SR_THREADS) table[i] = table_g[i];
+  if constexpr (MODE == 4)
+    for (int i = threadIdx.x; i < SR_TABLE_EXT; i += SR_THREADS) table[i] = table_g[i];
   if constexpr (MODE == 3) {
     const uint32_t k0 = P.log_lo >> SR_LOG_SHIFT;
     for (int i = threadIdx.x; i < SR_TABLE_LOG; i += SR_THREADS) table[(k0 + i) % SR_LOG_WRAP] = table_g[i];
@@ -678,16 +690,45 @@ static void sr_table_log_host(pmgpu_ctx* c, std::vector<float>& table) {
     table.resize(2 * SR_TABLE_LOG, 0.0f);
     return;
   }
+  c->sr_log_lo = lo - ((uint32_t)(2 * q) << 23);  // 2^-126
+  c->sr_log_hi = hi - ((uint32_t)(2 * q) << 23);
+  c->sr_pos_scale = std::ldexp(1.0f, -q);
+  c->sr_mass_scale = std::ldexp(1.0f, q + t);
+  // The whole table (MODE 4) when it fits: the short-range copy's positions
+  // are clamped to [-B/2, 3B/2] per axis (B the particle box), so r^2 (scaled)
+  // stays below (1 + 1e-5) sum (2 B s)^2, FP32 rounding of the sum included;
+  // every float up to that bound has its entry: below the first bin (r^2 <
+  // 2^-126, zero included) the first bin's value at its start, which the
+  // clamp gave, past the last bin zero.
+  double bound = 0.0;
+  bool boxed = true;
+  for (int i = 0; i < 3; ++i) {
+    boxed = boxed && c->p3m.box[i] > 0.0f;
+    const double side = 2.0 * (double)c->p3m.box[i] * (double)c->sr_pos_scale;
+    bound += side * side;
+  }
+  const float bound_f = (float)(bound * (1.0 + 1e-5));
+  uint32_t bound_bits;
+  std::memcpy(&bound_bits, &bound_f, sizeof bound_bits);
+  const int whole = (int)(bound_bits >> SR_LOG_SHIFT) + 2;
+  if (boxed && std::isfinite(bound_f) && whole <= SR_TABLE_EXT && SR_LOG_ABS0 + nbins < whole) {
+    table.assign(2 * SR_TABLE_EXT, 0.0f);
+    const float b0 = std::ldexp(bins[0], -t), a0 = std::ldexp(bins[1], 2 * q - t);
+    const float first = fmaf(a0, 0x1p-126f, b0);  // the first bin at its start, as the clamp gave it
+    for (int i = 0; i < SR_LOG_ABS0; ++i) table[2 * i] = first;
+    for (int i = 0; i < nbins; ++i) {
+      table[2 * (SR_LOG_ABS0 + i)] = std::ldexp(bins[2 * i], -t);
+      table[2 * (SR_LOG_ABS0 + i) + 1] = std::ldexp(bins[2 * i + 1], 2 * q - t);
+    }
+    c->sr_log_mode = 4;
+    return;
+  }
   table.assign(2 * SR_TABLE_LOG, 0.0f);
   for (int i = 0; i < nbins; ++i) {
     table[2 * i] = std::ldexp(bins[2 * i], -t);
     table[2 * i + 1] = std::ldexp(bins[2 * i + 1], 2 * q - t);
   }
   c->sr_log_mode = 2;
-  c->sr_log_lo = lo - ((uint32_t)(2 * q) << 23);  // 2^-126
-  c->sr_log_hi = hi - ((uint32_t)(2 * q) << 23);
-  c->sr_pos_scale = std::ldexp(1.0f, -q);
-  c->sr_mass_scale = std::ldexp(1.0f, q + t);
 }
 
 void p3m_setup(pmgpu_ctx* c) {
@@ -751,6 +792,9 @@ static void sr_sort(pmgpu_ctx* c, int iteration) {
   hipLaunchKernelGGL(k_sr_scatter, dim3(g), dim3(256), 0, c->stream, c->pm[c->cur], c->vel[c->cur], c->n,
                      c->sr_key,
                      c->sr_rank, c->sr_start, c->sr_pm, c->sr_idx, c->sr_pos_scale, c->sr_mass_scale,
+                     (int)(c->sr_log_mode == 4 && c->p3m.use_table == 2),
+                     make_float3(-0.5f * c->p3m.box[0], -0.5f * c->p3m.box[1], -0.5f * c->p3m.box[2]),
+                     make_float3(1.5f * c->p3m.box[0], 1.5f * c->p3m.box[1], 1.5f * c->p3m.box[2]),
                      c->sr_m0, c->sr_nodd, c->sr_odd_ids,
                      c->sr_odd_pos, c->escape, iteration);
   HIP_CHECK(hipGetLastError());
@@ -796,7 +840,8 @@ void p3m_pairs(pmgpu_ctx* c, int mode, float kick, int iteration) {
     case 0: launch_sr<0>(c, mode, kick, iteration); break;
     case 1: launch_sr<1>(c, mode, kick, iteration); break;
     default:
-      if (c->sr_log_mode == 2) launch_sr<2>(c, mode, kick, iteration);
+      if (c->sr_log_mode == 4) launch_sr<4>(c, mode, kick, iteration);
+      else if (c->sr_log_mode == 2) launch_sr<2>(c, mode, kick, iteration);
       else launch_sr<3>(c, mode, kick, iteration);
   }
   stage_end(c, tok);

@@ -67,10 +67,15 @@ constexpr int SR_TABLE_REF = 500;
 //     entry (bits(s) >> SR_LOG_SHIFT) - SR_LOG_ABS0, one shift of the bits
 //     with the constant folded into the LDS address;
 //   wrapped (otherwise): bin k sits at k mod SR_LOG_WRAP, a shift and a mask.
+// The absolute layout also comes whole (no clamp of s): entry bits(s) >>
+// SR_LOG_SHIFT for every s from 0 to the largest r^2 the short-range copy's
+// positions allow (clamped to half a box outside the particle box), when that
+// fits SR_TABLE_EXT entries.
 constexpr int SR_LOG_SHIFT = 16;
 constexpr int SR_LOG_WRAP = 4096;
 constexpr int SR_TABLE_LOG = SR_LOG_WRAP;
 constexpr int SR_LOG_ABS0 = (1 << 23) >> SR_LOG_SHIFT;
+constexpr int SR_TABLE_EXT = 6144;
 
 struct Stage {
   const char* name;
@@ -197,11 +202,12 @@ struct pmgpu_ctx {
   pmgpu_p3m_params p3m{};
   int mcx = 0, mcy = 0, mcz = 0, ncell_sr = 0;
   float hcx = 0, hcy = 0, hcz = 0;
-  float2* sr_table = nullptr;           // SR_TABLE_LOG entries (the larger of the two tables)
+  float2* sr_table = nullptr;           // SR_TABLE_EXT entries (the largest of the tables)
   float sr_inv_d2 = 0.0f;
   float sr_log_scale = 1.0f;             // the accurate table's argument s = r^2 * scale
   uint32_t sr_log_lo = 0, sr_log_hi = 0;  // float bits of its first and last (zero) bin
-  int sr_log_mode = 3;                    // 2: absolute layout over r^2 (scaled), 3: wrapped
+  int sr_log_mode = 3;                    // 2: absolute layout over r^2 (scaled), 3: wrapped,
+                                          // 4: absolute and whole (no clamp)
   // the pass's sorted copy holds positions x sr_pos_scale and masses x
   // sr_mass_scale (powers of two; 1 outside the absolute layout)
   float sr_pos_scale = 1.0f, sr_mass_scale = 1.0f;
