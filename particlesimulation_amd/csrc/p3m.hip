@@ -186,6 +186,13 @@ struct SrParams {
 //   MODE 3 puts the cutoff s = 1 on a bin boundary with one multiply (or
 //           runs over r^2 itself, scale 1); its LDS copy holds bin k at
 //           k mod SR_LOG_WRAP, a shift and a mask.
+//   MODE 4 is MODE 2 without the clamp: the short-range copy's positions are
+//           clamped to half a box outside the particle box, which bounds
+//           r^2, and the table holds an entry for every float from 0 to that
+//           bound (below the first bin the first bin's value at its start,
+//           past the last bin zero), at entry bits >> 16: one shift and the
+//           FMA (11 VALU instructions per pair).  Used when that is at most
+//           SR_TABLE_EXT entries (a particle box up to about 73 diameters).
 template <int MODE>
 __device__ __forceinline__ float pair_factor(float r2, float mj, const float2* table,
                                              const SrParams& P, uint32_t log_hi_v) {
@@ -649,6 +656,8 @@ static void sr_table_log_bins(float a, float eps, int cs, float scale, uint32_t 
 //    all scalings are powers of two.  Where the scaled masses could overflow
 //    (sum |m| 2^(q + t) >= 2^120) the same range stays unscaled and wrapped
 //    (MODE 3, scale 1).
+//    The scaled table comes whole (MODE 4) when every r^2 of the clamped
+//    positions fits SR_TABLE_EXT entries, else clamped (MODE 2).
 //  - Otherwise over r^2 / rc^2, the zero bin at s = 1 (the cutoff) and
 //    SR_TABLE_LOG - 1 bins below it, wrapped: bin k at k mod SR_LOG_WRAP
 //    (MODE 3).

@@ -253,11 +253,36 @@ def test_p3m_matches_oracle(gpu_lib, oracle, cs, use_table, eps, a):
     assert abs(pm.p3m_count_pairs() - pairs) <= 2 + 1e-5 * pairs
 
 
+def test_p3m_clamped_table_layout_matches_oracle(gpu_lib, oracle):
+    """The accurate table over r^2 comes whole (every r^2 of the clamped
+    positions has its entry, no clamp in the pair loop) while the particle
+    box is at most about 73 particle diameters (the table then spans at most
+    SR_TABLE_EXT entries); a 128-cell box with a = rc = 1.5 cells (85
+    diameters) takes the clamped absolute layout instead."""
+    grid = (128, 128, 128)
+    H = 0.25
+    pos, vel, m = _particles(6000, grid, 43, clustered=True)
+    rc = a = 1.5
+    pm = _make(gpu_lib, grid, 2, 0, 2, a=a)
+    pm.set_particles(pos, vel, m)
+    pm.init_greens()
+    pm.p3m_configure(cutoff=rc, particle_diameter=a, cloud_shape=1, use_table=2)
+    pm.p3m_record_short_range(True)
+    pm.pm_step()
+    pm.p3m_short_range()
+    a_sr = pm.p3m_short_range_accelerations()
+    ref_sr, pairs = oracle.p3m_short_range(pos, m, np.array(grid, np.float32) * H, rc * H, H, rc, a, 0.0, 1,
+                                           use_table=2)
+    assert _rel(a_sr, ref_sr) <= 10 * FORCE_TOL
+    assert abs(pm.p3m_count_pairs() - pairs) <= 2 + 1e-5 * pairs
+
+
 @pytest.mark.parametrize("common_mass", [False, True])
 def test_p3m_pair_table_layouts_agree_up_to_the_mass_scale(gpu_lib, oracle, common_mass):
     # The accurate table over r^2 is read in its absolute layout (positions
-    # scaled by 2^-q, masses by 2^(q+t)) unless the scaled masses could
-    # overflow; then the same bins are read wrapped and unscaled.  Masses
+    # scaled by 2^-q, masses by 2^(q+t); here the whole table, no clamp)
+    # unless the scaled masses could overflow; then the same bins are read
+    # wrapped and unscaled.  Masses
     # times 2^60 take the second layout.  Every product of the two differs by
     # a power of two, so the accelerations scale by 2^60 up to the order of
     # the sums (the cell sort's order within a cell varies from run to run)
