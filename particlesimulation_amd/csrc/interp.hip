@@ -192,14 +192,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       float sx = 0.f, sy = 0.f, sz = 0.f;
 #pragma unroll 1
       for (int a = 0; a < Gm::CNT; ++a) {
-        const float t1 = wx[a];
+        // the TSC's 1/8 once per x weight, not per tap (a power of two: the
+        // same products)
+        const float t1 = SCHEME == PMGPU_TSC ? wx[a] * 0.125f : wx[a];
 #pragma unroll
         for (int b = 0; b < Gm::CNT; ++b) {
           const float t2 = t1 * wy[b];
 #pragma unroll
           for (int c = 0; c < Gm::CNT; ++c) {
-            float t3 = t2 * wz[c];
-            if constexpr (SCHEME == PMGPU_TSC) t3 *= 0.125f;
+            const float t3 = t2 * wz[c];
             const int e = (lx + a) + Gm::EX * ((ly + b) + Gm::EY * (lz + c));
             sx += t3 * sE0[e];
             sy += t3 * sE1[e];
@@ -313,14 +314,15 @@ __global__ __launch_bounds__(256) void k_interp_gather(
   float sx = 0.f, sy = 0.f, sz = 0.f;
 #pragma unroll
   for (int a = 0; a < CNT; ++a) {
-    const float t1 = wx[a];
+    // the TSC's 1/8 once per x weight, not per tap (a power of two: the same
+    // products)
+    const float t1 = SCHEME == PMGPU_TSC ? wx[a] * 0.125f : wx[a];
 #pragma unroll
     for (int b = 0; b < CNT; ++b) {
       const float t2 = t1 * wy[b];
 #pragma unroll
       for (int c = 0; c < CNT; ++c) {
-        float t3 = t2 * wz[c];
-        if constexpr (SCHEME == PMGPU_TSC) t3 *= 0.125f;
+        const float t3 = t2 * wz[c];
         const float4 e = E[(uint32_t)gx[a] + rowoff[b][c]];
         sx += t3 * e.x;
         sy += t3 * e.y;
