@@ -157,21 +157,41 @@ def measured_traffic(workload):
 
 def measure(pm, w, steps, warmup, pos, vel, m):
     """Warm-up, then exactly `steps` run-loop iterations timed between two
-    stream synchronisations; per-stage times from HIP events on the library
-    stream, roofline of the dominant kernel.  Exits when the run escaped."""
+    stream synchronisations.  In a P3M run the short-range pair kernel alone
+    is bracketed by HIP events inside the timed region (its live launch time
+    for the roofline; the run loop then enqueues every iteration directly);
+    a PM run is timed without events, replaying the run loop's graph.  Then
+    as many iterations again with every kernel bracketed by events (the
+    per-stage times), and for P3M once more without events (the
+    graph-replayed rate).  Exits when the run escaped."""
     pm.set_particles(pos, vel, m)
     pm.run(warmup, initial=True, p3m=w.p3m)
     pairs0, tested0 = pm.p3m_count_pairs(tested=True) if w.p3m else (0, 0)
-    pm.profile(True)
-    pm.profile_reset()
-    pm.synchronize()
-    t0 = time.perf_counter()
-    # the run ends with a 4-byte read of the escape word (the first iteration
-    # whose drift left the box) and a stream synchronisation
-    done = pm.run(steps, initial=False, p3m=w.p3m, sync=True)
-    t1 = time.perf_counter()
-    stages = pm.profile_read()
-    pm.profile(False)
+
+    def timed(events):
+        # events: None, "short_range" or "all"
+        if events is None:
+            # four untimed iterations capture the graph that the timed run
+            # replays (switching events on or off retires the graphs; an even
+            # count ends on the buffer the capture did not start from, which
+            # is the one the timed run's graph starts from)
+            pm.run(4, initial=False, p3m=w.p3m, sync=True)
+        else:
+            pm.profile(True, short_range_only=events == "short_range")
+            pm.profile_reset()
+        pm.synchronize()
+        t0 = time.perf_counter()
+        # the run ends with a 4-byte read of the escape word (the first
+        # iteration whose drift left the box) and a stream synchronisation
+        done = pm.run(steps, initial=False, p3m=w.p3m, sync=True)
+        t1 = time.perf_counter()
+        stages = {}
+        if events is not None:
+            stages = pm.profile_read()
+            pm.profile(False)
+        return t1 - t0, done, stages
+
+    elapsed, done, sr_stage = timed("short_range" if w.p3m else None)
     escaped = pm.escaped()
     if escaped or done != steps:
         # a particle outside the box stops every later kernel of the run, so
@@ -179,32 +199,24 @@ def measure(pm, w, steps, warmup, pos, vel, m):
         log(f"[bench] invalid run of {w.name}: {done} of {steps} iterations done, escaped={escaped}")
         sys.exit(3)
     pairs1, tested1 = pm.p3m_count_pairs(tested=True) if w.p3m else (0, 0)
-    # the same number of iterations again with the stage events off: the run
-    # loop then replays two iterations at a time from a captured graph, and
-    # the events' own launches are gone (reported beside `value`, which keeps
-    # the timed region the stage events measured); four untimed iterations
-    # first capture the graph that the timed run replays (turning the events
-    # off retired the graphs; an even count ends on the buffer the capture
-    # did not start from, which is the one the timed run's graph starts from)
-    pm.run(4, initial=False, p3m=w.p3m, sync=True)
-    pm.synchronize()
-    t2 = time.perf_counter()
-    done2 = pm.run(steps, initial=False, p3m=w.p3m, sync=True)
-    t3 = time.perf_counter()
-    unprofiled = None
-    if not pm.escaped() and done2 == steps:
-        unprofiled = {"value": w.n * steps / (t3 - t2), "ms_per_step": (t3 - t2) / steps * 1e3}
-    log(f"[bench] {w.name} without stage events: {unprofiled}")
+    t_st, done_st, stages = timed("all")
+    with_stage_events = {"value": w.n * steps / t_st, "ms_per_step": t_st / steps * 1e3}
+    without_events = None
+    if w.p3m:
+        t_ne, done_ne, _ = timed(None)
+        if not pm.escaped() and done_ne == steps:
+            without_events = {"value": w.n * steps / t_ne, "ms_per_step": t_ne / steps * 1e3}
+    log(f"[bench] {w.name} with every stage bracketed by events: {with_stage_events}; "
+        f"without events: {without_events}")
     pairs = 0.5 * (pairs0 + pairs1)
     tested = 0.5 * (tested0 + tested1)
-    elapsed = t1 - t0
     value = w.n * steps / elapsed
     log(f"[bench] {w.name}: {steps} steps in {elapsed:.4f}s -> {value:.4e} upd/s; iterations={done} "
         f"escaped={escaped}")
     per_step = {k: v[0] / max(1, steps) for k, v in stages.items()}
     roofline = None
     if w.p3m:
-        ms, cnt = stages["sr_pairs"]
+        ms, cnt = sr_stage["sr_pairs"]
         avg_s = ms / cnt * 1e-3
         achieved = SR_FLOP_PER_PAIR * pairs / avg_s / 1e12
         roofline = {"bound": "valu", "kernel": "k_sr (short-range pairs)", "achieved": achieved,
@@ -227,9 +239,16 @@ def measure(pm, w, steps, warmup, pos, vel, m):
         if k in stages and stages[k][1]:
             s = stages[k][0] / stages[k][1] * 1e-3
             hbm[k] = {"ms": s * 1e3, "GB/s": b / s / 1e9, "frac": b / s / 1e9 / PEAK_HBM_GBS}
+    if not w.p3m and hbm:
+        # a PM step's longest kernel, from the run with every stage bracketed
+        # (the timed PM run carries no events)
+        k = max(hbm, key=lambda s: hbm[s]["ms"])
+        roofline = {"bound": "hbm", "kernel": k, "achieved": hbm[k]["GB/s"], "peak": PEAK_HBM_GBS,
+                    "unit": "GB/s", "frac": hbm[k]["frac"], "traffic": None, "avg_launch_ms": hbm[k]["ms"],
+                    "algorithmic_bytes": alg[k], "measured_in": "the run with every stage bracketed by events"}
     return {"value": value, "ms_per_step": elapsed / steps * 1e3, "steps": steps, "iterations": done,
             "escaped": escaped, "roofline": roofline, "stage_ms_per_step": per_step, "hbm_stages": hbm,
-            "pairs": pairs, "unprofiled": unprofiled}
+            "pairs": pairs, "with_stage_events": with_stage_events, "without_events": without_events}
 
 
 def cpu_baseline(w, pos, m, sr_scale_from=None):
@@ -373,7 +392,8 @@ def main():
         return {"config": workload_config(we, args.seed), "value": r["value"], "unit": "particle-updates/s",
                 "ms_per_step": r["ms_per_step"], "force_error": e_err,
                 "error_target_met": bool(e_err and e_err["e_rms"] <= 1e-3), "roofline": r["roofline"],
-                "stage_ms_per_step": r["stage_ms_per_step"], "cpu_baseline": c,
+                "stage_ms_per_step": r["stage_ms_per_step"], "without_events": r["without_events"],
+                "cpu_baseline": c,
                 "gpu_vs_cpu": (r["value"] / c["value"]) if c else None, "iterations": r["iterations"],
                 "escaped": r["escaped"]}
 
@@ -403,7 +423,8 @@ def main():
         "learned_vs_plain": learned,
         "stage_ms_per_step": res["stage_ms_per_step"],
         "hbm_stages": res["hbm_stages"],
-        "without_stage_events": res["unprofiled"],
+        "with_stage_events": res["with_stage_events"],
+        "without_events": res["without_events"],
         "gpu_vs_cpu": (res["value"] / cpu["value"]) if cpu else None,
         "iterations": res["iterations"],
         "escaped": res["escaped"],

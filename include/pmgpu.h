@@ -294,9 +294,12 @@ int pmgpu_set_tuning(pmgpu_ctx* ctx, const int32_t* knobs);
 int pmgpu_get_tuning(pmgpu_ctx* ctx, int32_t* knobs);
 
 /* ---- profiling -------------------------------------------------------- */
-/* When enabled, every kernel of the force step is bracketed by HIP events
-   on the context's stream; pmgpu_profile_read returns per-stage totals
-   (milliseconds) and launch counts.  Names are static strings. */
+/* on = 1: every kernel of the force step is bracketed by HIP events on the
+   context's stream; on = 2: only the short-range pair kernel ("sr_pairs"),
+   so that a timed run carries two event records per step; 0: off.  While
+   either is on, pmgpu_run enqueues every iteration directly (no graph
+   replay).  pmgpu_profile_read returns per-stage totals (milliseconds) and
+   launch counts.  Names are static strings. */
 int pmgpu_profile_enable(pmgpu_ctx* ctx, int32_t on);
 int pmgpu_profile_reset(pmgpu_ctx* ctx);
 int pmgpu_profile_read(pmgpu_ctx* ctx, int32_t max, const char** names, double* total_ms,

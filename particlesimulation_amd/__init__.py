@@ -387,8 +387,10 @@ class PM:
         return tuple(k)
 
     # -- profiling --------------------------------------------------------
-    def profile(self, on=True):
-        self._check(self._lib.pmgpu_profile_enable(self._ctx, int(bool(on))))
+    def profile(self, on=True, short_range_only=False):
+        """HIP events around every kernel of the step, or (short_range_only)
+        around the short-range pair kernel alone; off with on=False."""
+        self._check(self._lib.pmgpu_profile_enable(self._ctx, 0 if not on else (2 if short_range_only else 1)))
 
     def profile_reset(self):
         self._check(self._lib.pmgpu_profile_reset(self._ctx))

@@ -42,6 +42,7 @@ static void dfree(T** p) {
 
 int stage_begin(pmgpu_ctx* c, const char* name) {
   if (!c->profiling) return -1;
+  if (c->profiling == 2 && std::strcmp(name, "sr_pairs") != 0) return -1;
   int s = -1;
   for (size_t i = 0; i < c->stages.size(); ++i)
     if (std::strcmp(c->stages[i].name, name) == 0) s = (int)i;
@@ -955,7 +956,7 @@ int pmgpu_get_tuning(pmgpu_ctx* c, int32_t* knobs) {
 int pmgpu_profile_enable(pmgpu_ctx* c, int32_t on) {
   PMGPU_API_BEGIN(c)
   graphs_stale(c);
-  c->profiling = on != 0;
+  c->profiling = on == 2 ? 2 : (on != 0 ? 1 : 0);
   PMGPU_API_END(c)
 }
 

@@ -236,8 +236,8 @@ struct pmgpu_ctx {
   int64_t run_exec_gen[2] = {-1, -1};
   int64_t graph_gen = 0;
 
-  // profiling
-  bool profiling = false;
+  // profiling: 0 off, 1 every stage, 2 only the short-range pair kernel
+  int profiling = 0;
   std::vector<pmgpu::Stage> stages;
   std::vector<hipEvent_t> ev_pool;
   struct Pending {

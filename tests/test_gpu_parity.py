@@ -534,6 +534,30 @@ def test_graph_replayed_run_matches_the_direct_loop(gpu_lib, p3m):
         assert np.array_equal(x1, x0) and np.array_equal(v1, v0) and np.array_equal(a1, a0)
 
 
+def test_profile_levels_bracket_the_named_stages(gpu_lib):
+    """Level 1 times every stage; level 2 only the short-range pair kernel,
+    one launch per force evaluation of the run."""
+    grid = (64, 64, 64)
+    pos, vel, m = _particles(4000, grid, 52, margin=8.0, clustered=True)
+    pm = _make(gpu_lib, grid, 2, 0, 2, a=3.0)
+    pm.set_particles(pos, vel, m)
+    pm.init_greens()
+    pm.p3m_configure(cutoff=3.0, particle_diameter=3.0, cloud_shape=1)
+    assert pm.run(2, initial=True, p3m=True) == 2
+    seen = []
+    for short_range_only in (True, False):
+        pm.profile(True, short_range_only=short_range_only)
+        pm.profile_reset()
+        assert pm.run(5, initial=False, p3m=True) == 5
+        st = pm.profile_read()
+        pm.profile(False)
+        seen.append({k for k, v in st.items() if v[1]})
+        assert st["sr_pairs"][1] == 5 and st["sr_pairs"][0] > 0
+    assert seen[0] == {"sr_pairs"}
+    assert {"deposit", "interp", "sr_pairs", "sr_sort"} <= seen[1]
+    pm.close()
+
+
 def test_leapfrog_ops_match_reference_formulas(gpu_lib):
     grid = (32, 32, 32)
     pos, vel, m = _particles(1000, grid, 60)
