@@ -13,8 +13,9 @@ The line carries, for the headline workload (the 1M-particle P3M disk on
   roofline      the dominant kernel's achieved rate vs the MI355X peak,
                 measured with HIP events on the library's stream,
   cpu_baseline  the CPU restatement of the reference (oracle/, OpenMP) timed
-                on this host: one whole step (PM and the whole short-range
-                pass),
+                on this host: the whole PM step and a quarter of the
+                short-range pass (every 4th particle of each cell), scaled
+                to the whole pass by its pair count,
   force_error   RMS and mean relative error of the GPU force against an FP64
                 direct sum (host CPU) on a seeded subsample of targets,
 and two more measured blocks:
@@ -45,6 +46,7 @@ METRIC = "particle-updates/s, P3M 1M on 128^3 at fixed force err; learned vs pla
 PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 vector peak
 PEAK_HBM_GBS = 8000.0
 SR_FLOP_PER_PAIR = 20      # per ordered pair within the cutoff
+SR_SAMPLE_STRIDE = 4       # CPU baseline: every 4th particle of each cell as the home particle
 SCHEMES = ["NGP", "CIC", "TSC", "TSC_CENTRED"]
 FDS = ["TWO_POINT", "FOUR_POINT"]
 GREENS = ["DISCRETE_LAPLACIAN", "S1_OPTIMAL", "S2_OPTIMAL", "POOR_MAN"]
@@ -161,15 +163,22 @@ def measure(pm, w, steps, warmup, pos, vel, m):
     is bracketed by HIP events inside the timed region (its live launch time
     for the roofline; the run loop then enqueues every iteration directly);
     a PM run is timed without events, replaying the run loop's graph.  Then
-    as many iterations again with every kernel bracketed by events (the
+    the same iterations again with every kernel bracketed by events (the
     per-stage times), and for P3M once more without events (the
-    graph-replayed rate).  Exits when the run escaped."""
-    pm.set_particles(pos, vel, m)
-    pm.run(warmup, initial=True, p3m=w.p3m)
+    graph-replayed rate); each of the three starts from the initial
+    particles and the same warm-up.  Exits when the run escaped."""
+
+    def start():
+        pm.set_particles(pos, vel, m)
+        pm.run(warmup, initial=True, p3m=w.p3m)
+
+    start()
     pairs0, tested0 = pm.p3m_count_pairs(tested=True) if w.p3m else (0, 0)
 
-    def timed(events):
+    def timed(events, fresh=True):
         # events: None, "short_range" or "all"
+        if not fresh:
+            start()
         if events is None:
             # four untimed iterations capture the graph that the timed run
             # replays (switching events on or off retires the graphs; an even
@@ -199,11 +208,13 @@ def measure(pm, w, steps, warmup, pos, vel, m):
         log(f"[bench] invalid run of {w.name}: {done} of {steps} iterations done, escaped={escaped}")
         sys.exit(3)
     pairs1, tested1 = pm.p3m_count_pairs(tested=True) if w.p3m else (0, 0)
-    t_st, done_st, stages = timed("all")
-    with_stage_events = {"value": w.n * steps / t_st, "ms_per_step": t_st / steps * 1e3}
+    t_st, done_st, stages = timed("all", fresh=False)
+    with_stage_events = None
+    if not pm.escaped() and done_st == steps:
+        with_stage_events = {"value": w.n * steps / t_st, "ms_per_step": t_st / steps * 1e3}
     without_events = None
     if w.p3m:
-        t_ne, done_ne, _ = timed(None)
+        t_ne, done_ne, _ = timed(None, fresh=False)
         if not pm.escaped() and done_ne == steps:
             without_events = {"value": w.n * steps / t_ne, "ms_per_step": t_ne / steps * 1e3}
     log(f"[bench] {w.name} with every stage bracketed by events: {with_stage_events}; "
@@ -251,14 +262,21 @@ def measure(pm, w, steps, warmup, pos, vel, m):
             "pairs": pairs, "with_stage_events": with_stage_events, "without_events": without_events}
 
 
-def cpu_baseline(w, pos, m, sr_scale_from=None):
+def cpu_baseline(w, pos, m, sr_pairs=None, sr_scale_from=None):
     """The oracle (CPU restatement of the reference path, OpenMP) on this
-    host: one whole step, the PM force and the whole short-range pass (the
-    reference's chaining mesh with y-sorted cells and the accurate table).
-    With sr_scale_from = (seconds, pairs) of a measured pass on the same
-    particles, the short-range time is that pass's time per ordered pair
-    times this workload's pair count instead (the pass on a larger cutoff
-    would take minutes); the sample says so."""
+    host, on a bounded sample of one step: the whole PM force, and the
+    short-range pass (the reference's chaining mesh with y-sorted cells and
+    the accurate table) with every SR_SAMPLE_STRIDE-th particle of each
+    cell's list as the home particle.  The pass's time is its part that does
+    not grow with the pairs (the same pass with one home particle per cell:
+    cell lists, y-sort, the slot sums) plus the sample's time per ordered
+    pair times the whole step's pair count (sr_pairs, counted on the device
+    on the same particles; the sample's count times the stride without it).
+    Sampling within every cell keeps the pass's balance over the threads:
+    a sample of whole cells is bounded by the densest cell's single thread.
+    With sr_scale_from = (seconds, pairs, fixed seconds, this workload's
+    pairs) of that estimate on the same particles, the short-range time is
+    its time per pair times this workload's pair count instead."""
     from oracle import oracle as O
     threads, counts = host_threads()
     O.set_num_threads(threads)
@@ -269,26 +287,35 @@ def cpu_baseline(w, pos, m, sr_scale_from=None):
     t0 = time.perf_counter()
     O.pm_accel(pos, m, grid, w.interpolation, w.finite_diff, G, wrap=True)
     t_pm = time.perf_counter() - t0
-    t_sr, pairs, sample = 0.0, 0, f"one whole PM step ({t_pm:.2f}s)"
+    t_sr, t_fixed, pairs, extra = 0.0, 0.0, 0, {}
+    sample = f"one whole PM step ({t_pm:.2f}s)"
     if w.p3m and sr_scale_from is None:
         H = w.H
         box_orig = np.array([w.box] * 3, np.float32)
+        args = (pos, m, box_orig, w.rc_cells * H, H, w.rc_cells, w.a_cells, w.softening, w.cloud_shape)
         t0 = time.perf_counter()
-        _, pairs = O.p3m_short_range(pos, m, box_orig, w.rc_cells * H, H, w.rc_cells, w.a_cells, w.softening,
-                                     w.cloud_shape, use_table=w.sr_table, ysort=True)
-        t_sr = time.perf_counter() - t0
-        sample = (f"one whole step: PM ({t_pm:.2f}s) + the whole short-range pass ({pairs:.3e} ordered pairs, "
-                  f"{t_sr:.1f}s)")
+        O.p3m_short_range(*args, use_table=w.sr_table, ysort=True, sample_stride=1 << 30)
+        t_fixed = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        _, p_s = O.p3m_short_range(*args, use_table=w.sr_table, ysort=True, sample_stride=SR_SAMPLE_STRIDE)
+        t_s = time.perf_counter() - t0
+        pairs = sr_pairs if sr_pairs else p_s * SR_SAMPLE_STRIDE
+        t_sr = t_fixed + (t_s - t_fixed) * pairs / max(p_s, 1)
+        sample = (f"the whole PM step ({t_pm:.2f}s) + the short-range pass on every {SR_SAMPLE_STRIDE}th particle "
+                  f"of each chaining cell ({p_s:.3e} ordered pairs, {t_s:.1f}s) scaled to the step's {pairs:.3e} "
+                  f"pairs ({t_sr:.1f}s; {t_fixed:.2f}s of it the part with one home particle per cell)")
+        extra = {"sr_sample_stride": SR_SAMPLE_STRIDE, "sr_sample_pairs": p_s, "sr_sample_s": t_s,
+                 "sr_fixed_s": t_fixed}
     elif w.p3m:
-        t_ref, pairs_ref, pairs = sr_scale_from
-        t_sr = t_ref * pairs / pairs_ref
+        t_ref, pairs_ref, t_fixed, pairs = sr_scale_from
+        t_sr = t_fixed + (t_ref - t_fixed) * pairs / pairs_ref
         sample = (f"PM step measured ({t_pm:.2f}s) + short-range pass estimated as {pairs:.3e} ordered pairs at "
-                  f"the measured time per pair of the 3-cell pass on the same particles ({t_ref:.1f}s for "
-                  f"{pairs_ref:.3e} pairs)")
+                  f"the time per pair of the 3-cell pass on the same particles ({t_ref:.1f}s for "
+                  f"{pairs_ref:.3e} pairs, itself from its sample)")
     t_step = t_pm + t_sr
     return {"value": len(m) / t_step, "unit": "particle-updates/s", "cores": threads, "kind": "port",
             "sample": sample, "s_per_step": t_step, "pm_s": t_pm, "sr_s": t_sr, "sr_pairs": pairs,
-            "green_init_s": t_green, **counts}
+            "sr_fixed_s": t_fixed, "green_init_s": t_green, **extra, **counts}
 
 
 def learned_live(P, seed, budget_steps=150, snapshots=24, n_targets=65536):
@@ -369,13 +396,15 @@ def main():
     err = ds.error(force_of(pm, w)) if ds else None
     if err:
         log(f"[bench] force error vs direct sum: {err}")
+    # ordered pairs within the cutoff in the initial state (the CPU baseline's sample is scaled to it)
+    sr_pairs = pm.p3m_count_pairs() if w.p3m else None
     res = measure(pm, w, args.steps, args.warmup, pos, vel, m)
     pm.close()
 
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(w, pos, m)
+            cpu = cpu_baseline(w, pos, m, sr_pairs=sr_pairs)
             log(f"[bench] cpu baseline: {cpu}")
         except Exception as e:  # the baseline must not hide the GPU measurement
             log(f"[bench] cpu baseline failed: {e!r}")
@@ -388,7 +417,8 @@ def main():
         pe.close()
         c = None
         if cpu and cpu["sr_pairs"]:
-            c = cpu_baseline(we, pos, m, sr_scale_from=(cpu["sr_s"], cpu["sr_pairs"], r["pairs"]))
+            c = cpu_baseline(we, pos, m, sr_scale_from=(cpu["sr_s"], cpu["sr_pairs"], cpu["sr_fixed_s"],
+                                                        r["pairs"]))
         return {"config": workload_config(we, args.seed), "value": r["value"], "unit": "particle-updates/s",
                 "ms_per_step": r["ms_per_step"], "force_error": e_err,
                 "error_target_met": bool(e_err and e_err["e_rms"] <= 1e-3), "roofline": r["roofline"],

@@ -75,7 +75,7 @@ def lib():
         L.orc_sr_table_log.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, _f32p, C.POINTER(C.c_float),
                                        C.POINTER(C.c_uint32)]
         L.orc_p3m_sr.argtypes = [C.c_int64, _f32p, _f32p, _f32p, C.c_float, C.c_float, C.c_float, C.c_float,
-                                 C.c_float, C.c_int, C.c_int, C.c_int, C.c_void_p, _f32p,
+                                 C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, _f32p,
                                  C.POINTER(C.c_int64)]
         L.orc_p3m_mesh_dims.argtypes = [_f32p, C.c_float, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")]
         L.orc_direct_sum.argtypes = [C.c_int64, C.c_void_p, C.c_int64, _f32p, _f32p, C.c_double, C.c_double,
@@ -251,20 +251,19 @@ def p3m_mesh_dims(box_orig, rc_orig):
 
 
 def p3m_short_range(pos, mass, box_orig, rc_orig, H, rc, a, eps, cs, use_table=True, ysort=True,
-                    cell_mask=None):
+                    sample_stride=1):
     """P3MMethod::calculateShortRangeForces + correctAccelerations
     (source/p3mMethod.cpp:168-192, 50-57) -> (acc_sr (n,3) float32, ordered pairs).
-    use_table: 0/False direct formula, 1/True reference table, 2 accurate table."""
+    use_table: 0/False direct formula, 1/True reference table, 2 accurate table.
+    sample_stride > 1: every sample_stride-th particle of each cell's list as
+    the home particle only (partial results; for timing a bounded sample)."""
     pos = _pos(pos)
     mass = np.ascontiguousarray(mass, np.float32)
     acc = np.zeros(pos.size, np.float32)
     pairs = C.c_int64(0)
-    mask = None
-    if cell_mask is not None:
-        mask = np.ascontiguousarray(cell_mask, np.uint8)
     rc_ = lib().orc_p3m_sr(len(mass), pos.ravel(), mass, np.asarray(box_orig, np.float32), float(rc_orig),
                            float(H), float(rc), float(a), float(eps), cs, int(use_table), int(ysort),
-                           mask.ctypes.data if mask is not None else None, acc, C.byref(pairs))
+                           int(sample_stride), acc, C.byref(pairs))
     if rc_ != 0:
         raise ValueError(f"orc_p3m_sr failed ({rc_})")
     return acc.reshape(-1, 3), pairs.value

@@ -812,15 +812,16 @@ static int cmp_yid(const void* pa, const void* pb) {
  * loop runs on OpenMP threads exactly as the reference's std::thread pool.
  * use_table: 0 direct formula, 1 the reference's table, 2 the accurate table
  * of orc_sr_table_log (an extension, same chaining and accumulation).
- * acc_out (n x 3) receives (sum of slots + own) / m.  cell_mask (one byte per
- * chaining cell, NULL = all) restricts the pass to the marked cells; a
- * particle's result is then complete only if all 27 cells around it are
- * marked.  It exists to time a bounded sample of the pass.
+ * acc_out (n x 3) receives (sum of slots + own) / m.  sample_stride > 1 takes
+ * as home particles only every sample_stride-th entry of each cell's list
+ * (the results are then partial): a bounded sample of the pass whose work per
+ * cell, and so its balance over the threads, is the whole pass's divided by
+ * the stride.  It exists to time the pass on large inputs.
  * Returns the number of ordered pairs within the cutoff in *pairs_out.
  */
 int orc_p3m_sr(int64_t n, const float* pos, const float* mass, const float box_orig[3],
                float rc_orig, float H, float rc, float a, float eps, int cs, int use_table,
-               int ysort, const uint8_t* cell_mask, float* acc_out,
+               int ysort, int sample_stride, float* acc_out,
                int64_t* pairs_out) {
   chain_mesh cm;
   cm.mx = (int)(box_orig[0] / rc_orig);
@@ -837,14 +838,14 @@ int orc_p3m_sr(int64_t n, const float* pos, const float* mass, const float box_o
   int* list = (int*)malloc(sizeof(int) * (size_t)n);
   float* own = (float*)calloc(3 * (size_t)n, sizeof(float));
   float* slots = (float*)calloc(39 * (size_t)n, sizeof(float));
-  uint8_t* active = (uint8_t*)calloc((size_t)ncell, 1);
-  if (!cell_of || !count || !start || !list || !own || !slots || !active) return -3;
+  if (sample_stride < 1) return -2;
+  if (!cell_of || !count || !start || !list || !own || !slots) return -3;
   for (int64_t i = 0; i < n; ++i) {
     int cx = (int)(pos[3 * i] / cm.hcx), cy = (int)(pos[3 * i + 1] / cm.hcy),
         cz = (int)(pos[3 * i + 2] / cm.hcz);
     int c = cm_flat(&cm, cx, cy, cz);
     if (c < 0) { /* outside the chaining mesh: undefined behaviour in the reference */
-      free(cell_of); free(count); free(start); free(list); free(own); free(slots); free(active);
+      free(cell_of); free(count); free(start); free(list); free(own); free(slots);
       return -1;
     }
     cell_of[i] = c;
@@ -882,8 +883,6 @@ int orc_p3m_sr(int64_t n, const float* pos, const float* mass, const float box_o
       }
     }
   }
-  if (cell_mask) memcpy(active, cell_mask, (size_t)ncell);
-  else memset(active, 1, (size_t)ncell);
   float* table = (float*)malloc(sizeof(float) * 2 * (size_t)orc_sr_table_log_entries());
   float d2 = 0.0f, log_scale = 1.0f;
   uint32_t log_lo = 0;
@@ -894,13 +893,13 @@ int orc_p3m_sr(int64_t n, const float* pos, const float* mass, const float box_o
   int64_t pairs = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : pairs)
   for (int q = 0; q < ncell; ++q) {
-    if (!active[q] || count[q] == 0) continue;
+    if (count[q] == 0) continue;
     int nb[14];
     cm_neighbors(&cm, q, nb);
     for (int slot = 0; slot < 14; ++slot) {
       int qn = nb[slot];
       if (qn < 0) continue;
-      for (int ii = start[q]; ii < start[q + 1]; ++ii) {
+      for (int ii = start[q]; ii < start[q + 1]; ii += sample_stride) {
         int i = list[ii];
         for (int jj = start[qn]; jj < start[qn + 1]; ++jj) {
           int j = list[jj];
@@ -963,7 +962,7 @@ int orc_p3m_sr(int64_t n, const float* pos, const float* mass, const float box_o
     acc_out[3 * i + 2] = tz / mass[i];
   }
   if (pairs_out) *pairs_out = pairs;
-  free(cell_of); free(count); free(start); free(list); free(own); free(slots); free(active);
+  free(cell_of); free(count); free(start); free(list); free(own); free(slots);
   free(table);
   return 0;
 }

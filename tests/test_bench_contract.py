@@ -44,3 +44,31 @@ def test_traffic_file_matches_its_own_counters():
         d = json.load(f)
     bytes_ = d["fetch_kib_per_dispatch"] * 1024 * d["fetch_scale"] + d["write_kib_per_dispatch"] * 1024 * d["write_scale"]
     assert abs(bytes_ - d["traffic_bytes_per_launch"]) <= 1e-6 * bytes_
+
+
+def test_cpu_baseline_samples_every_cell_and_scales_by_pairs():
+    """The short-range sample takes every 4th particle of each cell's list as
+    the home particle: about a quarter of the pass's ordered pairs, scaled
+    back by the whole pass's pair count.  A stride of 1 is the whole pass."""
+    import dataclasses
+    import numpy as np
+    import bench
+    from oracle import oracle as O
+    from particlesimulation_amd import ics
+    w = dataclasses.replace(ics.C2_P3M_DISK, name="disk_small", n=20000, grid=(32, 32, 32), box=8.0)
+    pos, _, m = w.code_units(seed=3)
+    box = np.array([w.box] * 3, np.float32)
+    args = (pos, m, box, w.rc_cells * w.H, w.H, w.rc_cells, w.a_cells, w.softening, w.cloud_shape)
+    a_full, p_full = O.p3m_short_range(*args, use_table=2)
+    a_one, p_one = O.p3m_short_range(*args, use_table=2, sample_stride=1)
+    assert p_one == p_full and np.array_equal(a_one, a_full)
+    _, p_s = O.p3m_short_range(*args, use_table=2, sample_stride=4)
+    assert 0.22 < p_s / p_full < 0.28
+    # a stride beyond every cell's count keeps one home particle per cell
+    _, p_first = O.p3m_short_range(*args, use_table=2, sample_stride=1 << 30)
+    assert 0 < p_first < p_s
+    c = bench.cpu_baseline(w, pos, m, sr_pairs=p_full)
+    assert c["sr_pairs"] == p_full and c["sr_sample_stride"] == 4 and c["sr_sample_pairs"] == p_s
+    assert c["sr_s"] >= c["sr_fixed_s"] > 0 and c["s_per_step"] == c["pm_s"] + c["sr_s"]
+    assert c["kind"] == "port" and "every 4th particle" in c["sample"]
+    json.dumps(c)
