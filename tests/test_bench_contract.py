@@ -72,3 +72,87 @@ def test_cpu_baseline_samples_every_cell_and_scales_by_pairs():
     assert c["sr_s"] >= c["sr_fixed_s"] > 0 and c["s_per_step"] == c["pm_s"] + c["sr_s"]
     assert c["kind"] == "port" and "every 4th particle" in c["sample"]
     json.dumps(c)
+
+
+class _RecordingContext:
+    """Stands in for particlesimulation_amd.PM in bench.measure: records the
+    calls and answers with fixed stage times (no GPU)."""
+
+    def __init__(self, escape_on_run=None):
+        self.calls, self.level, self.runs, self.escape_on_run = [], 0, 0, escape_on_run
+
+    def set_particles(self, *a):
+        self.calls.append(("set_particles",))
+
+    def run(self, n, initial=False, p3m=False, sync=False):
+        self.runs += 1
+        self.calls.append(("run", n, bool(initial), self.level))
+        return n
+
+    def escaped(self):
+        return self.escape_on_run is not None and self.runs >= self.escape_on_run
+
+    def profile(self, on=True, short_range_only=False):
+        self.level = 0 if not on else (2 if short_range_only else 1)
+        self.calls.append(("profile", self.level))
+
+    def profile_reset(self):
+        pass
+
+    def profile_read(self):
+        if self.level == 2:
+            return {"sr_pairs": (50.0, 5)}
+        return {"sr_pairs": (50.0, 5), "deposit": (1.0, 5), "interp": (0.5, 5), "sr_sort": (2.0, 5)}
+
+    def synchronize(self):
+        pass
+
+    def p3m_count_pairs(self, tested=False):
+        return (1000, 1200) if tested else 1000
+
+    def get_tuning(self):
+        return (0, 0, 32, 6, 0, 0)
+
+    def p3m_cell_geometry(self):
+        return (48, 18, 1, 1000)
+
+
+def test_measure_times_the_value_with_short_range_events_only_and_restarts_each_region():
+    import bench
+    from particlesimulation_amd import ics
+    w = ics.C2_P3M_DISK
+    pm = _RecordingContext()
+    res = bench.measure(pm, w, 5, 2, None, None, None)
+    runs = [c for c in pm.calls if c[0] == "run"]
+    # three regions, each from the initial particles and the warm-up: the
+    # value's run with the short-range events only, every stage bracketed,
+    # then the graph's capture (4 untimed iterations) and a run with no events
+    assert [c for c in pm.calls if c[0] == "set_particles"] == [("set_particles",)] * 3
+    assert runs == [("run", 2, True, 0), ("run", 5, False, 2), ("run", 2, True, 0), ("run", 5, False, 1),
+                    ("run", 2, True, 0), ("run", 4, False, 0), ("run", 5, False, 0)]
+    assert res["roofline"]["avg_launch_ms"] == 10.0 and res["roofline"]["ordered_pairs"] == 1000
+    assert set(res["stage_ms_per_step"]) == {"sr_pairs", "deposit", "interp", "sr_sort"}
+    assert res["iterations"] == 5 and res["escaped"] is False
+    assert res["with_stage_events"] and res["without_events"]
+
+
+def test_measure_exits_without_a_line_when_the_timed_run_escaped():
+    import pytest
+    import bench
+    from particlesimulation_amd import ics
+    pm = _RecordingContext(escape_on_run=2)
+    with pytest.raises(SystemExit) as e:
+        bench.measure(pm, ics.C2_P3M_DISK, 5, 2, None, None, None)
+    assert e.value.code == 3
+
+
+def test_measure_times_a_pm_workload_on_the_graph_without_events():
+    import bench
+    from particlesimulation_amd import ics
+    pm = _RecordingContext()
+    res = bench.measure(pm, ics.C3_PM_CLUMPS, 5, 2, None, None, None)
+    runs = [c for c in pm.calls if c[0] == "run"]
+    assert runs == [("run", 2, True, 0), ("run", 4, False, 0), ("run", 5, False, 0), ("run", 2, True, 0),
+                    ("run", 5, False, 1)]
+    assert res["roofline"]["bound"] == "hbm" and res["roofline"]["kernel"] in ("deposit", "interp")
+    assert res["without_events"] is None
